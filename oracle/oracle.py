@@ -1,0 +1,217 @@
+"""ctypes wrapper of oracle/liboracle.so, the CPU restatement of the
+IroIro++ Wilson even/odd solve path.
+
+TEST INFRASTRUCTURE ONLY.  Importable from tests/, __graft_entry__.smoke()
+and the cpu_baseline leg of bench.py, where it is the checker.  The product
+(iroiro_amd) never imports it.
+
+Arrays are float64 numpy arrays in the reference layouts (Format_F spinors,
+Format_G links, even/odd half fields in esec/osec order); see
+oracle/wilson_oracle.h for the citations.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        c_int, c_long, c_double, c_u64 = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_uint64
+        vp = ctypes.c_void_p
+        sig = {
+            "or_geom_new": (vp, [_ip]),
+            "or_geom_free": (None, [vp]),
+            "or_geom_volume": (c_long, [vp]),
+            "or_geom_eo_tables": (None, [vp, _ip, _ip]),
+            "or_apbc": (None, [vp, _dp]),
+            "or_gamma5": (None, [c_long, _dp, _dp]),
+            "or_hop_eo": (None, [vp, _dp, _dp, _dp, c_int, c_int, c_double]),
+            "or_dirac_eo": (None, [vp, _dp, _dp, _dp, c_int, c_double]),
+            "or_ddagd": (None, [vp, _dp, _dp, _dp, c_double]),
+            "or_dirac_full": (None, [vp, _dp, _dp, _dp, c_int, c_double]),
+            "or_cg": (c_int, [vp, _dp, c_double, _dp, _dp, c_double, c_int,
+                              ctypes.POINTER(c_int), ctypes.POINTER(c_double)]),
+            "or_multishift_cg": (c_int, [vp, _dp, c_double, _dp, c_int, _dp, _dp, c_double,
+                                         c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_double),
+                                         ctypes.POINTER(c_double)]),
+            "or_eo_invert": (c_int, [vp, _dp, c_double, _dp, _dp, c_double, c_int,
+                                     ctypes.POINTER(c_int), ctypes.POINTER(c_double)]),
+            "or_true_residual_full": (c_double, [vp, _dp, c_double, _dp, _dp]),
+            "or_reunit": (None, [_dp]),
+            "or_gauge_expH": (None, [vp, c_double, c_u64, _dp]),
+            "or_gauss_field": (None, [c_long, c_u64, _dp]),
+            "or_plaquette": (c_double, [vp, _dp]),
+            "or_heatbath_sweep": (None, [vp, _dp, c_double, c_u64, c_int]),
+            "or_uniform": (c_double, [c_u64, c_u64, c_u64]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def kappa_from_mass(mass):
+    """Dirac_Wilson: kpp_ = 0.5/(4.0+mass) (lib/Dirac_ops/dirac_wilson.hpp:93)."""
+    return 0.5 / (4.0 + mass)
+
+
+class Lattice:
+    """Geometry of one lattice (lexicographic and even/odd tables)."""
+
+    def __init__(self, dims):
+        self.dims = tuple(int(d) for d in dims)
+        if self.dims[0] % 2:
+            raise ValueError("Lx must be even (siteIndex_EvenOdd.hpp:45)")
+        self._g = lib().or_geom_new(np.asarray(self.dims, dtype=np.int32))
+        self.V = int(np.prod(self.dims))
+        self.Vh = self.V // 2
+
+    def __del__(self):
+        g = getattr(self, "_g", None)
+        if g:
+            lib().or_geom_free(g)
+            self._g = None
+
+    # sizes in doubles
+    @property
+    def gauge_size(self):
+        return 18 * 4 * self.V
+
+    @property
+    def full_size(self):
+        return 24 * self.V
+
+    @property
+    def half_size(self):
+        return 24 * self.Vh
+
+    def eo_tables(self):
+        e = np.empty(self.Vh, np.int32)
+        o = np.empty(self.Vh, np.int32)
+        lib().or_geom_eo_tables(self._g, e, o)
+        return e, o
+
+    def split(self, full):
+        e, o = self.eo_tables()
+        f = full.reshape(self.V, 24)
+        return np.ascontiguousarray(f[e].ravel()), np.ascontiguousarray(f[o].ravel())
+
+    def merge(self, fe, fo):
+        e, o = self.eo_tables()
+        out = np.empty((self.V, 24))
+        out[e] = fe.reshape(self.Vh, 24)
+        out[o] = fo.reshape(self.Vh, 24)
+        return out.ravel()
+
+    # ---- operators
+    def apbc(self, u):
+        u = np.array(u, dtype=np.float64, copy=True)
+        lib().or_apbc(self._g, u)
+        return u
+
+    def hop_eo(self, u, f, eotype, dag, kappa):
+        w = np.empty(self.half_size)
+        lib().or_hop_eo(self._g, u, w, np.ascontiguousarray(f), int(eotype), int(dag), kappa)
+        return w
+
+    def dirac_eo(self, u, f, dag, kappa):
+        w = np.empty(self.half_size)
+        lib().or_dirac_eo(self._g, u, w, np.ascontiguousarray(f), int(dag), kappa)
+        return w
+
+    def ddagd(self, u, f, kappa):
+        w = np.empty(self.half_size)
+        lib().or_ddagd(self._g, u, w, np.ascontiguousarray(f), kappa)
+        return w
+
+    def dirac_full(self, u, f, dag, kappa):
+        w = np.empty(self.full_size)
+        lib().or_dirac_full(self._g, u, w, np.ascontiguousarray(f), int(dag), kappa)
+        return w
+
+    def cg(self, u, kappa, b, prec, maxiter):
+        x = np.empty(self.half_size)
+        it = ctypes.c_int()
+        diff = ctypes.c_double()
+        rc = lib().or_cg(self._g, u, kappa, x, np.ascontiguousarray(b), prec, maxiter,
+                         ctypes.byref(it), ctypes.byref(diff))
+        return x, {"converged": rc == 0, "iterations": it.value, "diff": diff.value}
+
+    def multishift_cg(self, u, kappa, shifts, b, prec, maxiter):
+        shifts = np.ascontiguousarray(shifts, dtype=np.float64)
+        x = np.empty(len(shifts) * self.half_size)
+        nconv = ctypes.c_int()
+        diff = ctypes.c_double()
+        odiff = ctypes.c_double()
+        rc = lib().or_multishift_cg(self._g, u, kappa, shifts, len(shifts), x,
+                                    np.ascontiguousarray(b), prec, maxiter, ctypes.byref(nconv),
+                                    ctypes.byref(diff), ctypes.byref(odiff))
+        return x.reshape(len(shifts), self.half_size), {
+            "converged": rc == 0, "nconv": nconv.value, "diff": diff.value,
+            "out_diff": odiff.value}
+
+    def eo_invert(self, u, kappa, src, prec, maxiter):
+        sol = np.empty(self.full_size)
+        it = ctypes.c_int()
+        diff = ctypes.c_double()
+        rc = lib().or_eo_invert(self._g, u, kappa, sol, np.ascontiguousarray(src), prec,
+                                maxiter, ctypes.byref(it), ctypes.byref(diff))
+        return sol, {"converged": rc == 0, "iterations": it.value, "diff": diff.value}
+
+    def true_residual_full(self, u, kappa, x, b):
+        return lib().or_true_residual_full(self._g, u, kappa, np.ascontiguousarray(x),
+                                           np.ascontiguousarray(b))
+
+    # ---- gauge fields and sources
+    def gauge_expH(self, eps, seed):
+        u = np.empty(self.gauge_size)
+        lib().or_gauge_expH(self._g, eps, seed, u)
+        return u
+
+    def plaquette(self, u):
+        return lib().or_plaquette(self._g, u)
+
+    def heatbath_sweep(self, u, beta, seed, sweep):
+        lib().or_heatbath_sweep(self._g, u, beta, seed, sweep)
+
+    def unit_gauge(self):
+        u = np.zeros((4 * self.V, 18))
+        u[:, 0] = u[:, 8] = u[:, 16] = 1.0
+        return u.ravel()
+
+
+def gamma5(f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    w = np.empty_like(f)
+    lib().or_gamma5(f.size // 24, w, f)
+    return w
+
+
+def gauss_field(n, seed):
+    f = np.empty(n)
+    lib().or_gauss_field(n, seed, f)
+    return f
+
+
+def uniform(seed, stream, counter):
+    return lib().or_uniform(seed, stream, counter)
+
+
+def reunit(m):
+    m = np.array(m, dtype=np.float64, copy=True)
+    lib().or_reunit(m)
+    return m

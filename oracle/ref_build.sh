@@ -1,0 +1,53 @@
+#!/bin/bash
+# Builds oracle/_ref/iroiro_ref: the reference IroIro++ Wilson solve path,
+# compiled from its own sources where they lie under the reference tree,
+# linked with oracle/ref_driver.cpp (a command-line driver).  Test
+# infrastructure only: used to write golden fixtures and to time the
+# reference CPU path.  Outputs go to oracle/_ref/ only.
+#
+# Flags follow the reference's configure defaults for a single-process,
+# non-BG/Q build with the improved Wilson kernel (configure.ac:88-95).
+# domainWallCore.cpp is listed only because lib/Fopr/fopr.h pulls in
+# Dirac_DomainWall, whose inline virtuals reference it.
+set -euo pipefail
+REF=${IROIRO_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+if [ ! -d "$REF/lib/Dirac_ops" ]; then
+  echo "ref_build: reference tree not found at $REF; skipping" >&2
+  exit 0
+fi
+mkdir -p "$OUT/obj"
+CXXFLAGS="-std=gnu++98 -O3 -fopenmp -w -D_GLIBCXX_USE_CXX11_ABI=0 -DIMPROVED_WILSON -DVERBOSITY=1"
+INC="-I$REF -I$REF/include -I$REF/lib"
+SRCS="
+lib/Dirac_ops/dirac_wilson.cpp
+lib/Dirac_ops/dirac.cpp
+lib/Dirac_ops/wilsonLikeUtils.cpp
+lib/Dirac_ops/eoUtils.cpp
+lib/Dirac_ops/domainWallCore.cpp
+lib/Solver/solver.cpp
+lib/Solver/solver_CG.cpp
+lib/Solver/multiShiftSolver_CG.cpp
+lib/Geometry/siteIndex.cpp
+lib/Geometry/siteIndex_EvenOdd.cpp
+lib/Geometry/siteMap.cpp
+lib/Communicator/communicator_single.cpp
+lib/Communicator/comm_io.cpp
+lib/Main/commonPrms.cpp
+lib/Tools/ErrorsHandling/base_errors.cpp
+lib/Tools/ErrorsHandling/convergence_errors.cpp
+lib/PugiXML/pugixml.cpp
+lib/PugiXML/pugi_interface.cpp
+"
+OBJS=""
+for s in $SRCS; do
+  o="$OUT/obj/$(echo "$s" | tr '/' '_' | sed 's/\.cpp$/.o/')"
+  if [ ! -f "$o" ] || [ "$REF/$s" -nt "$o" ]; then
+    g++ $CXXFLAGS $INC -c "$REF/$s" -o "$o"
+  fi
+  OBJS="$OBJS $o"
+done
+g++ $CXXFLAGS $INC -c "$HERE/ref_driver.cpp" -o "$OUT/obj/ref_driver.o"
+g++ -fopenmp -o "$OUT/iroiro_ref" "$OUT/obj/ref_driver.o" $OBJS
+echo "ref_build: built $OUT/iroiro_ref"

@@ -1,0 +1,265 @@
+// Command-line driver around the reference IroIro++ Wilson solve path.
+//
+// Test infrastructure only: this file is compiled together with the
+// reference's own translation units (see oracle/ref_build.sh) into
+// oracle/_ref/iroiro_ref.  It is used to write golden fixtures and to time
+// the reference CPU path; nothing in the product links or runs it.
+//
+// Every operator call below goes through the reference classes unchanged:
+//   Dirac_Wilson_EvenOdd<Dirac_Wilson>  lib/Dirac_ops/dirac_wilson_EvenOdd.hpp:12-92
+//   Dirac_Wilson (full index)           lib/Dirac_ops/dirac_wilson.hpp:118-134
+//   Fopr_DdagD                          lib/Fopr/fopr.h:146-153
+//   Solver_CG                           lib/Solver/solver_CG.cpp:29-160
+//   MultiShiftSolver_CG                 lib/Solver/multiShiftSolver_CG.cpp:10-185
+//   Inverter_WilsonLike::invert         lib/Dirac_ops/eoUtils.cpp:43-80
+//   AntiPeriodicBC<GaugeField>          lib/Dirac_ops/BoundaryConditions/antiPeriodicBC.hpp:25-37
+//
+// All binary files are raw little-endian float64 in the reference layouts
+// (Format_F spinors, Format_G links, even/odd half fields in esec/osec order).
+#include "include/commonPrms.hpp"
+#include "include/common_fields.hpp"
+#include "Communicator/communicator.hpp"
+#include "Geometry/siteIndex.hpp"
+#include "Geometry/siteIndex_EvenOdd.hpp"
+#include "Dirac_ops/dirac_wilson.hpp"
+#include "Dirac_ops/dirac_wilson_EvenOdd.hpp"
+#include "Dirac_ops/eoUtils.hpp"
+#include "Dirac_ops/BoundaryConditions/antiPeriodicBC.hpp"
+#include "Fopr/fopr.h"
+#include "Solver/solver_CG.hpp"
+#include "Solver/multiShiftSolver_CG.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cmath>
+#include <string>
+#include <vector>
+#include <sys/time.h>
+#include <unistd.h>
+
+namespace {
+
+void die(const char* msg) {
+  std::fprintf(stderr, "iroiro_ref: %s\n", msg);
+  std::exit(2);
+}
+
+Field read_field(const std::string& path, size_t n) {
+  Field f(n);
+  FILE* fp = std::fopen(path.c_str(), "rb");
+  if (!fp) die(("cannot open " + path).c_str());
+  size_t got = std::fread(f.getaddr(0), sizeof(double), n, fp);
+  std::fclose(fp);
+  if (got != n) die(("short read from " + path).c_str());
+  return f;
+}
+
+void write_field(const std::string& path, const Field& f) {
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  if (!fp) die(("cannot create " + path).c_str());
+  Field& g = const_cast<Field&>(f);
+  std::fwrite(g.getaddr(0), sizeof(double), f.size(), fp);
+  std::fclose(fp);
+}
+
+double now_ms() {
+  timeval tv;
+  gettimeofday(&tv, NULL);
+  return tv.tv_sec * 1000.0 + tv.tv_usec / 1000.0;
+}
+
+// Deterministic random numbers for timing runs (xorshift64), values in
+// [-1, 1).
+struct Filler {
+  unsigned long long s;
+  explicit Filler(unsigned long long seed) : s(seed) {}
+  double next() {
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    return (double)(s >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+  }
+};
+
+// Random SU(3) link: Gram-Schmidt on two random complex rows, third row
+// = conj(row0 x row1), so that the matrix is unitary with det 1.
+void random_su3(Filler& rng, double* m) {
+  double a[3][2], b[3][2];
+  for (int i = 0; i < 3; ++i) { a[i][0] = rng.next(); a[i][1] = rng.next(); }
+  for (int i = 0; i < 3; ++i) { b[i][0] = rng.next(); b[i][1] = rng.next(); }
+  double na = 0; for (int i = 0; i < 3; ++i) na += a[i][0]*a[i][0] + a[i][1]*a[i][1];
+  na = 1.0 / std::sqrt(na);
+  for (int i = 0; i < 3; ++i) { a[i][0] *= na; a[i][1] *= na; }
+  double pr = 0, pi = 0;  // <a,b>
+  for (int i = 0; i < 3; ++i) {
+    pr += a[i][0]*b[i][0] + a[i][1]*b[i][1];
+    pi += a[i][0]*b[i][1] - a[i][1]*b[i][0];
+  }
+  for (int i = 0; i < 3; ++i) {
+    b[i][0] -= pr*a[i][0] - pi*a[i][1];
+    b[i][1] -= pr*a[i][1] + pi*a[i][0];
+  }
+  double nb = 0; for (int i = 0; i < 3; ++i) nb += b[i][0]*b[i][0] + b[i][1]*b[i][1];
+  nb = 1.0 / std::sqrt(nb);
+  for (int i = 0; i < 3; ++i) { b[i][0] *= nb; b[i][1] *= nb; }
+  for (int j = 0; j < 3; ++j) {
+    m[2*j] = a[j][0]; m[2*j+1] = a[j][1];
+    m[6+2*j] = b[j][0]; m[6+2*j+1] = b[j][1];
+  }
+  for (int j = 0; j < 3; ++j) {
+    int k = (j+1)%3, l = (j+2)%3;
+    double cr = a[k][0]*b[l][0] - a[k][1]*b[l][1] - (a[l][0]*b[k][0] - a[l][1]*b[k][1]);
+    double ci = a[k][0]*b[l][1] + a[k][1]*b[l][0] - (a[l][0]*b[k][1] + a[l][1]*b[k][0]);
+    m[12+2*j] = cr; m[12+2*j+1] = -ci;
+  }
+}
+
+void setup_lattice(char** argv) {
+  Lattice latt;
+  latt.stdinput = false;
+  latt.Lx = std::atoi(argv[0]);
+  latt.Ly = std::atoi(argv[1]);
+  latt.Lz = std::atoi(argv[2]);
+  latt.Lt = std::atoi(argv[3]);
+  latt.NPEx = latt.NPEy = latt.NPEz = latt.NPEt = 1;
+  CommonPrms::instance(latt);
+  Communicator::instance();
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 7) {
+    std::fprintf(stderr,
+      "usage: iroiro_ref ops    Lx Ly Lz Lt mass gauge in_full outdir\n"
+      "       iroiro_ref apbc   Lx Ly Lz Lt gauge_in gauge_out\n"
+      "       iroiro_ref cg     Lx Ly Lz Lt mass gauge src_even prec maxiter outdir\n"
+      "       iroiro_ref mshift Lx Ly Lz Lt mass gauge src_even prec maxiter outdir s0 s1 ...\n"
+      "       iroiro_ref invert Lx Ly Lz Lt mass gauge src_full prec maxiter outdir\n"
+      "       iroiro_ref bench  Lx Ly Lz Lt mass nrep [cg_prec]\n");
+    return 2;
+  }
+  std::string cmd = argv[1];
+  setup_lattice(argv + 2);
+  const int V = CommonPrms::Nvol();
+  const int Vh = V / 2;
+  const size_t gsz = (size_t)18 * V * 4;
+  const size_t fsz = (size_t)24 * V;
+  const size_t hsz = (size_t)24 * Vh;
+  SiteIndex_EvenOdd* eo = SiteIndex_EvenOdd::instance();
+
+  if (cmd == "apbc") {
+    GaugeField U;
+    U.data = read_field(argv[6], gsz);
+    AntiPeriodicBC<GaugeField>(TDIR).apply_bc(U);
+    write_field(argv[7], U.data);
+    return 0;
+  }
+
+  if (cmd == "bench") {
+    // Times the reference EO hop (Doe.mult, one hop over V/2 sites) and a
+    // complete Solver_CG run on a random SU(3) field with a loose stopping
+    // precision, so that a bounded sample of CG iterations is measured.
+    double mass = std::atof(argv[6]);
+    int nrep = std::atoi(argv[7]);
+    double prec = argc > 8 ? std::atof(argv[8]) : 0.0;
+    Filler rng(0x9E3779B97F4A7C15ULL);
+    Field u(gsz), fe(hsz);
+    for (size_t l = 0; l < gsz / 18; ++l) random_su3(rng, u.getaddr(18 * l));
+    for (size_t i = 0; i < hsz; ++i) fe.set(i, rng.next());
+    Dirac_Wilson_EvenOdd<Dirac_Wilson> D(mass, &u);
+    Field w = D.mult_oe(fe);  // warm-up
+    double t0 = now_ms();
+    for (int r = 0; r < nrep; ++r) w = D.mult_oe(fe);
+    double hop_ms = (now_ms() - t0) / nrep;
+    double ddagd_ms = -1.0, cg_ms = -1.0;
+    int iters = -1;
+    if (prec > 0.0) {
+      Fopr_DdagD op(&D);
+      double t2 = now_ms();
+      w = op.mult(fe);
+      ddagd_ms = now_ms() - t2;
+      Solver_CG cg(prec, 100000, &op);
+      Field x(hsz);
+      SolverOutput so = cg.solve(x, fe);
+      cg_ms = so.timing;
+      iters = so.Iterations;
+    }
+    std::printf("IIREF_BENCH {\"hop_ms\": %.6f, \"nrep\": %d, \"Vh\": %d, "
+                "\"ddagd_ms\": %.6f, \"cg_ms\": %.6f, \"cg_iterations\": %d}\n",
+                hop_ms, nrep, Vh, ddagd_ms, cg_ms, iters);
+    return 0;
+  }
+
+  double mass = std::atof(argv[6]);
+  Field u = read_field(argv[7], gsz);
+  Dirac_Wilson_EvenOdd<Dirac_Wilson> D(mass, &u);
+
+  if (cmd == "ops") {
+    std::string out = argv[9];
+    Field ff = read_field(argv[8], fsz);
+    Format::Format_F fmt(V);
+    Field fe(ff[fmt.get_sub(eo->esec())]);
+    Field fo(ff[fmt.get_sub(eo->osec())]);
+    write_field(out + "/deo.bin", D.mult_eo(fo));
+    write_field(out + "/doe.bin", D.mult_oe(fe));
+    write_field(out + "/deo_dag.bin", D.mult_eo_dag(fe));
+    write_field(out + "/doe_dag.bin", D.mult_oe_dag(fo));
+    write_field(out + "/dhat.bin", D.mult(fe));
+    write_field(out + "/dhat_dag.bin", D.mult_dag(fe));
+    write_field(out + "/g5.bin", D.gamma5(fe));
+    Fopr_DdagD op(&D);
+    write_field(out + "/ddagd.bin", op.mult(fe));
+    Dirac_Wilson Df(mass, &u);
+    write_field(out + "/dfull.bin", Df.mult(ff));
+    write_field(out + "/dfull_dag.bin", Df.mult_dag(ff));
+    std::printf("IIREF_OK ops kappa=%.17g\n", D.getKappa());
+    return 0;
+  }
+
+  double prec = std::atof(argv[9]);
+  int maxiter = std::atoi(argv[10]);
+  std::string out = argv[11];
+  Fopr_DdagD op(&D);
+
+  if (cmd == "cg") {
+    Field b = read_field(argv[8], hsz);
+    Solver_CG cg(prec, maxiter, &op);
+    Field x(hsz);
+    SolverOutput so = cg.solve(x, b);
+    write_field(out + "/cg_x.bin", x);
+    std::printf("IIREF_CG {\"iterations\": %d, \"diff\": %.17g}\n",
+                so.Iterations, so.diff);
+    return 0;
+  }
+  if (cmd == "mshift") {
+    Field b = read_field(argv[8], hsz);
+    std::vector<double> shifts;
+    for (int i = 12; i < argc; ++i) shifts.push_back(std::atof(argv[i]));
+    MultiShiftSolver_CG ms(&op, prec, maxiter);
+    vector_Field xs(shifts.size());
+    for (size_t i = 0; i < xs.size(); ++i) xs[i].resize(hsz);
+    double diff = 0.0;
+    int nconv = -1;
+    SolverOutput so = ms.solve(xs, b, shifts, diff, nconv);
+    for (size_t i = 0; i < xs.size(); ++i) {
+      char name[64];
+      std::snprintf(name, sizeof(name), "/ms_x%zu.bin", i);
+      write_field(out + name, xs[i]);
+    }
+    std::printf("IIREF_MS {\"nconv\": %d, \"diff\": %.17g, \"out_diff\": %.17g}\n",
+                nconv, diff, so.diff);
+    return 0;
+  }
+  if (cmd == "invert") {
+    Field b = read_field(argv[8], fsz);
+    Solver_CG cg(prec, maxiter, &op);
+    EvenOddUtils::Inverter_WilsonLike inv(&D, &cg);
+    Field sol(fsz);
+    inv.invert(sol, b);
+    write_field(out + "/inv_x.bin", sol);
+    std::printf("IIREF_INV ok\n");
+    return 0;
+  }
+  die("unknown command");
+  return 2;
+}

@@ -1,0 +1,112 @@
+"""Writes the golden fixtures of the Wilson even/odd solve path.
+
+The expected outputs come from the reference IroIro++ itself, compiled from
+its own sources by oracle/ref_build.sh into oracle/_ref/iroiro_ref and run in
+this container (the reference tree is not available elsewhere).  Inputs are
+generated here:
+  * gauge field U = exp(i eps H) reunitarised (eps = 0.3), H traceless
+    Hermitian with unit-variance Gaussian entries, fixed seed; the reference
+    then applies its antiperiodic time boundary (AntiPeriodicBC, TDIR);
+  * a complex Gaussian spinor source with unit variance per real component.
+Reference calls made (see oracle/ref_driver.cpp): Deo/Doe mult and mult_dag,
+Dirac_Wilson_EvenOdd mult/mult_dag/gamma5, Fopr_DdagD, the full-index
+Dirac_Wilson mult/mult_dag, Solver_CG, MultiShiftSolver_CG and
+Inverter_WilsonLike::invert.
+
+Usage: python tests/golden/make_golden.py   (writes tests/golden/*.npz)
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import oracle  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "iroiro_ref")
+
+CASES = {
+    # config 1 of the solve path: 4^3 x 8, kappa = 0.12 (mass 1/6)
+    "wilson_c1_4x4x4x8": dict(dims=(4, 4, 4, 8), mass=1.0 / 6.0, eps=0.3, seed=20240611,
+                              src_seed=0x123, solvers=True),
+    # anisotropic lattice to pin the index arithmetic of every direction
+    "wilson_aniso_4x6x2x4": dict(dims=(4, 6, 2, 4), mass=0.05, eps=0.3, seed=77,
+                                 src_seed=0x234, solvers=False),
+}
+CG_PREC = 1e-20
+MS_SHIFTS = [0.05, 0.10, 0.20, 0.30]   # tests/test_MultiShiftSolver.cpp shifts
+
+
+def run(args):
+    out = subprocess.run([REF] + [str(a) for a in args], capture_output=True, text=True,
+                         check=True)
+    return out.stdout
+
+
+def parse(stdout, tag):
+    for line in stdout.splitlines():
+        if line.startswith(tag):
+            return json.loads(line[len(tag):].strip())
+    raise RuntimeError("no %s line in reference output" % tag)
+
+
+def make(name, dims, mass, eps, seed, src_seed, solvers):
+    lat = oracle.Lattice(dims)
+    u_raw = lat.gauge_expH(eps, seed)
+    src = oracle.gauss_field(lat.full_size, src_seed)
+    d = [str(x) for x in dims]
+    m = repr(float(mass))
+    res = {"dims": np.array(dims), "mass": mass, "eps": eps, "seed": seed,
+           "src_seed": src_seed, "src": src}
+    with tempfile.TemporaryDirectory() as tmp:
+        u_raw.tofile(os.path.join(tmp, "u_raw.bin"))
+        src.tofile(os.path.join(tmp, "src.bin"))
+        run(["apbc"] + d + [os.path.join(tmp, "u_raw.bin"), os.path.join(tmp, "u.bin")])
+        u = np.fromfile(os.path.join(tmp, "u.bin"))
+        res["u_apbc"] = u
+        out = run(["ops"] + d + [m, os.path.join(tmp, "u.bin"), os.path.join(tmp, "src.bin"), tmp])
+        res["kappa_line"] = out.strip().splitlines()[-1]
+        for k in ["deo", "doe", "deo_dag", "doe_dag", "dhat", "dhat_dag", "g5", "ddagd",
+                  "dfull", "dfull_dag"]:
+            res[k] = np.fromfile(os.path.join(tmp, k + ".bin"))
+        if solvers:
+            se, so = lat.split(src)
+            se.tofile(os.path.join(tmp, "se.bin"))
+            out = run(["cg"] + d + [m, os.path.join(tmp, "u.bin"), os.path.join(tmp, "se.bin"),
+                                    repr(CG_PREC), 1000, tmp])
+            info = parse(out, "IIREF_CG")
+            res["cg_x"] = np.fromfile(os.path.join(tmp, "cg_x.bin"))
+            res["cg_iterations"] = info["iterations"]
+            res["cg_diff"] = info["diff"]
+            res["cg_prec"] = CG_PREC
+            out = run(["mshift"] + d + [m, os.path.join(tmp, "u.bin"), os.path.join(tmp, "se.bin"),
+                                        repr(CG_PREC), 1000, tmp] + MS_SHIFTS)
+            info = parse(out, "IIREF_MS")
+            res["ms_shifts"] = np.array(MS_SHIFTS)
+            res["ms_x"] = np.stack([np.fromfile(os.path.join(tmp, "ms_x%d.bin" % i))
+                                    for i in range(len(MS_SHIFTS))])
+            res["ms_nconv"] = info["nconv"]
+            res["ms_diff"] = info["diff"]
+            res["ms_out_diff"] = info["out_diff"]
+            out = run(["invert"] + d + [m, os.path.join(tmp, "u.bin"), os.path.join(tmp, "src.bin"),
+                                        repr(CG_PREC), 1000, tmp])
+            res["inv_x"] = np.fromfile(os.path.join(tmp, "inv_x.bin"))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **res)
+    print("wrote", name, {k: (v.shape if hasattr(v, "shape") else v) for k, v in res.items()
+                          if k not in ("kappa_line",)})
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("oracle/_ref/iroiro_ref is not built (run oracle/ref_build.sh)")
+    for name, kw in CASES.items():
+        make(name, **kw)
+
+
+if __name__ == "__main__":
+    main()
