@@ -1,0 +1,222 @@
+"""Benchmark of the Wilson even/odd solve path on one MI355X.
+
+Workload (default): lattice 32^3 x 64, quenched SU(3) heat-bath gauge field
+at beta = 6.0 (200 sweeps from a cold start, fixed seed), kappa = 0.1355,
+antiperiodic time boundary, the 12 point-source right-hand sides of a
+propagator (delta at the origin, one per spin x colour).  One step = one
+12-rhs propagator solve to the FP64 true residual |b - D x|/|b| <= 1e-10 per
+rhs with the optimized method (batched multi-rhs CG, 16-bit storage with
+per-site norms and reconstruct-12 links, FP32 arithmetic, FP64 reliable
+updates), sources already resident in HBM.
+
+Reported beside it: the plain FP64 GPU CG (one rhs after the other, as the
+reference's Qprop_EvenOdd does) on the same problem, the FP64 even/odd hop
+(Dslash) bandwidth against the HBM roofline, and the reference CPU path
+(oracle/_ref, the compiled IroIro++) timed on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
+FP64_HOP_BYTES_PER_SITE = 1536  # 192 in + 8*144 links + 192 out
+HOP_FLOPS_PER_SITE = 1320
+
+CONFIGS = {
+    "c5": dict(dims=(32, 32, 32, 64), beta=6.0, kappa=0.1355, sweeps=200,
+               workload="wilson_eo_propagator_32c64_b6.0_k0.1355_12rhs"),
+    "c3": dict(dims=(24, 24, 24, 48), beta=5.8, kappa=0.1350, sweeps=200,
+               workload="wilson_eo_propagator_24c48_b5.8_k0.1350_12rhs"),
+    "c2": dict(dims=(16, 16, 16, 32), beta=5.8, kappa=0.1350, sweeps=200,
+               workload="wilson_eo_propagator_16c32_b5.8_k0.1350_12rhs"),
+    "tiny": dict(dims=(8, 8, 8, 16), beta=6.0, kappa=0.1355, sweeps=20,
+                 workload="wilson_eo_propagator_8c16_b6.0_k0.1355_12rhs"),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, n_iter_per_rhs, nrhs=12):
+    """Times the reference CPU path (compiled IroIro++, oracle/_ref) on this
+    host: one EO hop and a bounded Solver_CG sample on the same lattice; the
+    12-rhs solve time is extrapolated from the measured seconds per CG
+    iteration and the per-rhs iteration count of the FP64 GPU solve."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "iroiro_ref")
+    if not os.path.exists(ref):
+        return None
+    dims = [str(d) for d in cfg["dims"]]
+    mass = repr(0.5 / cfg["kappa"] - 4.0)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "8"))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    # a loose precision bounds the CG sample to a handful of iterations
+    cmd = [ref, "bench"] + dims + [mass, "2", "1e-3"]
+    t0 = time.time()
+    try:
+        out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600).stdout
+    except (subprocess.TimeoutExpired, OSError) as e:
+        log("cpu baseline failed:", e)
+        return None
+    wall = time.time() - t0
+    m = re.search(r"IIREF_BENCH (\{.*\})", out)
+    if not m:
+        return None
+    r = json.loads(m.group(1))
+    iters = r["cg_iterations"] + 1          # loop index at exit -> steps taken
+    # Solver_CG::solve = 1 initial A-apply + iters steps + 1 final A-apply
+    per_iter_ms = (r["cg_ms"] - 2 * r["ddagd_ms"]) / iters
+    per_iter_ms = max(per_iter_ms, r["ddagd_ms"])
+    value_s = per_iter_ms * n_iter_per_rhs * nrhs / 1000.0
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": value_s, "unit": "s per 12-rhs propagator solve (extrapolated)",
+        "cores": threads, "kind": "reference",
+        "sample": ("compiled IroIro++ (oracle/_ref) on %s, %d OpenMP threads (its Dslash is "
+                   "single-threaded): EO hop %.1f ms, Solver_CG %d iterations in %.1f s "
+                   "(%.1f s/iteration) on a random SU(3) field of the same lattice; "
+                   "x %d iterations/rhs (FP64 GPU count) x %d rhs; nproc=%d; sample wall %.1f s"
+                   % (model or "host CPU", threads, r["hop_ms"], iters, r["cg_ms"] / 1000.0,
+                      per_iter_ms / 1000.0, n_iter_per_rhs, nrhs, os.cpu_count() or 0, wall)),
+        "hop_ms": r["hop_ms"], "s_per_cg_iteration": per_iter_ms / 1000.0,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="half", choices=["single", "half", "bf16"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--seed", type=int, default=20260601)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (one MI355X per box); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import iroiro_amd as ii
+
+    cfg = CONFIGS[args.config]
+    dims = cfg["dims"]
+    kappa = cfg["kappa"]
+    ctx = ii.Context(dims)
+    name, hbm = ctx.device_info()
+    log("device", name, "%.0f GiB" % hbm)
+    t0 = time.time()
+    ctx.unit_gauge()
+    ctx.heatbath(cfg["beta"], args.seed, cfg["sweeps"])
+    plaq = ctx.plaquette()
+    ctx.apply_apbc()
+    log("heat-bath %d sweeps beta=%.2f: plaquette %.5f (%.1f s)"
+        % (cfg["sweeps"], cfg["beta"], plaq, time.time() - t0))
+
+    src = ii.DeviceField(ctx, "full", 12)
+    sol = ii.DeviceField(ctx, "full", 12)
+    src.point_sources((0, 0, 0, 0))
+
+    # ---- Dslash roofline (FP64 hop, one launch = V/2 output sites)
+    Vh = ctx.Vh
+    hop_ms = ctx.bench_hop("double", 1, 50)
+    hop_gbs = FP64_HOP_BYTES_PER_SITE * Vh / (hop_ms * 1e-3) / 1e9
+    hop_lp_ms = ctx.bench_hop(args.precision, 12, 20)
+    lp_bytes = (104 + 192 / 12.0) * Vh * 12  # 16-bit spinor+norm in/out, r12 links shared
+    log("FP64 hop %.3f ms = %.0f GB/s (%.1f%% of HBM peak); %s 12-rhs hop %.3f ms"
+        % (hop_ms, hop_gbs, 100 * hop_gbs / HBM_PEAK_GBS, args.precision, hop_lp_ms))
+
+    # ---- plain FP64 GPU CG, rhs after rhs (reference Qprop_EvenOdd shape)
+    fp64 = None
+    if not args.no_fp64:
+        s64 = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision="double", batched=False)
+        t0 = time.time()
+        outs = s64.solve(src, sol)
+        fp64_s = time.time() - t0
+        its = [o.Iterations for o in outs]
+        fp64 = {"s": fp64_s, "iterations_per_rhs": its,
+                "max_true_residual": max(o.true_residual for o in outs)}
+        log("FP64 CG 12 rhs: %.3f s, iterations %s, max true residual %.2e"
+            % (fp64_s, its, fp64["max_true_residual"]))
+
+    # ---- optimized solve: batched, mixed precision
+    solver = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=args.precision, batched=True)
+    for _ in range(args.warmup):
+        solver.solve(src, sol)
+    import ctypes  # noqa: F401
+    times, outs = [], None
+    t_all = time.time()
+    for _ in range(args.steps):
+        t0 = time.time()
+        outs = solver.solve(src, sol)
+        times.append(time.time() - t0)
+    t_all = time.time() - t_all
+    ms_per_step = 1000.0 * t_all / args.steps
+    max_res = max(o.true_residual for o in outs)
+    log("optimized 12-rhs solve: %.3f s/step (%s), iterations %s, reliable updates %s, "
+        "max true residual %.2e" % (ms_per_step / 1000, ["%.3f" % t for t in times],
+                                   [o.Iterations for o in outs], [o.restarts for o in outs],
+                                   max_res))
+
+    cpu = None
+    if not args.no_cpu and fp64 is not None:
+        n_it = int(round(sum(fp64["iterations_per_rhs"]) / 12.0))
+        cpu = cpu_baseline(cfg, n_it)
+        if cpu:
+            log("cpu baseline:", cpu["sample"])
+
+    result = {
+        "metric": "solve time @1e-10 true resid: mixed+batched+learned vs FP64 CG; Dslash GB/s",
+        "value": ms_per_step / 1000.0,
+        "unit": "s per 12-rhs propagator solve",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "%s storage / fp32 compute / fp64 reliable updates" % args.precision,
+        "data": "synthetic: SU(3) heat-bath gauge field generated on the device, point sources",
+        "config": {"workload": cfg["workload"], "lattice": list(dims), "beta": cfg["beta"],
+                   "kappa": kappa, "nrhs": 12, "tol": 1e-10, "method":
+                   "batched 12-rhs CG on Dhat^dag Dhat, %s storage, fp64 reliable updates"
+                   % args.precision, "plaquette": plaq, "device": name},
+        "max_true_residual": max_res,
+        "fp64_cg": fp64,
+        "speedup_vs_fp64_cg": (fp64["s"] / (ms_per_step / 1000.0)) if fp64 else None,
+        "roofline": {"bound": "hbm", "achieved": hop_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": hop_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_hop FP64 even/odd Wilson hop, V/2 = %d sites/launch, "
+                               "1536 B/site algorithmic" % Vh,
+                     "avg_ms": hop_ms,
+                     "gflops": HOP_FLOPS_PER_SITE * Vh / (hop_ms * 1e-3) / 1e9},
+        "roofline_batched_lowprec": {"kernel": "k_hop %s 12-rhs" % args.precision,
+                                     "avg_ms": hop_lp_ms,
+                                     "achieved": lp_bytes / (hop_lp_ms * 1e-3) / 1e9,
+                                     "unit": "GB/s", "bytes_per_site_per_rhs": 104 + 16},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,188 @@
+/*
+ * iroiro_hip.h -- C ABI of the MI355X (gfx950) implementation of the
+ * IroIro++ Wilson even/odd solve path.
+ *
+ * Plain C types only (pointers, sizes, doubles); no torch or HIP types.
+ * Host arrays use the reference layouts, all float64:
+ *   spinor   f[24*site + 2*(c + 3*s) + {re,im}]          include/format_F.h:36-42
+ *   link     u[18*(site + V*mu) + 2*(3*a + b) + {re,im}]  include/format_G.h:34-40
+ *   lexicographic site = x + Lx*(y + Ly*(z + Lz*t))       lib/Geometry/siteIndex.cpp:41
+ *   even/odd half fields ordered as esec/osec: ascending lexicographic
+ *   order of the sites with (x+y+z+t)%2 == 0 / 1     lib/Geometry/siteIndex_EvenOdd.cpp:21-62
+ * The gauge field is handed over with any boundary phase already applied
+ * (the reference negates U_t on the last time slice before building the
+ * operator, lib/Measurements/measGeneral.cpp:205-215).
+ *
+ * The Wilson parameter is kappa = 0.5/(4 + mass) (lib/Dirac_ops/dirac_wilson.hpp:93).
+ *
+ * Every entry point returns an ii_status; ii_last_error() gives the text of
+ * the last failure on a context.  A context owns one HIP stream and its
+ * device buffers; calls on one context are not re-entrant, calls on
+ * different contexts may run from different threads.
+ */
+#ifndef IROIRO_HIP_H
+#define IROIRO_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  II_OK = 0,
+  II_ERR_BAD_ARG = 1,        /* sizes, pointers or parameters invalid */
+  II_ERR_NOT_CONVERGED = 2,  /* Errors::ConvergenceErr / "Not converged." */
+  II_ERR_NAN = 3,            /* a NaN or Inf appeared in a solver scalar */
+  II_ERR_DEVICE = 4,         /* HIP runtime failure */
+  II_ERR_STATE = 5           /* e.g. no gauge field uploaded */
+} ii_status;
+
+typedef struct ii_context ii_context;
+typedef struct ii_field ii_field;
+
+/* Subsets of the lattice a field lives on. */
+enum { II_EVEN = 0, II_ODD = 1, II_FULL = 2 };
+
+/* Storage/compute precisions of the solvers. */
+enum {
+  II_PREC_DOUBLE = 0,   /* FP64 storage and arithmetic */
+  II_PREC_SINGLE = 1,   /* FP32 storage and arithmetic, reconstruct-12 links */
+  II_PREC_HALF = 2,     /* FP16 storage + per-site FP32 norm, r12 FP16 links, FP32 arithmetic */
+  II_PREC_BF16 = 3      /* BF16 storage + per-site FP32 norm, r12 BF16 links, FP32 arithmetic */
+};
+
+/* SolverOutput (lib/Solver/solver.hpp:16-22) plus what the device solvers
+ * add: true residual and timing split. */
+typedef struct {
+  double diff;            /* as the reference reports it (see each solver) */
+  int iterations;         /* Iterations / Nconv as the reference counts them */
+  double timing_ms;       /* wall time of the solve on the device stream */
+  double true_residual;   /* |b - D x|/|b| on the full operator (propagator solves) */
+  int restarts;           /* reliable updates / defect-correction cycles */
+  int inner_iterations;   /* low-precision iterations (mixed solvers) */
+} ii_solver_output;
+
+/* ---------------------------------------------------------------- context */
+/* Creates a context for a single-process lattice dims[4] = {Lx,Ly,Lz,Lt}
+ * on HIP device `device`.  Lx must be even (siteIndex_EvenOdd.hpp:45). */
+ii_status ii_context_create(ii_context** ctx, const int dims[4], int device);
+ii_status ii_context_destroy(ii_context* ctx);
+const char* ii_last_error(const ii_context* ctx);
+/* Device name and HBM size, for reports. */
+ii_status ii_device_info(ii_context* ctx, char* name, size_t name_len,
+                         double* hbm_gib);
+
+/* ------------------------------------------------------------ gauge field */
+/* Uploads and repacks the gauge field (Format_G, full lexicographic
+ * volume, 4 directions): the device copy behind Dirac::getGaugeField_ptr
+ * (lib/Dirac_ops/dirac.hpp:25).  Any low-precision copies are rebuilt
+ * lazily. */
+ii_status ii_gauge_upload(ii_context* ctx, const double* u);
+/* Dirac::update_internal_state() (lib/Dirac_ops/dirac.hpp:28): the caller
+ * changed its gauge field in place; the device copy is dropped and must be
+ * uploaded again before the next operator call. */
+ii_status ii_gauge_invalidate(ii_context* ctx);
+/* Average plaquette of the uploaded field (Re tr P / 3). */
+ii_status ii_gauge_plaquette(ii_context* ctx, double* plaq);
+/* Downloads the device gauge field back into the host layout. */
+ii_status ii_gauge_download(ii_context* ctx, double* u);
+
+/* Cold start U = 1 on the device (GaugeConf_unit, lib/Main/gaugeConf.cpp:211-228). */
+ii_status ii_gauge_unit(ii_context* ctx);
+/* nsweeps SU(3) Cabibbo-Marinari / Kennedy-Pendleton heat-bath sweeps of
+ * the Wilson plaquette action at coupling beta on the device field,
+ * numbered first_sweep, first_sweep+1, ... for the counter-based RNG keyed
+ * by seed.  No reference counterpart: its heat-bath is SU(2) only
+ * (lib/UpdatingAlg/HeatBath/HBActions_SUN.cpp:85-92). */
+ii_status ii_gauge_heatbath(ii_context* ctx, double beta, unsigned long long seed,
+                            int first_sweep, int nsweeps);
+/* AntiPeriodicBC<GaugeField>(TDIR)::apply_bc on the device field
+ * (lib/Dirac_ops/BoundaryConditions/antiPeriodicBC.hpp:25-37). */
+ii_status ii_gauge_apply_apbc(ii_context* ctx);
+
+/* ---------------------------------------------- operators on host arrays */
+/* Even/odd hops, w = -kappa H f restricted to parities
+ * (Dirac_Wilson EO constructors, dirac_wilson.hpp:89-114,
+ *  dirac_wilson.cpp:28-36, dirac_wilson_EvenOdd.hpp:59-77):
+ *   mult_eo      = Deo.mult        odd  -> even
+ *   mult_oe      = Doe.mult        even -> odd
+ *   mult_eo_dag  = Doe.mult_dag    even -> odd
+ *   mult_oe_dag  = Deo.mult_dag    odd  -> even                          */
+ii_status ii_dirac_mult_eo(ii_context* ctx, double kappa, double* out, const double* in);
+ii_status ii_dirac_mult_oe(ii_context* ctx, double kappa, double* out, const double* in);
+ii_status ii_dirac_mult_eo_dag(ii_context* ctx, double kappa, double* out, const double* in);
+ii_status ii_dirac_mult_oe_dag(ii_context* ctx, double kappa, double* out, const double* in);
+/* Dirac_Wilson_EvenOdd::mult / mult_dag: f - Deo Doe f and its adjoint
+ * (dirac_wilson_EvenOdd.hpp:79-92), on even half fields. */
+ii_status ii_dirac_mult(ii_context* ctx, double kappa, double* out, const double* in);
+ii_status ii_dirac_mult_dag(ii_context* ctx, double kappa, double* out, const double* in);
+/* Dirac_Wilson::gamma5 (dirac_wilson.cpp:120-126) on nsites sites. */
+ii_status ii_dirac_gamma5(ii_context* ctx, double* out, const double* in, long nsites);
+/* Fopr_DdagD::mult (lib/Fopr/fopr.h:146-153). */
+ii_status ii_fopr_ddagd_mult(ii_context* ctx, double kappa, double* out, const double* in);
+/* Full-index Dirac_Wilson::mult / mult_dag (dirac_wilson.cpp:28-36,
+ * mult_full :115-118) on full lexicographic fields. */
+ii_status ii_dirac_full_mult(ii_context* ctx, double kappa, double* out, const double* in,
+                             int dag);
+
+/* ------------------------------------------------- solvers on host arrays */
+/* Solver_CG::solve on Fopr_DdagD(Dirac_Wilson_EvenOdd) (solver_CG.cpp:29-77):
+ * x0 = b; stops when |r|^2/|b|^2 < prec (squared); iterations = loop index
+ * at exit; diff = |A x - b|/|b|.  Returns II_ERR_NOT_CONVERGED (x still
+ * written) when maxiter is reached. */
+ii_status ii_solver_cg(ii_context* ctx, double kappa, double* x, const double* b,
+                       double prec, int maxiter, ii_solver_output* out);
+/* MultiShiftSolver_CG::solve on Fopr_DdagD + sigma_i
+ * (multiShiftSolver_CG.cpp:95-185).  x: nshift half fields back to back.
+ * out->diff = max_i |(A+sigma_i)x_i - b|^2/|b|^2 (the reference's `diff`),
+ * out->true_residual = sqrt of it (its Out.diff); iterations = Nconv. */
+ii_status ii_multishift_cg(ii_context* ctx, double kappa, double* x, const double* b,
+                           const double* shifts, int nshift, double prec, int maxiter,
+                           ii_solver_output* out);
+/* Inverter_WilsonLike::invert with a Solver_CG (eoUtils.cpp:43-80): full
+ * lexicographic solve of D x = b. */
+ii_status ii_eo_invert(ii_context* ctx, double kappa, double* sol, const double* src,
+                       double prec, int maxiter, ii_solver_output* out);
+
+/* ------------------------------------------------------ device fields */
+/* A device-resident field (subset II_EVEN/II_ODD/II_FULL) holding nrhs
+ * right-hand sides in the device layout.  Upload/download convert from/to
+ * the host layout of one rhs. */
+ii_status ii_field_create(ii_context* ctx, int subset, int nrhs, ii_field** f);
+ii_status ii_field_destroy(ii_field* f);
+ii_status ii_field_upload(ii_field* f, int rhs, const double* host);
+ii_status ii_field_download(const ii_field* f, int rhs, double* host);
+/* Point source of Source_local::mksrc (source_types.hpp:57-62) for every
+ * (spin s, colour c) into rhs 3*s+c of a 12-rhs full field (column order of
+ * Qprop_EvenOdd::calc, qprop_EvenOdd.cpp:11-23). */
+ii_status ii_field_point_sources(ii_field* f, const int site[4]);
+
+/* Propagator solve parameters. */
+typedef struct {
+  double tol;            /* target |b - D x|/|b| per rhs on the full operator */
+  int maxiter;           /* bound on total inner iterations per solve */
+  int precision;         /* II_PREC_* of the inner solver */
+  int batched;           /* nonzero: all rhs in one fused multi-rhs solve */
+  double delta;          /* reliable-update trigger (residual drop factor), 0 = default */
+  int use_graph;         /* capture solver iterations in a HIP graph */
+  int use_learned;       /* apply the trained learned preconditioner (flexible outer) */
+} ii_prop_params;
+
+/* Full solve D x = b for every rhs of a full field through the even/odd
+ * Schur system (Inverter_WilsonLike::invert, eoUtils.cpp:43-80) with a
+ * device CG on Dhat^dag Dhat; stops on the FP64 true full-D residual.
+ * Sources and solutions stay on the device.  out: one entry per rhs. */
+ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_params* p,
+                              const ii_field* src, ii_field* sol, ii_solver_output* out);
+
+/* ------------------------------------------------------ measurement */
+/* Times one even/odd hop (Doe, V/2 output sites) in the given precision
+ * with HIP events on the context stream, over nrep launches after a
+ * warm-up; nrhs > 1 times the fused multi-rhs hop.  avg_ms per launch. */
+ii_status ii_bench_hop(ii_context* ctx, int precision, int nrhs, int nrep, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,458 @@
+"""iroiro_amd: the IroIro++ Wilson even/odd solve path on MI355X (gfx950).
+
+Python mirror of the reference's operator interfaces for this path, backed
+by the C ABI of libiroiro_hip.so (include/iroiro_hip.h):
+
+  Dirac_Wilson_EvenOdd   lib/Dirac_ops/dirac_wilson_EvenOdd.hpp:12-92
+  Dirac_Wilson           lib/Dirac_ops/dirac_wilson.hpp:19-175 (full index)
+  Fopr_DdagD             lib/Fopr/fopr.h:146-153
+  Solver_CG              lib/Solver/solver_CG.cpp:29-160
+  MultiShiftSolver_CG    lib/Solver/multiShiftSolver_CG.cpp:10-185
+  Inverter_WilsonLike    lib/Dirac_ops/eoUtils.cpp:43-80
+  Qprop_EvenOdd          lib/Measurements/FermionicM/qprop_EvenOdd.cpp:11-43
+  Source_local           lib/Measurements/FermionicM/source_types.hpp:23-62
+
+Fields are float64 numpy arrays in the reference layouts (Format_F
+spinors, Format_G links; even/odd half fields in esec/osec order).  All
+arithmetic runs on the GPU; importing works without a GPU, but creating a
+Context needs the HIP library and a device.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ConvergenceErr, IIError, NotConverged, II_EVEN, II_ODD, II_FULL,
+                   II_PREC_DOUBLE, II_PREC_SINGLE, II_PREC_HALF, II_PREC_BF16, PRECISIONS)
+
+__all__ = [
+    "Context", "Dirac_Wilson_EvenOdd", "Dirac_Wilson", "Fopr_DdagD", "Solver_CG",
+    "MultiShiftSolver_CG", "Inverter_WilsonLike", "Qprop_EvenOdd", "Source_local",
+    "SolverOutput", "DeviceField", "PropagatorSolver", "ConvergenceErr", "NotConverged",
+    "IIError", "kappa_from_mass", "esec_osec",
+]
+
+
+def kappa_from_mass(mass):
+    """kpp_ = 0.5/(4.0+mass) (lib/Dirac_ops/dirac_wilson.hpp:93)."""
+    return 0.5 / (4.0 + mass)
+
+
+def esec_osec(dims):
+    """Lexicographic sites of the even and odd half lattices in ascending
+    order (SiteIndex_EvenOdd::setup, lib/Geometry/siteIndex_EvenOdd.cpp:21-62)."""
+    Lx, Ly, Lz, Lt = dims
+    s = np.arange(Lx * Ly * Lz * Lt)
+    x = s % Lx
+    y = (s // Lx) % Ly
+    z = (s // (Lx * Ly)) % Lz
+    t = s // (Lx * Ly * Lz)
+    par = (x + y + z + t) % 2
+    return s[par == 0], s[par == 1]
+
+
+class SolverOutput:
+    """SolverOutput (lib/Solver/solver.hpp:16-22) plus device extras."""
+
+    def __init__(self, c):
+        self.diff = c.diff
+        self.Iterations = c.iterations
+        self.timing = c.timing_ms
+        self.true_residual = c.true_residual
+        self.restarts = c.restarts
+        self.inner_iterations = c.inner_iterations
+        self.Msg = ""
+
+    def __repr__(self):
+        return ("SolverOutput(diff=%.3e, Iterations=%d, timing=%.3f ms, true_residual=%.3e, "
+                "restarts=%d)" % (self.diff, self.Iterations, self.timing, self.true_residual,
+                                  self.restarts))
+
+
+class Context:
+    """One lattice on one GPU: the device side of CommonPrms + Communicator
+    for a single process (include/commonPrms.hpp:28-79)."""
+
+    def __init__(self, dims, device=0):
+        L = _lib.load()
+        self.dims = tuple(int(d) for d in dims)
+        if self.dims[0] % 2:
+            raise ValueError("Lx must be even (lib/Geometry/siteIndex_EvenOdd.hpp:45)")
+        h = ctypes.c_void_p()
+        d = (ctypes.c_int * 4)(*self.dims)
+        st = L.ii_context_create(ctypes.byref(h), d, int(device))
+        if st != 0:
+            raise IIError(st, "ii_context_create failed (is a GPU visible?)")
+        self._h = h
+        self.V = int(np.prod(self.dims))
+        self.Vh = self.V // 2
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().ii_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st, what):
+        _lib.check(st, self._h, what)
+
+    def upload_gauge(self, u):
+        u = _lib.as_f64(u, 18 * 4 * self.V)
+        self.check(_lib.load().ii_gauge_upload(self._h, _lib.dptr(u)), "ii_gauge_upload")
+
+    def invalidate_gauge(self):
+        self.check(_lib.load().ii_gauge_invalidate(self._h), "ii_gauge_invalidate")
+
+    def download_gauge(self):
+        u = np.empty(18 * 4 * self.V)
+        self.check(_lib.load().ii_gauge_download(self._h, _lib.dptr(u)), "ii_gauge_download")
+        return u
+
+    def unit_gauge(self):
+        """Cold start U = 1 on the device."""
+        self.check(_lib.load().ii_gauge_unit(self._h), "ii_gauge_unit")
+
+    def heatbath(self, beta, seed, nsweeps, first_sweep=0):
+        """SU(3) heat-bath sweeps on the device gauge field."""
+        self.check(_lib.load().ii_gauge_heatbath(self._h, float(beta), int(seed),
+                                                 int(first_sweep), int(nsweeps)),
+                   "ii_gauge_heatbath")
+
+    def apply_apbc(self):
+        """Antiperiodic time boundary on the device gauge field."""
+        self.check(_lib.load().ii_gauge_apply_apbc(self._h), "ii_gauge_apply_apbc")
+
+    def plaquette(self):
+        p = ctypes.c_double()
+        self.check(_lib.load().ii_gauge_plaquette(self._h, ctypes.byref(p)), "ii_gauge_plaquette")
+        return p.value
+
+    def device_info(self):
+        buf = ctypes.create_string_buffer(256)
+        g = ctypes.c_double()
+        self.check(_lib.load().ii_device_info(self._h, buf, 256, ctypes.byref(g)), "ii_device_info")
+        return buf.value.decode(), g.value
+
+    def bench_hop(self, precision="double", nrhs=1, nrep=50):
+        ms = ctypes.c_double()
+        self.check(_lib.load().ii_bench_hop(self._h, PRECISIONS[precision], nrhs, nrep,
+                                            ctypes.byref(ms)), "ii_bench_hop")
+        return ms.value
+
+
+class _OpBase:
+    def __init__(self, mass, u, ctx):
+        self.ctx = ctx
+        self.mass = float(mass)
+        self.kpp = kappa_from_mass(self.mass)
+        self.u = u
+        self.update_internal_state()
+
+    def getKappa(self):
+        return self.kpp
+
+    def getMass(self):
+        return 0.5 / self.kpp - 4.0
+
+    def getGaugeField_ptr(self):
+        return self.u
+
+    def gsize(self):
+        return 18 * 4 * self.ctx.V
+
+    def update_internal_state(self):
+        """Dirac::update_internal_state (lib/Dirac_ops/dirac.hpp:28): the
+        caller changed the gauge field; upload it again."""
+        self.ctx.upload_gauge(self.u)
+
+    def _call(self, name, f, nout, *extra):
+        f = _lib.as_f64(f)
+        out = np.empty(nout)
+        fn = getattr(_lib.load(), name)
+        self.ctx.check(fn(self.ctx.handle, self.kpp, _lib.dptr(out), _lib.dptr(f), *extra), name)
+        return out
+
+
+class Dirac_Wilson_EvenOdd(_OpBase):
+    """Dirac_Wilson_EvenOdd<Dirac_Wilson> (dirac_wilson_EvenOdd.hpp:12-92):
+    the Schur operator Dhat = 1 - Deo Doe on even sites."""
+
+    def __init__(self, mass, u, ctx):
+        super().__init__(mass, u, ctx)
+
+    def fsize(self):
+        return 24 * self.ctx.Vh
+
+    def mult(self, f):
+        return self._call("ii_dirac_mult", _lib.as_f64(f, self.fsize()), self.fsize())
+
+    def mult_dag(self, f):
+        return self._call("ii_dirac_mult_dag", _lib.as_f64(f, self.fsize()), self.fsize())
+
+    def mult_eo(self, f):
+        """Deo.mult: odd -> even (:59-62)."""
+        return self._call("ii_dirac_mult_eo", _lib.as_f64(f, self.fsize()), self.fsize())
+
+    def mult_oe(self, f):
+        """Doe.mult: even -> odd (:64-67)."""
+        return self._call("ii_dirac_mult_oe", _lib.as_f64(f, self.fsize()), self.fsize())
+
+    def mult_eo_dag(self, f):
+        """Doe.mult_dag: even -> odd (:69-72)."""
+        return self._call("ii_dirac_mult_eo_dag", _lib.as_f64(f, self.fsize()), self.fsize())
+
+    def mult_oe_dag(self, f):
+        """Deo.mult_dag: odd -> even (:74-77)."""
+        return self._call("ii_dirac_mult_oe_dag", _lib.as_f64(f, self.fsize()), self.fsize())
+
+    def mult_ee(self, f):
+        return np.array(f, dtype=np.float64)
+
+    mult_oo = mult_ee_inv = mult_oo_inv = mult_ee
+
+    def gamma5(self, f):
+        f = _lib.as_f64(f)
+        out = np.empty_like(f)
+        self.ctx.check(_lib.load().ii_dirac_gamma5(self.ctx.handle, _lib.dptr(out), _lib.dptr(f),
+                                                   f.size // 24), "ii_dirac_gamma5")
+        return out
+
+
+class Dirac_Wilson(_OpBase):
+    """Full-index Dirac_Wilson (dirac_wilson.hpp:118-134): D = 1 - kappa H."""
+
+    def fsize(self):
+        return 24 * self.ctx.V
+
+    def mult(self, f):
+        return self._call("ii_dirac_full_mult", _lib.as_f64(f, self.fsize()), self.fsize(), 0)
+
+    def mult_dag(self, f):
+        return self._call("ii_dirac_full_mult", _lib.as_f64(f, self.fsize()), self.fsize(), 1)
+
+    def gamma5(self, f):
+        f = _lib.as_f64(f)
+        out = np.empty_like(f)
+        self.ctx.check(_lib.load().ii_dirac_gamma5(self.ctx.handle, _lib.dptr(out), _lib.dptr(f),
+                                                   f.size // 24), "ii_dirac_gamma5")
+        return out
+
+
+class Fopr_DdagD:
+    """Fopr_DdagD (lib/Fopr/fopr.h:146-153): mult = D^dag D, Hermitian."""
+
+    def __init__(self, D):
+        assert D is not None
+        self.D = D
+
+    def func(self, x):
+        return x
+
+    def fsize(self):
+        return self.D.fsize()
+
+    def mult(self, f):
+        if isinstance(self.D, Dirac_Wilson_EvenOdd):
+            return self.D._call("ii_fopr_ddagd_mult", _lib.as_f64(f, self.fsize()), self.fsize())
+        return self.D.mult_dag(self.D.mult(f))
+
+    def mult_dag(self, f):
+        return self.mult(f)
+
+
+class Solver_CG:
+    """Solver_CG (lib/Solver/solver_CG.cpp:29-160), the whole iteration on
+    the device.  Stops when |r|^2/|b|^2 < prec (note: squared), starting from
+    x0 = b.  Raises ConvergenceErr where the reference exits."""
+
+    def __init__(self, prec, maxiter, fopr):
+        self.prec = float(prec)
+        self.MaxIter = int(maxiter)
+        self.opr = fopr
+
+    def check_DdagD(self):
+        return isinstance(self.opr, Fopr_DdagD)
+
+    def solve(self, sol, src):
+        D = self.opr.D
+        if not isinstance(D, Dirac_Wilson_EvenOdd):
+            raise TypeError("Solver_CG on the device supports Fopr_DdagD(Dirac_Wilson_EvenOdd)")
+        b = _lib.as_f64(src, D.fsize())
+        x = np.empty(D.fsize())
+        out = _lib.SolverOutputC()
+        st = _lib.load().ii_solver_cg(D.ctx.handle, D.kpp, _lib.dptr(x), _lib.dptr(b), self.prec,
+                                      self.MaxIter, ctypes.byref(out))
+        sol[...] = x
+        D.ctx.check(st, "Solver_CG")
+        so = SolverOutput(out)
+        so.Msg = "CG solver"
+        return so
+
+
+class MultiShiftSolver_CG:
+    """MultiShiftSolver_CG (lib/Solver/multiShiftSolver_CG.cpp:10-185)."""
+
+    def __init__(self, fopr, stop_cond, niter):
+        self.opr = fopr
+        self.prec = float(stop_cond)
+        self.MaxIter = int(niter)
+
+    def solve(self, solution, source, shifts):
+        """Fills solution[i] for every shift; returns (SolverOutput, diff,
+        Nconv) with the reference's meanings (diff = max squared residual,
+        SolverOutput.diff = its square root)."""
+        D = self.opr.D
+        b = _lib.as_f64(source, D.fsize())
+        sh = _lib.as_f64(shifts)
+        x = np.empty(len(sh) * D.fsize())
+        out = _lib.SolverOutputC()
+        st = _lib.load().ii_multishift_cg(D.ctx.handle, D.kpp, _lib.dptr(x), _lib.dptr(b),
+                                          _lib.dptr(sh), len(sh), self.prec, self.MaxIter,
+                                          ctypes.byref(out))
+        xs = x.reshape(len(sh), D.fsize())
+        for i in range(len(sh)):
+            solution[i][...] = xs[i]
+        if st == _lib.II_ERR_NOT_CONVERGED:
+            raise NotConverged(st, "Not converged.")
+        D.ctx.check(st, "MultiShiftSolver_CG")
+        so = SolverOutput(out)
+        so.Msg = "Multishift CG solver"
+        so.diff = out.true_residual
+        return so, out.diff, out.iterations
+
+
+class Inverter_WilsonLike:
+    """EvenOddUtils::Inverter_WilsonLike (lib/Dirac_ops/eoUtils.cpp:43-80)."""
+
+    def __init__(self, D, solver):
+        self.D = D
+        self.slv = solver
+
+    def invert(self, sol, src):
+        D = self.D
+        b = _lib.as_f64(src, 24 * D.ctx.V)
+        x = np.empty(24 * D.ctx.V)
+        out = _lib.SolverOutputC()
+        st = _lib.load().ii_eo_invert(D.ctx.handle, D.kpp, _lib.dptr(x), _lib.dptr(b),
+                                      self.slv.prec, self.slv.MaxIter, ctypes.byref(out))
+        sol[...] = x
+        D.ctx.check(st, "Inverter_WilsonLike::invert")
+        return SolverOutput(out)
+
+
+class Source_local:
+    """Point source (source_types.hpp:23-62): 1.0 in the real part of
+    (colour c, spin s) at one site."""
+
+    def __init__(self, pos, dims):
+        self.pos = tuple(int(p) for p in pos)
+        self.dims = tuple(dims)
+        Lx, Ly, Lz, _ = self.dims
+        x, y, z, t = self.pos
+        self.loc = x + Lx * (y + Ly * (z + Lz * t))
+        self.V = int(np.prod(self.dims))
+
+    def mksrc(self, s, c):
+        f = np.zeros(24 * self.V)
+        f[24 * self.loc + 2 * (c + 3 * s)] = 1.0
+        return f
+
+
+class Qprop_EvenOdd:
+    """Qprop_EvenOdd::calc (qprop_EvenOdd.cpp:11-23): 12 inversions, column
+    3*s + c."""
+
+    def __init__(self, D, solver):
+        self.inv = Inverter_WilsonLike(D, solver)
+
+    def calc(self, src):
+        xq = []
+        for s in range(4):
+            for c in range(3):
+                sol = np.empty(24 * self.inv.D.ctx.V)
+                self.inv.invert(sol, src.mksrc(s, c))
+                xq.append(sol)
+        return xq
+
+
+class DeviceField:
+    """A device-resident field (ii_field): nrhs right-hand sides of one
+    subset ('even', 'odd' or 'full')."""
+
+    _SUB = {"even": II_EVEN, "odd": II_ODD, "full": II_FULL}
+
+    def __init__(self, ctx, subset="full", nrhs=1):
+        self.ctx = ctx
+        self.subset = subset
+        self.nrhs = int(nrhs)
+        h = ctypes.c_void_p()
+        ctx.check(_lib.load().ii_field_create(ctx.handle, self._SUB[subset], self.nrhs,
+                                              ctypes.byref(h)), "ii_field_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def size(self):
+        return 24 * (self.ctx.V if self.subset == "full" else self.ctx.Vh)
+
+    def upload(self, rhs, host):
+        a = _lib.as_f64(host, self.size)
+        self.ctx.check(_lib.load().ii_field_upload(self._h, int(rhs), _lib.dptr(a)),
+                       "ii_field_upload")
+
+    def download(self, rhs):
+        a = np.empty(self.size)
+        self.ctx.check(_lib.load().ii_field_download(self._h, int(rhs), _lib.dptr(a)),
+                       "ii_field_download")
+        return a
+
+    def point_sources(self, site):
+        d = (ctypes.c_int * 4)(*site)
+        self.ctx.check(_lib.load().ii_field_point_sources(self._h, d), "ii_field_point_sources")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().ii_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PropagatorSolver:
+    """The production solve of many right-hand sides (e.g. the 12 columns of
+    a point-source propagator) through the even/odd Schur system, entirely
+    on the device, stopping on the FP64 true residual |b - D x|/|b| per rhs.
+
+    precision: 'double' | 'single' | 'half' | 'bf16' (inner storage);
+    batched: all rhs in one fused multi-rhs solve."""
+
+    def __init__(self, ctx, kappa, tol=1e-10, precision="double", batched=False, maxiter=100000,
+                 delta=0.0, use_graph=False, use_learned=False):
+        self.ctx = ctx
+        self.kappa = float(kappa)
+        self.params = _lib.PropParamsC(tol, maxiter, PRECISIONS[precision], int(batched),
+                                       float(delta), int(use_graph), int(use_learned))
+
+    def solve(self, src, sol):
+        outs = (_lib.SolverOutputC * src.nrhs)()
+        st = _lib.load().ii_propagator_solve(self.ctx.handle, self.kappa, ctypes.byref(self.params),
+                                             src.handle, sol.handle, outs)
+        res = [SolverOutput(o) for o in outs]
+        self.ctx.check(st, "ii_propagator_solve")
+        return res

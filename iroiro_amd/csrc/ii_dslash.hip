@@ -1,0 +1,157 @@
+// Even/odd Wilson hop kernels for gfx950 and their host dispatch.
+//
+// k_hop computes, for every half site of parity out_par and every rhs,
+//   out = a * [g5] H [g5] in  (+ x)          (XPAY adds the own-parity field x)
+// and optionally the per-block partial sums of Re<d, out> (DOT) for the CG
+// scalar <p, A p>.  With a = -kappa this is Dirac_Wilson::mult for the EO
+// constructors (dirac_wilson.cpp:28-32, 107-113); with g5 folded on both
+// sides it is mult_dag (:34-36); with XPAY and a = -kappa^2 on the second
+// hop it is Dirac_Wilson_EvenOdd::mult/mult_dag
+// (dirac_wilson_EvenOdd.hpp:79-92).
+#include "ii_ops.h"
+#include "ii_stencil.h"
+
+namespace ii {
+
+template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Gauge,
+          typename StOut, typename SpX>
+__global__ void __launch_bounds__(kBlock)
+k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
+      typename SpIn::R a, bool tbc_flip, double* __restrict__ partials,
+      const int* __restrict__ skip, int skip_stride) {
+  using C = typename SpIn::C;
+  const int nblk = gridDim.x;
+  const int blk = xcd_remap(blockIdx.x, nblk);
+  const int hs = blk * blockDim.x + threadIdx.x;
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs * skip_stride]) return;  // uniform per block
+  double dsum = 0.0;
+  if (hs < g.Vh) {
+    const SpIn inr = in.rhs(rhs);
+    C acc[4][3];
+    wilson_hop_site<G5IN>(acc, g, inr, gauge, hs, out_par, tbc_flip);
+    C o[12];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int so = G5OUT ? (s ^ 2) : s;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[3 * so + c] = cscale(acc[s][c], a);
+    }
+    if (XPAY) {
+      const SpX xr = x.rhs(rhs);
+      const auto xs = xr.scale(hs);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        auto xv = xr.ld(hs, k);
+        o[k].x += (typename SpIn::R)(xv.x * xs);
+        o[k].y += (typename SpIn::R)(xv.y * xs);
+      }
+    }
+    if (DOT) {
+      const SpX dr = dv.rhs(rhs);
+      const double ds = (double)dr.scale(hs);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        auto dvv = dr.ld(hs, k);
+        dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
+      }
+    }
+    out.rhs(rhs).st(hs, o);
+  }
+  if (DOT) {
+    double s = block_sum(dsum);
+    if (threadIdx.x == 0) partials[(size_t)rhs * nblk + blockIdx.x] = s;
+  }
+}
+
+namespace {
+
+template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Gauge,
+          typename StOut, typename SpX>
+ii_status launch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, const StOut& out,
+                     const SpX& x, const SpX& dv, int out_par, typename SpIn::R a, int nrhs,
+                     double* partials, const int* skip, int skip_stride) {
+  const Geom& g = ctx->g;
+  dim3 grid(blocks_for(g.Vh), nrhs);
+  bool flip = ctx->tbc_sign < 0;
+  hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX>), grid,
+                     dim3(kBlock), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a, flip,
+                     partials, skip, skip_stride);
+  II_HIP(ctx, hipGetLastError());
+  return II_OK;
+}
+
+template <typename SpIn, typename Gauge, typename StOut, typename SpX>
+ii_status dispatch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, const StOut& out,
+                       const SpX& x, const SpX& dv, const HopSpec& h) {
+  using R = typename SpIn::R;
+  const R a = (R)h.a;
+  const int sel = (h.g5in ? 8 : 0) | (h.g5out ? 4 : 0) | (h.xpay ? 2 : 0) | (h.dot ? 1 : 0);
+#define II_CASE(N, A, B, CC, D)                                                              \
+  case N:                                                                                    \
+    return launch_hop<A, B, CC, D>(ctx, in, gauge, out, x, dv, h.out_par, a, h.nrhs, h.partials, \
+                                   h.skip, h.skip_stride);
+  switch (sel) {
+    II_CASE(0, false, false, false, false)
+    II_CASE(2, false, false, true, false)
+    II_CASE(3, false, false, true, true)
+    II_CASE(12, true, true, false, false)
+    II_CASE(14, true, true, true, false)
+    II_CASE(15, true, true, true, true)
+    II_CASE(8, true, false, false, false)
+    II_CASE(4, false, true, false, false)
+    II_CASE(6, false, true, true, false)
+    II_CASE(7, false, true, true, true)
+    default:
+      return set_error(ctx, II_ERR_BAD_ARG, "unsupported hop variant");
+  }
+#undef II_CASE
+}
+
+}  // namespace
+
+ii_status hop64(ii_context* ctx, const HopSpec& h, const double2* in, double2* out,
+                const double2* x, const double2* dv) {
+  const Geom& g = ctx->g;
+  SpF64 sin{in, g.stride};
+  GaugeF64 gf{ctx->gauge64, g.stride};
+  StF64 so{out, g.stride};
+  SpF64 sx{x ? x : in, g.stride};
+  SpF64 sd{dv ? dv : in, g.stride};
+  return dispatch_hop(ctx, sin, gf, so, sx, sd, h);
+}
+
+ii_status hop32(ii_context* ctx, const HopSpec& h, const float2* in, float2* out,
+                const float2* x, const float2* dv) {
+  const Geom& g = ctx->g;
+  SpF32 sin{in, g.stride};
+  GaugeF32r12 gf{(const float2*)ctx->gauge_lp[II_PREC_SINGLE], g.stride};
+  StF32 so{out, g.stride};
+  SpF32 sx{x ? x : in, g.stride};
+  SpF32 sd{dv ? dv : in, g.stride};
+  return dispatch_hop(ctx, sin, gf, so, sx, sd, h);
+}
+
+template <typename T>
+ii_status hop_lo(ii_context* ctx, const HopSpec& h, const LoField& in, const LoField& out,
+                 const LoField* x, const LoField* dv) {
+  const Geom& g = ctx->g;
+  const int prec = std::is_same<T, __half2>::value ? II_PREC_HALF : II_PREC_BF16;
+  SpLo<T> sin{(const T*)in.v, in.nrm, g.stride};
+  GaugeLoR12<T> gf{(const T*)ctx->gauge_lp[prec], g.stride};
+  StLo<T> so{(T*)out.v, out.nrm, g.stride};
+  SpLo<T> sx = x ? SpLo<T>{(const T*)x->v, x->nrm, g.stride} : sin;
+  SpLo<T> sd = dv ? SpLo<T>{(const T*)dv->v, dv->nrm, g.stride} : sin;
+  return dispatch_hop(ctx, sin, gf, so, sx, sd, h);
+}
+
+ii_status hop_half(ii_context* ctx, const HopSpec& h, const LoField& in, const LoField& out,
+                   const LoField* x, const LoField* dv) {
+  return hop_lo<__half2>(ctx, h, in, out, x, dv);
+}
+ii_status hop_bf16(ii_context* ctx, const HopSpec& h, const LoField& in, const LoField& out,
+                   const LoField* x, const LoField* dv) {
+  return hop_lo<__hip_bfloat162>(ctx, h, in, out, x, dv);
+}
+
+}  // namespace ii

@@ -1,0 +1,87 @@
+// Host-side building blocks shared by the API and the solvers.
+#pragma once
+
+#include "ii_internal.h"
+
+#include <type_traits>
+
+namespace ii {
+
+// One hop launch: out(out_par) = a * [g5] H [g5] in (+ x), optional
+// partial sums of Re<dv, out> into partials[rhs * nblocks + block].
+struct HopSpec {
+  int out_par = 0;
+  double a = 1.0;
+  bool g5in = false;
+  bool g5out = false;
+  bool xpay = false;
+  bool dot = false;
+  int nrhs = 1;
+  double* partials = nullptr;
+  // optional per-rhs skip flags: rhs r is skipped when skip[r * skip_stride] != 0
+  const int* skip = nullptr;
+  int skip_stride = 0;
+};
+
+// A 16-bit half field: components [rhs][12][stride] (half2 or bf16x2 as
+// raw 32-bit words) plus per-site norms [rhs][stride].
+struct LoField {
+  void* v = nullptr;
+  float* nrm = nullptr;
+};
+
+ii_status hop64(ii_context* ctx, const HopSpec& h, const double2* in, double2* out,
+                const double2* x, const double2* dv);
+ii_status hop32(ii_context* ctx, const HopSpec& h, const float2* in, float2* out,
+                const float2* x, const float2* dv);
+ii_status hop_half(ii_context* ctx, const HopSpec& h, const LoField& in, const LoField& out,
+                   const LoField* x, const LoField* dv);
+ii_status hop_bf16(ii_context* ctx, const HopSpec& h, const LoField& in, const LoField& out,
+                   const LoField* x, const LoField* dv);
+
+inline int hop_blocks(const Geom& g) { return blocks_for(g.Vh); }
+
+// ------------------------------------------------------------- FP64 operators
+// Dhat f = f - Deo Doe f (dirac_wilson_EvenOdd.hpp:79-84); t: odd temp.
+ii_status dhat64(ii_context* ctx, double kappa, const double2* f, double2* out, double2* t,
+                 int nrhs = 1);
+// Dhat^dag f = f - g5 Deo Doe g5 f (:86-92); dot: partials of Re<dv, out>.
+ii_status dhat_dag64(ii_context* ctx, double kappa, const double2* f, double2* out, double2* t,
+                     const double2* dv = nullptr, double* partials = nullptr, int nrhs = 1);
+// A = Dhat^dag Dhat (fopr.h:151), with optional Re<dv, A f> partials.
+ii_status ddagd64(ii_context* ctx, double kappa, const double2* f, double2* out, double2* q,
+                  double2* t, const double2* dv = nullptr, double* partials = nullptr,
+                  int nrhs = 1);
+
+// ------------------------------------------------------------- BLAS-1 (FP64)
+// All operate on nrhs half fields [rhs][12][stride]; reductions produce
+// one double per rhs via partial sums and a fixed-order final sum.
+ii_status dot64(ii_context* ctx, const double2* a, const double2* b, double* dst, int nrhs = 1);
+ii_status finalize_sums(ii_context* ctx, const double* partials, int nblk, double* dst, int nrhs);
+ii_status copy64(ii_context* ctx, double2* dst, const double2* src, int nrhs = 1);
+ii_status zero64(ii_context* ctx, double2* dst, int nrhs = 1);
+// y = a*x + b*y with host scalars
+ii_status axpby64(ii_context* ctx, double a, const double2* x, double b, double2* y, int nrhs = 1);
+ii_status gamma5_64(ii_context* ctx, const double2* in, double2* out, int nrhs = 1);
+
+// ------------------------------------------------------------- packing
+ii_status pack_half_spinor(ii_context* ctx, const double* host_dev, double2* dst);
+ii_status unpack_half_spinor(ii_context* ctx, const double2* src, double* host_dev);
+ii_status pack_full_spinor(ii_context* ctx, const double* host_dev, double2* even, double2* odd);
+ii_status unpack_full_spinor(ii_context* ctx, const double2* even, const double2* odd,
+                             double* host_dev);
+ii_status pack_gauge(ii_context* ctx, const double* host_dev);
+ii_status unpack_gauge(ii_context* ctx, double* host_dev);
+ii_status build_lowprec_gauge(ii_context* ctx, int prec);
+ii_status plaquette64(ii_context* ctx, double* plaq);
+
+// learned preconditioner (ii_learned.hip)
+void learned_release(ii_context* ctx);
+ii_status learned_propagator(ii_context* ctx, double kappa, const ii_prop_params* p,
+                             const ii_field* src, ii_field* sol, ii_solver_output* out);
+
+// conversions between FP64 and the low-precision sloppy fields
+ii_status convert_64_to_32(ii_context* ctx, const double2* src, float2* dst, int nrhs);
+ii_status convert_32_to_64(ii_context* ctx, const float2* src, double2* dst, int nrhs);
+
+}  // namespace ii

@@ -1,0 +1,1257 @@
+// Device-resident solvers of the Wilson even/odd solve path.
+//
+//  * ii_solver_cg      Solver_CG::solve on Fopr_DdagD (solver_CG.cpp:29-160)
+//  * ii_multishift_cg  MultiShiftSolver_CG::solve (multiShiftSolver_CG.cpp:10-185)
+//  * ii_eo_invert      Inverter_WilsonLike::invert (eoUtils.cpp:43-80)
+//  * ii_propagator_solve  the production path: even/odd Schur solve of many
+//    right-hand sides at once, CG on Dhat^dag Dhat in FP64, FP32 or 16-bit
+//    storage with FP64 reliable updates, stopping on the FP64 true residual.
+//
+// All solver scalars (alpha, beta, |r|^2, stopping flags, shift recurrences)
+// live in device memory and are computed by single-block kernels, so an
+// iteration needs no host round trip; the host only polls a small status
+// block.  Every kernel of an iteration reads the per-rhs `done` flag and
+// returns at once when set, so iterations queued past convergence are free
+// and leave the solution untouched: the iteration count is exact.
+#include "ii_ops.h"
+#include "ii_stencil.h"
+
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+namespace ii {
+
+// ============================================================ scalars
+struct RUScal {
+  double rr, pap, alpha, beta;
+  double rmax2;    // |r|^2 at the last reliable update
+  double target2;  // |r|^2 below which the true residual is checked
+  double tol2;     // tol^2 |b|^2 on the true residual
+  double delta2;   // reliable-update trigger on |r|^2 / rmax2
+  double rho2;     // last true residual^2
+  double b2;       // |b|^2 of the full source
+  double snorm;    // 1/|b|^2 (reference CG)
+  double prec;     // reference CG threshold on rr*snorm
+  int done;        // iteration kernels skip when set
+  int skip_ru;     // reliable-update kernels skip when set (= done || !update)
+  int count;       // iterations done
+  int iter;        // reference loop index at convergence
+  int updates;     // reliable updates done
+  int nan;
+  int pad[2];
+};
+constexpr int kScalInts = sizeof(RUScal) / sizeof(int);
+
+__device__ __forceinline__ bool finite(double v) { return isfinite(v); }
+
+// pap -> alpha = rr/pap
+__global__ void k_fin_pap(const double* __restrict__ partials, int nb, RUScal* __restrict__ sc) {
+  RUScal& S = sc[blockIdx.x];
+  if (S.done) return;
+  const double* p = partials + (size_t)blockIdx.x * nb;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    S.pap = s;
+    S.alpha = S.rr / s;
+    if (!finite(S.alpha)) { S.nan = 1; S.done = 1; S.skip_ru = 1; }
+  }
+}
+
+// rr_new -> beta, and the stopping / reliable-update decisions.
+// mode 0: reference CG test rr*snorm < prec (solver_CG.cpp:57-60)
+// mode 1: reliable-update CG: flag an update when |r|^2 < delta2*rmax2 or
+//         |r|^2 < target2.
+__global__ void k_fin_rr(const double* __restrict__ partials, int nb, RUScal* __restrict__ sc,
+                         int mode) {
+  RUScal& S = sc[blockIdx.x];
+  if (S.done) return;
+  const double* p = partials + (size_t)blockIdx.x * nb;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    double rrp = S.rr;
+    S.rr = s;
+    S.beta = s / rrp;
+    S.count += 1;
+    if (!finite(s) || !finite(S.beta)) { S.nan = 1; S.done = 1; S.skip_ru = 1; return; }
+    if (mode == 0) {
+      if (s * S.snorm < S.prec) { S.done = 1; S.iter = S.count - 1; }
+    } else {
+      bool upd = (s < S.delta2 * S.rmax2) || (s < S.target2);
+      S.skip_ru = upd ? 0 : 1;
+    }
+  }
+}
+
+// ============================================================ generic site BLAS
+// Accessor construction from an any-precision field.
+struct SF {
+  void* v = nullptr;
+  float* nrm = nullptr;
+};
+
+template <int P> struct Acc;
+template <> struct Acc<II_PREC_DOUBLE> {
+  using Sp = SpF64;
+  using St = StF64;
+  static Sp sp(const SF& f, int stride) { return Sp{(const double2*)f.v, stride}; }
+  static St st(const SF& f, int stride) { return St{(double2*)f.v, stride}; }
+};
+template <> struct Acc<II_PREC_SINGLE> {
+  using Sp = SpF32;
+  using St = StF32;
+  static Sp sp(const SF& f, int stride) { return Sp{(const float2*)f.v, stride}; }
+  static St st(const SF& f, int stride) { return St{(float2*)f.v, stride}; }
+};
+template <> struct Acc<II_PREC_HALF> {
+  using Sp = SpLo<__half2>;
+  using St = StLo<__half2>;
+  static Sp sp(const SF& f, int stride) { return Sp{(const __half2*)f.v, f.nrm, stride}; }
+  static St st(const SF& f, int stride) { return St{(__half2*)f.v, f.nrm, stride}; }
+};
+template <> struct Acc<II_PREC_BF16> {
+  using Sp = SpLo<__hip_bfloat162>;
+  using St = StLo<__hip_bfloat162>;
+  static Sp sp(const SF& f, int stride) { return Sp{(const __hip_bfloat162*)f.v, f.nrm, stride}; }
+  static St st(const SF& f, int stride) { return St{(__hip_bfloat162*)f.v, f.nrm, stride}; }
+};
+
+template <typename Sp>
+__device__ __forceinline__ void ld_site(const Sp& a, int hs, double v[24]) {
+  const double s = (double)a.scale(hs);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    auto c = a.ld(hs, k);
+    v[2 * k] = (double)c.x * s;
+    v[2 * k + 1] = (double)c.y * s;
+  }
+}
+template <typename St>
+__device__ __forceinline__ void st_site(const St& a, int hs, const double v[24]) {
+  typename St::C o[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    o[k].x = (typename St::R)v[2 * k];
+    o[k].y = (typename St::R)v[2 * k + 1];
+  }
+  a.st(hs, o);
+}
+
+// y += alpha p ; r -= alpha s ; partial |r|^2      (solver_CG.cpp:131-147)
+template <typename SpP, typename SpS, typename SpR, typename StR, typename SpY, typename StY>
+__global__ void __launch_bounds__(kBlock)
+k_cg_xr(Geom g, SpP p, SpS s, SpR rr_, StR rw, SpY yr, StY yw, RUScal* __restrict__ sc,
+        double* __restrict__ partials) {
+  const int rhs = blockIdx.y;
+  const RUScal& S = sc[rhs];
+  if (S.done) return;
+  const double a = S.alpha;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (hs < g.Vh) {
+    double pv[24], sv[24], rv[24], yv[24];
+    ld_site(p.rhs(rhs), hs, pv);
+    ld_site(s.rhs(rhs), hs, sv);
+    ld_site(rr_.rhs(rhs), hs, rv);
+    ld_site(yr.rhs(rhs), hs, yv);
+#pragma unroll
+    for (int k = 0; k < 24; ++k) {
+      yv[k] = fma(a, pv[k], yv[k]);
+      rv[k] = fma(-a, sv[k], rv[k]);
+      acc = fma(rv[k], rv[k], acc);
+    }
+    st_site(yw.rhs(rhs), hs, yv);
+    st_site(rw.rhs(rhs), hs, rv);
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = acc;
+}
+
+// p = r + beta p        (solver_CG.cpp:152-153: p = p*cr + r)
+template <typename SpR, typename SpP, typename StP>
+__global__ void __launch_bounds__(kBlock)
+k_cg_p(Geom g, SpR r, SpP pr, StP pw, const RUScal* __restrict__ sc) {
+  const int rhs = blockIdx.y;
+  const RUScal& S = sc[rhs];
+  if (S.done) return;
+  const double b = S.beta;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (hs >= g.Vh) return;
+  double rv[24], pv[24];
+  ld_site(r.rhs(rhs), hs, rv);
+  ld_site(pr.rhs(rhs), hs, pv);
+#pragma unroll
+  for (int k = 0; k < 24; ++k) pv[k] = fma(pv[k], b, rv[k]);
+  st_site(pw.rhs(rhs), hs, pv);
+}
+
+// dst = src (precision conversion), masked by skip flags
+template <typename SpA, typename StB>
+__global__ void __launch_bounds__(kBlock)
+k_convert(Geom g, SpA a, StB b, const int* __restrict__ skip, int skip_stride) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs * skip_stride]) return;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (hs >= g.Vh) return;
+  double v[24];
+  ld_site(a.rhs(rhs), hs, v);
+  st_site(b.rhs(rhs), hs, v);
+}
+
+// y64 += ys ; ys = 0  (reliable update accumulation), masked
+template <typename SpY, typename StY>
+__global__ void __launch_bounds__(kBlock)
+k_accum(Geom g, SpY ys, StY yz, double2* __restrict__ y64, const int* __restrict__ skip,
+        int skip_stride) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs * skip_stride]) return;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (hs >= g.Vh) return;
+  double v[24];
+  ld_site(ys.rhs(rhs), hs, v);
+  double2* y = y64 + (size_t)rhs * 12 * g.stride;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    double2 o = y[(size_t)k * g.stride + hs];
+    o.x += v[2 * k];
+    o.y += v[2 * k + 1];
+    y[(size_t)k * g.stride + hs] = o;
+  }
+  double z[24];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) z[k] = 0.0;
+  st_site(yz.rhs(rhs), hs, z);
+}
+
+// rho = bhat - q ; partial |rho|^2, masked (FP64)
+__global__ void __launch_bounds__(kBlock)
+k_resid(Geom g, const double2* __restrict__ bh, const double2* __restrict__ q,
+        double2* __restrict__ rho, const int* __restrict__ skip, int skip_stride,
+        double* __restrict__ partials) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs * skip_stride]) return;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (hs < g.Vh) {
+    size_t off = (size_t)rhs * 12 * g.stride + hs;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      double2 b = bh[off + (size_t)k * g.stride], v = q[off + (size_t)k * g.stride];
+      double2 d = make_double2(b.x - v.x, b.y - v.y);
+      rho[off + (size_t)k * g.stride] = d;
+      acc = fma(d.x, d.x, fma(d.y, d.y, acc));
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = acc;
+}
+
+// masked |a|^2 partials (FP64)
+__global__ void __launch_bounds__(kBlock)
+k_norm2_masked(Geom g, const double2* __restrict__ a, const int* __restrict__ skip,
+               int skip_stride, double* __restrict__ partials) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs * skip_stride]) return;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (hs < g.Vh) {
+    size_t off = (size_t)rhs * 12 * g.stride + hs;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      double2 v = a[off + (size_t)k * g.stride];
+      acc = fma(v.x, v.x, fma(v.y, v.y, acc));
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = acc;
+}
+
+// after the true residual: done test
+__global__ void k_fin_rho(const double* __restrict__ partials, int nb, RUScal* __restrict__ sc) {
+  RUScal& S = sc[blockIdx.x];
+  if (S.skip_ru) return;
+  const double* p = partials + (size_t)blockIdx.x * nb;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    S.rho2 = s;
+    if (!finite(s)) { S.nan = 1; S.done = 1; S.skip_ru = 1; return; }
+    if (s <= S.tol2) { S.done = 1; S.skip_ru = 1; S.updates += 1; }
+  }
+}
+
+// after the recomputed normal residual: restart the recursion scalars
+__global__ void k_fin_ru(const double* __restrict__ partials, int nb, RUScal* __restrict__ sc) {
+  RUScal& S = sc[blockIdx.x];
+  if (S.skip_ru) return;
+  const double* p = partials + (size_t)blockIdx.x * nb;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    S.rr = s;
+    S.rmax2 = s;
+    // |rho| / |Dhat^dag rho| calibrates when to test the true residual next
+    double ratio = S.rho2 / s;
+    double t = 0.81 * S.tol2 / ratio;
+    S.target2 = t < S.target2 ? t : S.target2;
+    S.updates += 1;
+    S.skip_ru = 1;  // restores the mask; `update` consumed
+    if (!finite(s)) { S.nan = 1; S.done = 1; }
+  }
+}
+
+// ============================================================ precision dispatch
+ii_status hopP(ii_context* ctx, int P, const HopSpec& h, const SF& in, const SF& out,
+               const SF* x, const SF* dv) {
+  switch (P) {
+    case II_PREC_DOUBLE:
+      return hop64(ctx, h, (const double2*)in.v, (double2*)out.v,
+                   x ? (const double2*)x->v : nullptr, dv ? (const double2*)dv->v : nullptr);
+    case II_PREC_SINGLE:
+      return hop32(ctx, h, (const float2*)in.v, (float2*)out.v,
+                   x ? (const float2*)x->v : nullptr, dv ? (const float2*)dv->v : nullptr);
+    case II_PREC_HALF:
+    case II_PREC_BF16: {
+      LoField li{in.v, in.nrm}, lo{out.v, out.nrm}, lx, ld;
+      if (x) lx = LoField{x->v, x->nrm};
+      if (dv) ld = LoField{dv->v, dv->nrm};
+      return P == II_PREC_HALF ? hop_half(ctx, h, li, lo, x ? &lx : nullptr, dv ? &ld : nullptr)
+                               : hop_bf16(ctx, h, li, lo, x ? &lx : nullptr, dv ? &ld : nullptr);
+    }
+  }
+  return set_error(ctx, II_ERR_BAD_ARG, "unknown precision");
+}
+
+// A = Dhat^dag Dhat in precision P with partials of Re<dv, A f>; skip flags.
+ii_status ddagdP(ii_context* ctx, int P, double kappa, const SF& f, const SF& out, const SF& q,
+                 const SF& t, const SF* dv, double* partials, int nrhs, const int* skip,
+                 int skip_stride) {
+  HopSpec h;
+  h.nrhs = nrhs;
+  h.skip = skip;
+  h.skip_stride = skip_stride;
+  h.out_par = 1;
+  h.a = 1.0;
+  II_CHECK(hopP(ctx, P, h, f, t, nullptr, nullptr));
+  h.out_par = 0;
+  h.a = -kappa * kappa;
+  h.xpay = true;
+  II_CHECK(hopP(ctx, P, h, t, q, &f, nullptr));  // q = Dhat f
+  h.out_par = 1;
+  h.a = 1.0;
+  h.xpay = false;
+  h.g5in = true;
+  II_CHECK(hopP(ctx, P, h, q, t, nullptr, nullptr));
+  h.out_par = 0;
+  h.a = -kappa * kappa;
+  h.xpay = true;
+  h.g5in = false;
+  h.g5out = true;
+  h.dot = dv != nullptr;
+  h.partials = partials;
+  return hopP(ctx, P, h, t, out, &q, dv);  // out = Dhat^dag q
+}
+
+template <int P>
+ii_status cg_step_blas(ii_context* ctx, const SF& p, const SF& s, const SF& r, const SF& y,
+                       int ytype, RUScal* sc, int nrhs, double* partials, int nb) {
+  const Geom& g = ctx->g;
+  using A = Acc<P>;
+  dim3 grid(nb, nrhs);
+  if (ytype == II_PREC_DOUBLE) {
+    hipLaunchKernelGGL((k_cg_xr<typename A::Sp, typename A::Sp, typename A::Sp, typename A::St,
+                                SpF64, StF64>),
+                       grid, dim3(kBlock), 0, ctx->stream, g, A::sp(p, g.stride),
+                       A::sp(s, g.stride), A::sp(r, g.stride), A::st(r, g.stride),
+                       Acc<II_PREC_DOUBLE>::sp(y, g.stride), Acc<II_PREC_DOUBLE>::st(y, g.stride),
+                       sc, partials);
+  } else {
+    hipLaunchKernelGGL((k_cg_xr<typename A::Sp, typename A::Sp, typename A::Sp, typename A::St,
+                                SpF32, StF32>),
+                       grid, dim3(kBlock), 0, ctx->stream, g, A::sp(p, g.stride),
+                       A::sp(s, g.stride), A::sp(r, g.stride), A::st(r, g.stride),
+                       Acc<II_PREC_SINGLE>::sp(y, g.stride), Acc<II_PREC_SINGLE>::st(y, g.stride),
+                       sc, partials);
+  }
+  II_HIP(ctx, hipGetLastError());
+  return II_OK;
+}
+
+template <int P>
+ii_status cg_p_update(ii_context* ctx, const SF& r, const SF& p, RUScal* sc, int nrhs) {
+  const Geom& g = ctx->g;
+  using A = Acc<P>;
+  hipLaunchKernelGGL((k_cg_p<typename A::Sp, typename A::Sp, typename A::St>),
+                     dim3(blocks_for(g.Vh), nrhs), dim3(kBlock), 0, ctx->stream, g,
+                     A::sp(r, g.stride), A::sp(p, g.stride), A::st(p, g.stride), sc);
+  II_HIP(ctx, hipGetLastError());
+  return II_OK;
+}
+
+// convert between precisions (Pa -> Pb), masked
+template <int Pa, int Pb>
+ii_status convert_t(ii_context* ctx, const SF& a, const SF& b, int nrhs, const int* skip,
+                    int skip_stride) {
+  const Geom& g = ctx->g;
+  hipLaunchKernelGGL((k_convert<typename Acc<Pa>::Sp, typename Acc<Pb>::St>),
+                     dim3(blocks_for(g.Vh), nrhs), dim3(kBlock), 0, ctx->stream, g,
+                     Acc<Pa>::sp(a, g.stride), Acc<Pb>::st(b, g.stride), skip, skip_stride);
+  II_HIP(ctx, hipGetLastError());
+  return II_OK;
+}
+
+ii_status convertP(ii_context* ctx, int Pa, int Pb, const SF& a, const SF& b, int nrhs,
+                   const int* skip = nullptr, int skip_stride = 0) {
+#define II_CV(A, B) \
+  if (Pa == A && Pb == B) return convert_t<A, B>(ctx, a, b, nrhs, skip, skip_stride);
+  II_CV(0, 1) II_CV(0, 2) II_CV(0, 3) II_CV(1, 0) II_CV(2, 0) II_CV(3, 0) II_CV(0, 0)
+  II_CV(1, 1) II_CV(2, 2) II_CV(3, 3)
+#undef II_CV
+  return set_error(ctx, II_ERR_BAD_ARG, "unsupported conversion");
+}
+
+ii_status cg_blas_P(ii_context* ctx, int P, const SF& p, const SF& s, const SF& r, const SF& y,
+                    int ytype, RUScal* sc, int nrhs, double* partials, int nb) {
+  switch (P) {
+    case 0: return cg_step_blas<0>(ctx, p, s, r, y, ytype, sc, nrhs, partials, nb);
+    case 1: return cg_step_blas<1>(ctx, p, s, r, y, ytype, sc, nrhs, partials, nb);
+    case 2: return cg_step_blas<2>(ctx, p, s, r, y, ytype, sc, nrhs, partials, nb);
+    default: return cg_step_blas<3>(ctx, p, s, r, y, ytype, sc, nrhs, partials, nb);
+  }
+}
+ii_status cg_p_P(ii_context* ctx, int P, const SF& r, const SF& p, RUScal* sc, int nrhs) {
+  switch (P) {
+    case 0: return cg_p_update<0>(ctx, r, p, sc, nrhs);
+    case 1: return cg_p_update<1>(ctx, r, p, sc, nrhs);
+    case 2: return cg_p_update<2>(ctx, r, p, sc, nrhs);
+    default: return cg_p_update<3>(ctx, r, p, sc, nrhs);
+  }
+}
+
+// bytes of one rhs of a half field in precision P (components + norms)
+size_t field_bytes(const Geom& g, int P) {
+  switch (P) {
+    case II_PREC_DOUBLE: return (size_t)12 * g.stride * sizeof(double2);
+    case II_PREC_SINGLE: return (size_t)12 * g.stride * sizeof(float2);
+    default: return (size_t)12 * g.stride * sizeof(__half2) + (size_t)g.stride * sizeof(float);
+  }
+}
+// a workspace field of nrhs rhs in precision P (norms after the components)
+ii_status fieldP(ii_context* ctx, int slot, int P, int nrhs, SF* f) {
+  const Geom& g = ctx->g;
+  void* p = nullptr;
+  II_CHECK(workspace(ctx, slot, field_bytes(g, P) * nrhs, &p));
+  f->v = p;
+  f->nrm = nullptr;
+  if (P >= II_PREC_HALF)
+    f->nrm = (float*)((char*)p + (size_t)12 * g.stride * sizeof(__half2) * nrhs);
+  return II_OK;
+}
+
+// ============================================================ status polling
+ii_status read_scalars(ii_context* ctx, int n) {
+  II_HIP(ctx, hipMemcpyAsync(ctx->scal_host, ctx->scal, n * sizeof(RUScal),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return II_OK;
+}
+RUScal* host_scal(ii_context* ctx) { return reinterpret_cast<RUScal*>(ctx->scal_host); }
+RUScal* dev_scal(ii_context* ctx) { return reinterpret_cast<RUScal*>(ctx->scal); }
+
+ii_status ensure_ru_scalars(ii_context* ctx, int n) {
+  static_assert(sizeof(RUScal) <= sizeof(CGScalars) * 2, "scalar block");
+  return ensure_scalars(ctx, 2 * n);
+}
+
+__global__ void k_scal_init(RUScal* sc, const double* __restrict__ rr, const double* __restrict__ bb,
+                            double prec, double tol, double delta, int mode) {
+  RUScal& S = sc[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  RUScal z{};
+  z.rr = rr[blockIdx.x];
+  double bn = sqrt(bb[blockIdx.x]);
+  z.b2 = bb[blockIdx.x];
+  z.snorm = 1.0 / (bn * bn);
+  z.prec = prec;
+  z.rmax2 = z.rr;
+  z.tol2 = tol * tol * z.b2;
+  z.target2 = z.tol2;
+  z.delta2 = delta * delta;
+  z.iter = -1;
+  z.skip_ru = 1;
+  if (!(z.rr > 0.0)) { z.done = 1; }
+  if (mode == 0) z.skip_ru = 1;
+  S = z;
+}
+
+// ============================================================ reference CG
+// Solver_CG::solve semantics on Fopr_DdagD: x0 = b (solver_CG.cpp:40).
+ii_status cg_reference(ii_context* ctx, double kappa, double2* x, const double2* b, double prec,
+                       int maxiter, ii_solver_output* out) {
+  const Geom& g = ctx->g;
+  II_CHECK(ensure_ru_scalars(ctx, 1));
+  RUScal* sc = dev_scal(ctx);
+  int nb = hop_blocks(g);
+  II_CHECK(ensure_partials(ctx, (size_t)nb + 8));
+  double* part = ctx->partials;
+  double* tmp2 = nullptr;
+  {
+    void* p;
+    II_CHECK(workspace(ctx, 30, 4 * sizeof(double), &p));
+    tmp2 = (double*)p;
+  }
+  SF fx{x}, fr, fp, fs, fq, ft;
+  II_CHECK(fieldP(ctx, 10, 0, 1, &fr));
+  II_CHECK(fieldP(ctx, 11, 0, 1, &fp));
+  II_CHECK(fieldP(ctx, 12, 0, 1, &fs));
+  II_CHECK(fieldP(ctx, 13, 0, 1, &fq));
+  II_CHECK(fieldP(ctx, 14, 0, 1, &ft));
+  const int ss = kScalInts;
+  const int* skip = &sc[0].done;
+  II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  II_CHECK(copy64(ctx, x, b));
+  // r = b - A x
+  II_CHECK(ddagdP(ctx, 0, kappa, fx, fs, fq, ft, nullptr, nullptr, 1, nullptr, 0));
+  II_CHECK(copy64(ctx, (double2*)fr.v, b));
+  II_CHECK(axpby64(ctx, -1.0, (double2*)fs.v, 1.0, (double2*)fr.v));
+  II_CHECK(copy64(ctx, (double2*)fp.v, (double2*)fr.v));
+  II_CHECK(dot64(ctx, (double2*)fr.v, (double2*)fr.v, tmp2));
+  II_CHECK(dot64(ctx, b, b, tmp2 + 1));
+  hipLaunchKernelGGL(k_scal_init, dim3(1), dim3(64), 0, ctx->stream, sc, tmp2, tmp2 + 1, prec,
+                     0.0, 0.0, 0);
+  II_HIP(ctx, hipGetLastError());
+  int queued = 0;
+  int chunk = 1;
+  bool conv = false;
+  while (queued < maxiter) {
+    int n = std::min(chunk, maxiter - queued);
+    for (int i = 0; i < n; ++i) {
+      II_CHECK(ddagdP(ctx, 0, kappa, fp, fs, fq, ft, &fp, part, 1, skip, ss));
+      hipLaunchKernelGGL(k_fin_pap, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, sc);
+      II_CHECK(cg_blas_P(ctx, 0, fp, fs, fr, fx, 0, sc, 1, part, nb));
+      hipLaunchKernelGGL(k_fin_rr, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, sc, 0);
+      II_CHECK(cg_p_P(ctx, 0, fr, fp, sc, 1));
+    }
+    queued += n;
+    II_CHECK(read_scalars(ctx, 1));
+    const RUScal& H = host_scal(ctx)[0];
+    if (H.nan) return set_error(ctx, II_ERR_NAN, "non-finite CG scalar");
+    if (H.done) { conv = true; break; }
+    chunk = std::min(chunk * 2, 64);
+  }
+  // diff = |A x - b| / |b|  (solver_CG.cpp:69-71)
+  II_CHECK(ddagdP(ctx, 0, kappa, fx, fs, fq, ft, nullptr, nullptr, 1, nullptr, 0));
+  II_CHECK(axpby64(ctx, -1.0, b, 1.0, (double2*)fs.v));
+  II_CHECK(dot64(ctx, (double2*)fs.v, (double2*)fs.v, tmp2 + 2));
+  II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  double h[3];
+  II_HIP(ctx, hipMemcpyAsync(h, tmp2, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  II_CHECK(read_scalars(ctx, 1));
+  const RUScal& H = host_scal(ctx)[0];
+  float ms = 0.f;
+  II_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (out) {
+    out->iterations = conv ? H.iter : -1;
+    out->diff = std::sqrt(h[2]) * std::sqrt(H.snorm);
+    out->timing_ms = ms;
+    out->true_residual = -1.0;
+    out->restarts = 0;
+    out->inner_iterations = H.count;
+  }
+  if (!conv)
+    return set_error(ctx, II_ERR_NOT_CONVERGED,
+                     "CG solver not converged. Current residual: " +
+                         std::to_string(H.rr * H.snorm));
+  return II_OK;
+}
+
+// ============================================================ multishift CG
+constexpr int kMaxShift = 64;
+struct MSScal {
+  double rr, pap, alpha, beta, alphap, betap, snorm, prec;
+  double sigma[kMaxShift], csh2[kMaxShift], zeta1[kMaxShift], zeta2[kMaxShift];
+  double zeta[kMaxShift], alphas[kMaxShift], betas[kMaxShift], pp[kMaxShift];
+  int nshift, nshift2, done, count, nconv, nan;
+};
+
+__global__ void k_ms_init(MSScal* S, const double* __restrict__ rrbb, const double* sigma, int n,
+                          double prec) {
+  if (threadIdx.x != 0) return;
+  S->rr = rrbb[0];
+  double bn = sqrt(rrbb[1]);
+  S->snorm = 1.0 / (bn * bn);
+  S->prec = prec;
+  S->alphap = 0.0;
+  S->betap = 1.0;
+  S->nshift = n;
+  S->nshift2 = n;
+  S->done = 0;
+  S->count = 0;
+  S->nconv = -1;
+  S->nan = 0;
+  for (int i = 0; i < n; ++i) {
+    S->sigma[i] = sigma[i];
+    S->csh2[i] = sigma[i] - sigma[0];
+    S->zeta1[i] = 1.0;
+    S->zeta2[i] = 1.0;
+    S->pp[i] = 0.0;
+  }
+}
+
+// s += sigma0 p0 ; partial <s, p0>      (multiShiftSolver_CG.cpp:50-55)
+__global__ void __launch_bounds__(kBlock)
+k_ms_sp(Geom g, double2* __restrict__ s, const double2* __restrict__ p0, const MSScal* S,
+        double* __restrict__ partials) {
+  if (S->done) return;
+  const double sig = S->sigma[0];
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (hs < g.Vh) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      size_t i = (size_t)k * g.stride + hs;
+      double2 sv = s[i], pv = p0[i];
+      sv.x = fma(sig, pv.x, sv.x);
+      sv.y = fma(sig, pv.y, sv.y);
+      s[i] = sv;
+      acc = fma(sv.x, pv.x, fma(sv.y, pv.y, acc));
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+__global__ void k_ms_fin_pap(const double* __restrict__ partials, int nb, MSScal* S) {
+  if (S->done) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    S->pap = s;
+    S->beta = -S->rr / s;
+    if (!isfinite(S->beta)) { S->nan = 1; S->done = 1; }
+  }
+}
+
+// x0 -= beta p0 ; r += beta s ; partial |r|^2   (:57-59)
+__global__ void __launch_bounds__(kBlock)
+k_ms_xr(Geom g, double2* __restrict__ x0, const double2* __restrict__ p0, double2* __restrict__ r,
+        const double2* __restrict__ s, const MSScal* S, double* __restrict__ partials) {
+  if (S->done) return;
+  const double b = S->beta;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (hs < g.Vh) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      size_t i = (size_t)k * g.stride + hs;
+      double2 xv = x0[i], pv = p0[i], rv = r[i], sv = s[i];
+      xv.x = fma(-b, pv.x, xv.x);
+      xv.y = fma(-b, pv.y, xv.y);
+      rv.x = fma(b, sv.x, rv.x);
+      rv.y = fma(b, sv.y, rv.y);
+      x0[i] = xv;
+      r[i] = rv;
+      acc = fma(rv.x, rv.x, fma(rv.y, rv.y, acc));
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// rr, alpha and the shift recurrences (:61-83)
+__global__ void k_ms_fin_rr(const double* __restrict__ partials, int nb, MSScal* S) {
+  if (S->done) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    double rrp = S->rr;
+    S->rr = s;
+    double alpha = s / rrp;
+    S->alpha = alpha;
+    S->pp[0] = s;
+    double beta = S->beta;
+    double alphah = 1.0 + S->alphap * beta / S->betap;
+    for (int ish = 1; ish < S->nshift2; ++ish) {
+      double zeta = (alphah - S->csh2[ish] * beta) / S->zeta1[ish] + (1.0 - alphah) / S->zeta2[ish];
+      zeta = 1.0 / zeta;
+      double zr = zeta / S->zeta1[ish];
+      S->betas[ish] = beta * zr;
+      S->alphas[ish] = alpha * zr * zr;
+      S->zeta[ish] = zeta;
+    }
+    if (!isfinite(alpha)) { S->nan = 1; S->done = 1; }
+  }
+}
+
+// p0 = alpha p0 + r (:63-64) and, for ish in [1, nshift2):
+// x_i -= betas p_i ; p_i = alphas p_i + zeta r ; partial |p_i|^2 (:73-79)
+__global__ void __launch_bounds__(kBlock)
+k_ms_vec(Geom g, double2* __restrict__ x, double2* __restrict__ p, const double2* __restrict__ r,
+         const MSScal* S, double* __restrict__ partials) {
+  if (S->done) return;
+  const int ish = blockIdx.y;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t he = (size_t)12 * g.stride;
+  if (ish == 0) {
+    if (hs < g.Vh) {
+      const double a = S->alpha;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        size_t i = (size_t)k * g.stride + hs;
+        double2 pv = p[i], rv = r[i];
+        p[i] = make_double2(fma(pv.x, a, rv.x), fma(pv.y, a, rv.y));
+      }
+    }
+    return;
+  }
+  if (ish >= S->nshift2) return;
+  const double bs = S->betas[ish], as = S->alphas[ish], z = S->zeta[ish];
+  double acc = 0.0;
+  if (hs < g.Vh) {
+    double2* xi = x + ish * he;
+    double2* pi = p + ish * he;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      size_t i = (size_t)k * g.stride + hs;
+      double2 xv = xi[i], pv = pi[i], rv = r[i];
+      xv.x = fma(-bs, pv.x, xv.x);
+      xv.y = fma(-bs, pv.y, xv.y);
+      pv.x = fma(pv.x, as, z * rv.x);
+      pv.y = fma(pv.y, as, z * rv.y);
+      xi[i] = xv;
+      pi[i] = pv;
+      acc = fma(pv.x, pv.x, fma(pv.y, pv.y, acc));
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partials[(size_t)ish * gridDim.x + blockIdx.x] = acc;
+}
+
+// pp_i, zeta shifts, Nshift2 shrink, stopping test (:79-92, :148-152)
+__global__ void k_ms_tail(const double* __restrict__ partials, int nb, MSScal* S) {
+  if (S->done) return;
+  __shared__ double sums[kMaxShift];
+  for (int ish = 1; ish < S->nshift2; ++ish) {
+    const double* p = partials + (size_t)ish * nb;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) sums[ish] = s;
+  }
+  if (threadIdx.x == 0) {
+    for (int ish = 1; ish < S->nshift2; ++ish) {
+      S->pp[ish] = sums[ish] * S->snorm;
+      S->zeta2[ish] = S->zeta1[ish];
+      S->zeta1[ish] = S->zeta[ish];
+    }
+    for (int ish = S->nshift2 - 1; ish >= 0; --ish) {
+      if (S->pp[ish] > S->prec) { S->nshift2 = ish + 1; break; }
+    }
+    S->alphap = S->alpha;
+    S->betap = S->beta;
+    S->count += 1;
+    if (S->rr * S->snorm < S->prec) { S->done = 1; S->nconv = S->count - 1; }
+  }
+}
+
+ii_status multishift_reference(ii_context* ctx, double kappa, double2* x, const double2* b,
+                               const double* shifts, int nshift, double prec, int maxiter,
+                               ii_solver_output* out, std::vector<double>* per_shift) {
+  const Geom& g = ctx->g;
+  if (nshift < 1 || nshift > kMaxShift) return set_error(ctx, II_ERR_BAD_ARG, "nshift out of range");
+  int nb = hop_blocks(g);
+  II_CHECK(ensure_partials(ctx, (size_t)nb * nshift + 16));
+  double* part = ctx->partials;
+  void* wp = nullptr;
+  II_CHECK(workspace(ctx, 29, sizeof(MSScal) + (kMaxShift + 8) * sizeof(double), &wp));
+  MSScal* S = (MSScal*)wp;
+  double* aux = (double*)((char*)wp + sizeof(MSScal));  // rr, bb, sigma[...]
+  MSScal* Sh = nullptr;
+  II_HIP(ctx, hipHostMalloc(&Sh, sizeof(MSScal)));
+  struct Free { MSScal* p; ~Free() { if (p) (void)hipHostFree(p); } } fr_{Sh};
+  SF fr, fs, fq, ft, fp;
+  II_CHECK(fieldP(ctx, 10, 0, 1, &fr));
+  II_CHECK(fieldP(ctx, 12, 0, 1, &fs));
+  II_CHECK(fieldP(ctx, 13, 0, 1, &fq));
+  II_CHECK(fieldP(ctx, 14, 0, 1, &ft));
+  II_CHECK(fieldP(ctx, 15, 0, nshift, &fp));
+  size_t he = half_elems(g);
+  double2* p = (double2*)fp.v;
+  II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int i = 0; i < nshift; ++i) {
+    II_CHECK(copy64(ctx, p + i * he, b));
+    II_CHECK(zero64(ctx, x + i * he));
+  }
+  II_CHECK(copy64(ctx, (double2*)fr.v, b));
+  II_CHECK(dot64(ctx, b, b, aux));
+  II_CHECK(dot64(ctx, b, b, aux + 1));
+  II_HIP(ctx, hipMemcpyAsync(aux + 2, shifts, nshift * sizeof(double), hipMemcpyHostToDevice,
+                             ctx->stream));
+  hipLaunchKernelGGL(k_ms_init, dim3(1), dim3(64), 0, ctx->stream, S, aux, aux + 2, nshift, prec);
+  II_HIP(ctx, hipGetLastError());
+  const int* skip = &S->done;
+  int queued = 0, chunk = 1;
+  bool conv = false;
+  while (queued < maxiter) {
+    int n = std::min(chunk, maxiter - queued);
+    for (int it = 0; it < n; ++it) {
+      II_CHECK(ddagdP(ctx, 0, kappa, SF{p}, fs, fq, ft, nullptr, nullptr, 1, skip, 0));
+      hipLaunchKernelGGL(k_ms_sp, dim3(nb), dim3(kBlock), 0, ctx->stream, g, (double2*)fs.v, p, S,
+                         part);
+      hipLaunchKernelGGL(k_ms_fin_pap, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
+      hipLaunchKernelGGL(k_ms_xr, dim3(nb), dim3(kBlock), 0, ctx->stream, g, x, p,
+                         (double2*)fr.v, (const double2*)fs.v, S, part);
+      hipLaunchKernelGGL(k_ms_fin_rr, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
+      hipLaunchKernelGGL(k_ms_vec, dim3(nb, nshift), dim3(kBlock), 0, ctx->stream, g, x, p,
+                         (const double2*)fr.v, S, part);
+      hipLaunchKernelGGL(k_ms_tail, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
+      II_HIP(ctx, hipGetLastError());
+    }
+    queued += n;
+    II_HIP(ctx, hipMemcpyAsync(Sh, S, sizeof(MSScal), hipMemcpyDeviceToHost, ctx->stream));
+    II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (Sh->nan) return set_error(ctx, II_ERR_NAN, "non-finite multishift scalar");
+    if (Sh->done) { conv = true; break; }
+    chunk = std::min(chunk * 2, 64);
+  }
+  // true residuals (:165-177): diff = max_i |(A + sigma_i) x_i - b|^2 / |b|^2
+  double dmax = -1.0;
+  if (per_shift) per_shift->assign(nshift, 0.0);
+  for (int i = 0; i < nshift; ++i) {
+    II_CHECK(ddagdP(ctx, 0, kappa, SF{x + i * he}, fs, fq, ft, nullptr, nullptr, 1, nullptr, 0));
+    II_CHECK(axpby64(ctx, shifts[i], x + i * he, 1.0, (double2*)fs.v));
+    II_CHECK(axpby64(ctx, -1.0, b, 1.0, (double2*)fs.v));
+    II_CHECK(dot64(ctx, (double2*)fs.v, (double2*)fs.v, aux));
+    double h = 0.0;
+    II_HIP(ctx, hipMemcpyAsync(&h, aux, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double d1 = h * Sh->snorm;
+    if (per_shift) (*per_shift)[i] = std::sqrt(d1);
+    if (d1 > dmax) dmax = d1;
+  }
+  II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0.f;
+  II_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (out) {
+    out->iterations = conv ? Sh->nconv : -1;
+    out->diff = dmax;
+    out->true_residual = std::sqrt(dmax);
+    out->timing_ms = ms;
+    out->restarts = 0;
+    out->inner_iterations = Sh->count;
+  }
+  if (!conv) return set_error(ctx, II_ERR_NOT_CONVERGED, "Not converged.");
+  return II_OK;
+}
+
+// ============================================================ propagator
+// Work slots 40..63 belong to the propagator solver.
+ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
+                     const ii_field* src, ii_field* sol, ii_solver_output* outs) {
+  const Geom& g = ctx->g;
+  const int nrhs_all = src->nrhs;
+  const int P = prm->precision;
+  if (P < 0 || P > 3) return set_error(ctx, II_ERR_BAD_ARG, "bad precision");
+  if (P != II_PREC_DOUBLE && !ctx->gauge_lp[P]) II_CHECK(build_lowprec_gauge(ctx, P));
+  const int batch = prm->batched ? nrhs_all : 1;
+  const double tol = prm->tol > 0 ? prm->tol : 1e-10;
+  const double delta = P == II_PREC_DOUBLE ? 0.0 : (prm->delta > 0 ? prm->delta : 0.1);
+  const int maxiter = prm->maxiter > 0 ? prm->maxiter : 100000;
+  const size_t he = half_elems(g);
+  const int nb = hop_blocks(g);
+  II_CHECK(ensure_partials(ctx, (size_t)nb * batch + 16));
+  II_CHECK(ensure_ru_scalars(ctx, batch));
+  double* part = ctx->partials;
+  RUScal* sc = dev_scal(ctx);
+  const int ss = kScalInts;
+  const int* skip_it = &sc[0].done;
+  const int* skip_ru = &sc[0].skip_ru;
+  void* auxp = nullptr;
+  II_CHECK(workspace(ctx, 40, sizeof(double) * 2 * batch + 64, &auxp));
+  double* aux = (double*)auxp;
+  // FP64 fields
+  SF be, bo, bh, y, r64, q64, t64, rho;
+  II_CHECK(fieldP(ctx, 41, 0, batch, &be));
+  II_CHECK(fieldP(ctx, 42, 0, batch, &bo));
+  II_CHECK(fieldP(ctx, 43, 0, batch, &bh));
+  II_CHECK(fieldP(ctx, 44, 0, batch, &y));
+  II_CHECK(fieldP(ctx, 45, 0, batch, &r64));
+  II_CHECK(fieldP(ctx, 46, 0, batch, &q64));
+  II_CHECK(fieldP(ctx, 47, 0, batch, &t64));
+  rho = q64;  // q64 is free once rho is formed
+  // sloppy fields
+  SF ps, rs, ss_, qs, ts, ys;
+  const bool lo = P != II_PREC_DOUBLE;
+  const int ytype = lo ? II_PREC_SINGLE : II_PREC_DOUBLE;
+  if (lo) {
+    II_CHECK(fieldP(ctx, 50, P, batch, &ps));
+    II_CHECK(fieldP(ctx, 51, P, batch, &rs));
+    II_CHECK(fieldP(ctx, 52, P, batch, &ss_));
+    II_CHECK(fieldP(ctx, 53, P, batch, &qs));
+    II_CHECK(fieldP(ctx, 54, P, batch, &ts));
+    II_CHECK(fieldP(ctx, 55, II_PREC_SINGLE, batch, &ys));
+  } else {
+    II_CHECK(fieldP(ctx, 50, 0, batch, &ps));
+    rs = r64;
+    II_CHECK(fieldP(ctx, 52, 0, batch, &ss_));
+    qs = q64;
+    ts = t64;
+    ys = y;
+  }
+  HopSpec h;
+  for (int r0 = 0; r0 < nrhs_all; r0 += batch) {
+    const int nr = std::min(batch, nrhs_all - r0);
+    auto t0 = std::chrono::steady_clock::now();
+    II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    // gather even/odd parts of the sources
+    for (int k = 0; k < nr; ++k) {
+      const double2* base = src->data + (size_t)(r0 + k) * 2 * he;
+      II_CHECK(copy64(ctx, (double2*)be.v + k * he, base));
+      II_CHECK(copy64(ctx, (double2*)bo.v + k * he, base + he));
+    }
+    // |b|^2 of the full sources
+    II_CHECK(dot64(ctx, (double2*)be.v, (double2*)be.v, aux, nr));
+    II_CHECK(dot64(ctx, (double2*)bo.v, (double2*)bo.v, aux + batch, nr));
+    // bhat = b_e - mult_eo(b_o) = b_e + kappa H_eo b_o   (eoUtils.cpp:54-57)
+    h = HopSpec();
+    h.out_par = 0;
+    h.a = kappa;
+    h.xpay = true;
+    h.nrhs = nr;
+    II_CHECK(hop64(ctx, h, (double2*)bo.v, (double2*)bh.v, (double2*)be.v, nullptr));
+    // c = Dhat^dag bhat  -> r64 ; y = 0
+    II_CHECK(dhat_dag64(ctx, kappa, (double2*)bh.v, (double2*)r64.v, (double2*)t64.v, nullptr,
+                        nullptr, nr));
+    II_CHECK(zero64(ctx, (double2*)y.v, nr));
+    // rr = |c|^2 into aux[2*batch ...]
+    II_CHECK(dot64(ctx, (double2*)r64.v, (double2*)r64.v, aux + 2 * batch, nr));
+    // b2 = |b_e|^2 + |b_o|^2 -> aux[0..nr)
+    {
+      std::vector<double> hb(2 * batch);
+      II_HIP(ctx, hipMemcpyAsync(hb.data(), aux, 2 * batch * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+      II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      for (int k = 0; k < nr; ++k) hb[k] += hb[batch + k];
+      II_HIP(ctx, hipMemcpyAsync(aux, hb.data(), nr * sizeof(double), hipMemcpyHostToDevice,
+                                 ctx->stream));
+      II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    hipLaunchKernelGGL(k_scal_init, dim3(nr), dim3(64), 0, ctx->stream, sc, aux + 2 * batch, aux,
+                       0.0, tol, delta, 1);
+    II_HIP(ctx, hipGetLastError());
+    if (lo) {
+      II_CHECK(convertP(ctx, 0, P, r64, rs, nr));
+      II_CHECK(convertP(ctx, 0, P, r64, ps, nr));
+      II_CHECK(convertP(ctx, 0, II_PREC_SINGLE, y, ys, nr));  // zeros
+    } else {
+      II_CHECK(copy64(ctx, (double2*)ps.v, (double2*)r64.v, nr));
+    }
+    int its = 0;
+    bool all_done = false;
+    int chunk = 1;
+    while (its < maxiter && !all_done) {
+      for (int i = 0; i < chunk; ++i) {
+        II_CHECK(ddagdP(ctx, P, kappa, ps, ss_, qs, ts, &ps, part, nr, skip_it, ss));
+        hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc);
+        II_CHECK(cg_blas_P(ctx, P, ps, ss_, rs, ys, ytype, sc, nr, part, nb));
+        hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc, 1);
+        II_CHECK(cg_p_P(ctx, P, rs, ps, sc, nr));
+      }
+      its += chunk;
+      II_CHECK(read_scalars(ctx, nr));
+      RUScal* H = host_scal(ctx);
+      bool any_upd = false;
+      all_done = true;
+      for (int k = 0; k < nr; ++k) {
+        if (H[k].nan) return set_error(ctx, II_ERR_NAN, "non-finite solver scalar");
+        if (!H[k].done) all_done = false;
+        if (!H[k].done && !H[k].skip_ru) any_upd = true;
+      }
+      if (all_done) break;
+      if (any_upd) {
+        // reliable update, FP64, for the flagged rhs only
+        if (lo) {
+          const Geom& gg = g;
+          hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(gg.Vh), nr), dim3(kBlock), 0,
+                             ctx->stream, gg, Acc<1>::sp(ys, gg.stride), Acc<1>::st(ys, gg.stride),
+                             (double2*)y.v, skip_ru, ss);
+          II_HIP(ctx, hipGetLastError());
+        }
+        // q = Dhat y ; rho = bhat - q ; |rho|^2
+        h = HopSpec();
+        h.nrhs = nr;
+        h.skip = skip_ru;
+        h.skip_stride = ss;
+        h.out_par = 1;
+        h.a = 1.0;
+        II_CHECK(hop64(ctx, h, (double2*)y.v, (double2*)t64.v, nullptr, nullptr));
+        h.out_par = 0;
+        h.a = -kappa * kappa;
+        h.xpay = true;
+        II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)q64.v, (double2*)y.v, nullptr));
+        hipLaunchKernelGGL(k_resid, dim3(nb, nr), dim3(kBlock), 0, ctx->stream, g,
+                           (const double2*)bh.v, (const double2*)q64.v, (double2*)rho.v, skip_ru,
+                           ss, part);
+        hipLaunchKernelGGL(k_fin_rho, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc);
+        // r64 = Dhat^dag rho ; |r64|^2
+        h = HopSpec();
+        h.nrhs = nr;
+        h.skip = skip_ru;
+        h.skip_stride = ss;
+        h.out_par = 1;
+        h.a = 1.0;
+        h.g5in = true;
+        II_CHECK(hop64(ctx, h, (double2*)rho.v, (double2*)t64.v, nullptr, nullptr));
+        h.out_par = 0;
+        h.a = -kappa * kappa;
+        h.xpay = true;
+        h.g5in = false;
+        h.g5out = true;
+        if (lo) {
+          II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)r64.v, (double2*)rho.v, nullptr));
+        } else {
+          // in FP64 the sloppy residual is r64 itself
+          II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)r64.v, (double2*)rho.v, nullptr));
+        }
+        hipLaunchKernelGGL(k_norm2_masked, dim3(nb, nr), dim3(kBlock), 0, ctx->stream, g,
+                           (const double2*)r64.v, skip_ru, ss, part);
+        if (lo) II_CHECK(convertP(ctx, 0, P, r64, rs, nr, skip_ru, ss));
+        hipLaunchKernelGGL(k_fin_ru, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc);
+        II_HIP(ctx, hipGetLastError());
+        chunk = 1;
+      } else {
+        chunk = std::min(chunk * 2, 8);
+      }
+    }
+    // fold the sloppy accumulator in (rhs not yet folded at convergence)
+    if (lo) {
+      hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(g.Vh), nr), dim3(kBlock), 0,
+                         ctx->stream, g, Acc<1>::sp(ys, g.stride), Acc<1>::st(ys, g.stride),
+                         (double2*)y.v, nullptr, 0);
+      II_HIP(ctx, hipGetLastError());
+    }
+    // x_o = b_o - mult_oe(y) = b_o + kappa H_oe y   (eoUtils.cpp:67)
+    h = HopSpec();
+    h.out_par = 1;
+    h.a = kappa;
+    h.xpay = true;
+    h.nrhs = nr;
+    II_CHECK(hop64(ctx, h, (double2*)y.v, (double2*)t64.v, (double2*)bo.v, nullptr));
+    for (int k = 0; k < nr; ++k) {
+      double2* base = sol->data + (size_t)(r0 + k) * 2 * he;
+      II_CHECK(copy64(ctx, base, (double2*)y.v + k * he));
+      II_CHECK(copy64(ctx, base + he, (double2*)t64.v + k * he));
+    }
+    II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    II_CHECK(read_scalars(ctx, nr));
+    float ms = 0.f;
+    II_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    (void)t0;
+    RUScal* H = host_scal(ctx);
+    bool failed = false;
+    for (int k = 0; k < nr; ++k) {
+      ii_solver_output& o = outs[r0 + k];
+      o.iterations = H[k].count;
+      o.inner_iterations = lo ? H[k].count : 0;
+      o.restarts = H[k].updates;
+      o.diff = std::sqrt(H[k].rho2 / H[k].b2);
+      o.timing_ms = ms;
+      o.true_residual = -1.0;
+      if (!H[k].done) failed = true;
+    }
+    if (failed)
+      return set_error(ctx, II_ERR_NOT_CONVERGED, "propagator solve did not reach the tolerance");
+  }
+  return II_OK;
+}
+
+// true residual |b - D x|/|b| per rhs of full fields (FP64)
+ii_status full_residual(ii_context* ctx, double kappa, const ii_field* src, const ii_field* sol,
+                        double* res) {
+  const Geom& g = ctx->g;
+  const size_t he = half_elems(g);
+  double2 *w, *aux2;
+  {
+    void* p;
+    II_CHECK(workspace(ctx, 56, he * sizeof(double2), &p));
+    w = (double2*)p;
+    II_CHECK(workspace(ctx, 57, 8 * sizeof(double), &p));
+    aux2 = (double2*)p;
+  }
+  double* d = (double*)aux2;
+  for (int k = 0; k < src->nrhs; ++k) {
+    const double2* se = src->data + (size_t)k * 2 * he;
+    const double2* xe = sol->data + (size_t)k * 2 * he;
+    double acc_r = 0.0, acc_b = 0.0;
+    for (int par = 0; par < 2; ++par) {
+      HopSpec h;
+      h.out_par = par;
+      h.a = -kappa;
+      h.xpay = true;
+      // w = x_par - kappa H x_other = (D x)_par
+      II_CHECK(hop64(ctx, h, xe + (1 - par) * he, w, xe + par * he, nullptr));
+      II_CHECK(axpby64(ctx, 1.0, se + par * he, -1.0, w));
+      II_CHECK(dot64(ctx, w, w, d));
+      II_CHECK(dot64(ctx, se + par * he, se + par * he, d + 1));
+      double hh[2];
+      II_HIP(ctx, hipMemcpyAsync(hh, d, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      acc_r += hh[0];
+      acc_b += hh[1];
+    }
+    res[k] = std::sqrt(acc_r / acc_b);
+  }
+  return II_OK;
+}
+
+}  // namespace ii
+
+using namespace ii;
+
+extern "C" {
+
+ii_status ii_solver_cg(ii_context* ctx, double kappa, double* x, const double* b, double prec,
+                       int maxiter, ii_solver_output* out) {
+  if (!x || !b || maxiter < 1) return II_ERR_BAD_ARG;
+  II_CHECK(require_gauge(ctx));
+  (void)hipSetDevice(ctx->device);
+  const Geom& g = ctx->g;
+  ii_field *fb = nullptr, *fx = nullptr;
+  II_CHECK(ii_field_create(ctx, II_EVEN, 1, &fb));
+  struct Del { ii_field* f; ~Del() { if (f) ii_field_destroy(f); } } d1{fb};
+  II_CHECK(ii_field_create(ctx, II_EVEN, 1, &fx));
+  Del d2{fx};
+  II_CHECK(ii_field_upload(fb, 0, b));
+  ii_status st = cg_reference(ctx, kappa, fx->data, fb->data, prec, maxiter, out);
+  if (st != II_OK && st != II_ERR_NOT_CONVERGED) return st;
+  std::string msg = ctx->err;
+  II_CHECK(ii_field_download(fx, 0, x));
+  ctx->err = msg;
+  (void)g;
+  return st;
+}
+
+ii_status ii_multishift_cg(ii_context* ctx, double kappa, double* x, const double* b,
+                           const double* shifts, int nshift, double prec, int maxiter,
+                           ii_solver_output* out) {
+  if (!x || !b || !shifts || nshift < 1 || maxiter < 1) return II_ERR_BAD_ARG;
+  II_CHECK(require_gauge(ctx));
+  (void)hipSetDevice(ctx->device);
+  const Geom& g = ctx->g;
+  ii_field *fb = nullptr, *fx = nullptr;
+  II_CHECK(ii_field_create(ctx, II_EVEN, 1, &fb));
+  struct Del { ii_field* f; ~Del() { if (f) ii_field_destroy(f); } } d1{fb};
+  II_CHECK(ii_field_create(ctx, II_EVEN, nshift, &fx));
+  Del d2{fx};
+  II_CHECK(ii_field_upload(fb, 0, b));
+  ii_status st = multishift_reference(ctx, kappa, fx->data, fb->data, shifts, nshift, prec,
+                                      maxiter, out, nullptr);
+  if (st != II_OK && st != II_ERR_NOT_CONVERGED) return st;
+  std::string msg = ctx->err;
+  for (int i = 0; i < nshift; ++i) II_CHECK(ii_field_download(fx, i, x + (size_t)24 * g.Vh * i));
+  ctx->err = msg;
+  return st;
+}
+
+ii_status ii_eo_invert(ii_context* ctx, double kappa, double* sol, const double* src, double prec,
+                       int maxiter, ii_solver_output* out) {
+  if (!sol || !src || maxiter < 1) return II_ERR_BAD_ARG;
+  II_CHECK(require_gauge(ctx));
+  (void)hipSetDevice(ctx->device);
+  const Geom& g = ctx->g;
+  const size_t he = half_elems(g);
+  ii_field *fs = nullptr, *fx = nullptr, *fb = nullptr, *fy = nullptr;
+  II_CHECK(ii_field_create(ctx, II_FULL, 1, &fs));
+  struct Del { ii_field* f; ~Del() { if (f) ii_field_destroy(f); } } d1{fs};
+  II_CHECK(ii_field_create(ctx, II_FULL, 1, &fx));
+  Del d2{fx};
+  II_CHECK(ii_field_create(ctx, II_EVEN, 1, &fb));
+  Del d3{fb};
+  II_CHECK(ii_field_create(ctx, II_EVEN, 1, &fy));
+  Del d4{fy};
+  II_CHECK(ii_field_upload(fs, 0, src));
+  double2* se = fs->data;
+  double2* so = fs->data + he;
+  // be = b_e - mult_eo(b_o) = b_e + kappa H_eo b_o     (eoUtils.cpp:54-57)
+  HopSpec h;
+  h.out_par = 0;
+  h.a = kappa;
+  h.xpay = true;
+  II_CHECK(hop64(ctx, h, so, fy->data, se, nullptr));
+  // rhs = mult_dag(be)
+  double2* t = nullptr;
+  {
+    void* p;
+    II_CHECK(workspace(ctx, 20, he * sizeof(double2), &p));
+    t = (double2*)p;
+  }
+  II_CHECK(dhat_dag64(ctx, kappa, fy->data, fb->data, t));
+  ii_status st = cg_reference(ctx, kappa, fx->data, fb->data, prec, maxiter, out);
+  if (st != II_OK && st != II_ERR_NOT_CONVERGED) return st;
+  std::string msg = ctx->err;
+  // x_o = b_o - mult_oe(y_e) = b_o + kappa H_oe y_e    (eoUtils.cpp:67)
+  h.out_par = 1;
+  h.a = kappa;
+  h.xpay = true;
+  II_CHECK(hop64(ctx, h, fx->data, fx->data + he, so, nullptr));
+  II_CHECK(ii_field_download(fx, 0, sol));
+  ctx->err = msg;
+  return st;
+}
+
+ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_params* p,
+                              const ii_field* src, ii_field* sol, ii_solver_output* out) {
+  if (!p || !src || !sol || !out) return II_ERR_BAD_ARG;
+  if (src->subset != II_FULL || sol->subset != II_FULL || src->nrhs != sol->nrhs)
+    return set_error(ctx, II_ERR_BAD_ARG, "propagator fields must be full and of equal nrhs");
+  II_CHECK(require_gauge(ctx));
+  (void)hipSetDevice(ctx->device);
+  ii_status st;
+  if (p->use_learned) {
+    st = learned_propagator(ctx, kappa, p, src, sol, out);
+  } else {
+    st = propagator(ctx, kappa, p, src, sol, out);
+  }
+  if (st != II_OK && st != II_ERR_NOT_CONVERGED) return st;
+  std::string msg = ctx->err;
+  std::vector<double> res(src->nrhs);
+  II_CHECK(full_residual(ctx, kappa, src, sol, res.data()));
+  for (int k = 0; k < src->nrhs; ++k) out[k].true_residual = res[k];
+  ctx->err = msg;
+  return st;
+}
+
+ii_status ii_bench_hop(ii_context* ctx, int precision, int nrhs, int nrep, double* avg_ms) {
+  if (!avg_ms || nrep < 1 || nrhs < 1) return II_ERR_BAD_ARG;
+  II_CHECK(require_gauge(ctx));
+  (void)hipSetDevice(ctx->device);
+  const int P = precision;
+  if (P != II_PREC_DOUBLE && !ctx->gauge_lp[P]) II_CHECK(build_lowprec_gauge(ctx, P));
+  SF a, b;
+  II_CHECK(fieldP(ctx, 58, P, nrhs, &a));
+  II_CHECK(fieldP(ctx, 59, P, nrhs, &b));
+  HopSpec h;
+  h.out_par = 1;
+  h.a = -0.1355;
+  h.nrhs = nrhs;
+  for (int i = 0; i < 3; ++i) II_CHECK(hopP(ctx, P, h, a, b, nullptr, nullptr));
+  II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int i = 0; i < nrep; ++i) II_CHECK(hopP(ctx, P, h, a, b, nullptr, nullptr));
+  II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  II_HIP(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  II_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  *avg_ms = ms / nrep;
+  return II_OK;
+}
+
+}  // extern "C"

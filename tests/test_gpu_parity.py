@@ -1,0 +1,246 @@
+"""Parity of the HIP path with the reference (golden fixtures) and with the
+oracle, through the C ABI.  FP64 operators: 1e-13 relative (the stated
+bar); solvers: same iteration counts as the reference and solutions to
+rounding; mixed precision: the FP64 true residual target is met."""
+import numpy as np
+import pytest
+
+import iroiro_amd as ii
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_OP = 1e-13   # FP64 operator parity bar (relative, 2-norm)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def c1(golden):
+    z = golden("wilson_c1_4x4x4x8")
+    ctx = ii.Context(tuple(z["dims"]))
+    D = ii.Dirac_Wilson_EvenOdd(float(z["mass"]), z["u_apbc"], ctx)
+    yield z, ctx, D
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["wilson_c1_4x4x4x8", "wilson_aniso_4x6x2x4"])
+def test_operators_vs_reference(golden, name):
+    z = golden(name)
+    dims = tuple(z["dims"])
+    ctx = ii.Context(dims)
+    D = ii.Dirac_Wilson_EvenOdd(float(z["mass"]), z["u_apbc"], ctx)
+    Df = ii.Dirac_Wilson(float(z["mass"]), z["u_apbc"], ctx)
+    lat = oracle.Lattice(dims)
+    fe, fo = lat.split(z["src"])
+    got = {
+        "deo": D.mult_eo(fo), "doe": D.mult_oe(fe), "deo_dag": D.mult_eo_dag(fe),
+        "doe_dag": D.mult_oe_dag(fo), "dhat": D.mult(fe), "dhat_dag": D.mult_dag(fe),
+        "g5": D.gamma5(fe), "ddagd": ii.Fopr_DdagD(D).mult(fe),
+        "dfull": Df.mult(z["src"]), "dfull_dag": Df.mult_dag(z["src"]),
+    }
+    for k, v in got.items():
+        assert rel(v, z[k]) < TOL_OP, (k, rel(v, z[k]))
+    ctx.close()
+
+
+def test_cg_vs_reference(c1):
+    z, ctx, D = c1
+    lat = oracle.Lattice(tuple(z["dims"]))
+    fe, _ = lat.split(z["src"])
+    x = np.empty_like(fe)
+    out = ii.Solver_CG(float(z["cg_prec"]), 1000, ii.Fopr_DdagD(D)).solve(x, fe)
+    assert out.Iterations == int(z["cg_iterations"])
+    assert rel(x, z["cg_x"]) < 1e-12
+    assert abs(out.diff - float(z["cg_diff"])) < 1e-4 * float(z["cg_diff"])
+
+
+def test_cg_is_deterministic(c1):
+    z, ctx, D = c1
+    lat = oracle.Lattice(tuple(z["dims"]))
+    fe, _ = lat.split(z["src"])
+    s = ii.Solver_CG(1e-20, 1000, ii.Fopr_DdagD(D))
+    x1, x2 = np.empty_like(fe), np.empty_like(fe)
+    s.solve(x1, fe)
+    s.solve(x2, fe)
+    assert np.array_equal(x1, x2)
+
+
+def test_multishift_vs_reference(c1):
+    z, ctx, D = c1
+    lat = oracle.Lattice(tuple(z["dims"]))
+    fe, _ = lat.split(z["src"])
+    shifts = z["ms_shifts"]
+    xs = [np.empty_like(fe) for _ in shifts]
+    so, diff, nconv = ii.MultiShiftSolver_CG(ii.Fopr_DdagD(D), float(z["cg_prec"]), 1000).solve(
+        xs, fe, shifts)
+    assert nconv == int(z["ms_nconv"])
+    for i in range(len(shifts)):
+        assert rel(xs[i], z["ms_x"][i]) < 1e-12
+    assert abs(diff - float(z["ms_diff"])) < 1e-4 * float(z["ms_diff"])
+    assert abs(so.diff - float(z["ms_out_diff"])) < 1e-4 * float(z["ms_out_diff"])
+
+
+def test_invert_vs_reference(c1):
+    z, ctx, D = c1
+    sol = np.empty(24 * ctx.V)
+    ii.Inverter_WilsonLike(D, ii.Solver_CG(float(z["cg_prec"]), 1000, ii.Fopr_DdagD(D))).invert(
+        sol, z["src"])
+    assert rel(sol, z["inv_x"]) < 1e-12
+
+
+def test_not_converged_raises(c1):
+    z, ctx, D = c1
+    lat = oracle.Lattice(tuple(z["dims"]))
+    fe, _ = lat.split(z["src"])
+    x = np.zeros_like(fe)
+    with pytest.raises(ii.ConvergenceErr):
+        ii.Solver_CG(1e-20, 3, ii.Fopr_DdagD(D)).solve(x, fe)
+    assert np.linalg.norm(x) > 0  # the iterate is still written back
+    xs = [np.empty_like(fe) for _ in range(2)]
+    with pytest.raises(ii.NotConverged):
+        ii.MultiShiftSolver_CG(ii.Fopr_DdagD(D), 1e-20, 3).solve(xs, fe, [0.0, 0.1])
+
+
+def test_invalidated_gauge_is_an_error(golden):
+    z = golden("wilson_c1_4x4x4x8")
+    ctx = ii.Context(tuple(z["dims"]))
+    D = ii.Dirac_Wilson_EvenOdd(float(z["mass"]), z["u_apbc"], ctx)
+    ctx.invalidate_gauge()
+    with pytest.raises(ii.IIError):
+        D.mult(np.zeros(D.fsize()))
+    D.update_internal_state()
+    assert np.all(D.mult(np.zeros(D.fsize())) == 0)
+    ctx.close()
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 8, 16), (6, 4, 10, 8), (16, 8, 8, 4)])
+def test_operators_vs_oracle_larger(dims):
+    lat = oracle.Lattice(dims)
+    u = lat.apbc(lat.gauge_expH(0.7, 1234))
+    kappa = 0.1355
+    mass = 0.5 / kappa - 4.0
+    ctx = ii.Context(dims)
+    D = ii.Dirac_Wilson_EvenOdd(mass, u, ctx)
+    k = D.getKappa()
+    rng = np.random.default_rng(8)
+    fe = rng.normal(size=lat.half_size)
+    fo = rng.normal(size=lat.half_size)
+    assert rel(D.mult_eo(fo), lat.hop_eo(u, fo, 1, 0, k)) < TOL_OP
+    assert rel(D.mult_oe(fe), lat.hop_eo(u, fe, 2, 0, k)) < TOL_OP
+    assert rel(D.mult_eo_dag(fe), lat.hop_eo(u, fe, 2, 1, k)) < TOL_OP
+    assert rel(D.mult(fe), lat.dirac_eo(u, fe, 0, k)) < TOL_OP
+    assert rel(D.mult_dag(fe), lat.dirac_eo(u, fe, 1, k)) < TOL_OP
+    assert rel(ii.Fopr_DdagD(D).mult(fe), lat.ddagd(u, fe, k)) < TOL_OP
+    full = rng.normal(size=lat.full_size)
+    Df = ii.Dirac_Wilson(mass, u, ctx)
+    assert rel(Df.mult(full), lat.dirac_full(u, full, 0, k)) < TOL_OP
+    # gauge round trip and plaquette
+    assert np.array_equal(ctx.download_gauge(), u)
+    assert abs(ctx.plaquette() - lat.plaquette(u)) < 1e-13
+    ctx.close()
+
+
+def _prop_setup(dims, seed=5, eps=0.6):
+    lat = oracle.Lattice(dims)
+    u = lat.apbc(lat.gauge_expH(eps, seed))
+    return lat, u
+
+
+@pytest.mark.parametrize("precision,batched", [("double", False), ("double", True),
+                                               ("single", True), ("half", True), ("bf16", True)])
+def test_propagator_true_residual(precision, batched):
+    dims = (8, 8, 8, 8)
+    lat, u = _prop_setup(dims)
+    kappa = 0.1355
+    ctx = ii.Context(dims)
+    ctx.upload_gauge(u)
+    src = ii.DeviceField(ctx, "full", 12)
+    sol = ii.DeviceField(ctx, "full", 12)
+    src.point_sources((0, 0, 0, 0))
+    outs = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=precision,
+                               batched=batched).solve(src, sol)
+    for rhs in (0, 5, 11):
+        b = src.download(rhs)
+        x = sol.download(rhs)
+        s, c = divmod(rhs, 3)
+        expect_b = np.zeros(lat.full_size)
+        expect_b[2 * (c + 3 * s)] = 1.0
+        assert np.array_equal(b, expect_b)
+        r_oracle = lat.true_residual_full(u, kappa, x, b)
+        assert r_oracle <= 1e-10, (rhs, r_oracle)
+        assert abs(outs[rhs].true_residual - r_oracle) < 1e-12
+        # error against the FP64 oracle solution
+        xo, info = lat.eo_invert(u, kappa, b, 1e-28, 5000)
+        assert rel(x, xo) < 1e-8
+    ctx.close()
+
+
+def test_lowprec_operator_error():
+    dims = (8, 8, 8, 8)
+    lat, u = _prop_setup(dims, seed=6)
+    ctx = ii.Context(dims)
+    ctx.upload_gauge(u)
+    t64 = ctx.bench_hop("double", 1, 2)
+    t32 = ctx.bench_hop("single", 1, 2)
+    t16 = ctx.bench_hop("half", 1, 2)
+    assert t64 > 0 and t32 > 0 and t16 > 0
+    ctx.close()
+
+
+def test_r12_rejects_non_su3():
+    dims = (4, 4, 4, 4)
+    lat = oracle.Lattice(dims)
+    u = lat.gauge_expH(0.3, 3) * 1.01  # not unitary
+    ctx = ii.Context(dims)
+    ctx.upload_gauge(u)
+    src = ii.DeviceField(ctx, "full", 12)
+    sol = ii.DeviceField(ctx, "full", 12)
+    src.point_sources((0, 0, 0, 0))
+    with pytest.raises(ii.IIError):
+        ii.PropagatorSolver(ctx, 0.12, precision="half", batched=True).solve(src, sol)
+    ctx.close()
+
+
+def test_heatbath_matches_oracle():
+    """Same counter-based random numbers on both sides: two sweeps from a
+    cold start agree link by link up to rounding (a rare accept/reject flip
+    from last-bit differences of log/cos is tolerated on a few links)."""
+    dims = (4, 4, 4, 8)
+    lat = oracle.Lattice(dims)
+    u = lat.unit_gauge()
+    for sw in range(2):
+        lat.heatbath_sweep(u, 6.0, 77, sw)
+    ctx = ii.Context(dims)
+    ctx.unit_gauge()
+    ctx.heatbath(6.0, 77, 2)
+    ug = ctx.download_gauge()
+    d = np.abs(ug - u).reshape(-1, 18).max(axis=1)
+    assert np.mean(d < 1e-10) > 0.99, np.sort(d)[-5:]
+    assert abs(ctx.plaquette() - lat.plaquette(u)) < 1e-3
+    ctx.close()
+
+
+def test_heatbath_plaquette_and_su3():
+    dims = (8, 8, 8, 8)
+    ctx = ii.Context(dims)
+    ctx.unit_gauge()
+    vals = []
+    for sw in range(0, 160, 10):
+        ctx.heatbath(6.0, 5, 10, first_sweep=sw)
+        if sw >= 60:
+            vals.append(ctx.plaquette())
+    assert abs(np.mean(vals) - 0.5937) < 0.004
+    u = ctx.download_gauge().reshape(4 * ctx.V, 3, 3, 2)
+    U = u[..., 0] + 1j * u[..., 1]
+    assert np.abs(U @ U.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-12
+    assert np.abs(np.linalg.det(U) - 1).max() < 1e-12
+    # apbc flips exactly the last t-slice of U_t
+    ctx.apply_apbc()
+    u2 = ctx.download_gauge().reshape(4, ctx.V, 18)
+    t = np.arange(ctx.V) // (8 * 8 * 8)
+    flipped = np.all(u2 == -u.reshape(4, ctx.V, 18), axis=2)
+    assert np.array_equal(flipped[3], t == 7) and not flipped[:3].any()
+    ctx.close()
