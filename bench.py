@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=20260601)
+    ap.add_argument("--hop-only", action="store_true",
+                    help="only time the hop kernels and the copy reference")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (one MI355X per box); "
@@ -130,6 +132,26 @@ def main():
     log("heat-bath %d sweeps beta=%.2f: plaquette %.5f (%.1f s)"
         % (cfg["sweeps"], cfg["beta"], plaq, time.time() - t0))
 
+    if args.hop_only:
+        # kernel-level measurement only (used under rocprofv3): every hop
+        # variant of the solve path, and the copy reference
+        rows = {}
+        for prec, nrhs, bpsr in [("double", 1, 1536.0), ("double", 12, 384 + 1152 / 12.0),
+                                 ("single", 12, 192 + 384 / 12.0), ("half", 12, 104 + 192 / 12.0),
+                                 ("bf16", 12, 104 + 192 / 12.0)]:
+            ms = ctx.bench_hop(prec, nrhs, 20)
+            rows["%s_x%d" % (prec, nrhs)] = {
+                "ms": ms, "alg_gbs": bpsr * ctx.Vh * nrhs / (ms * 1e-3) / 1e9,
+                "site_rhs_per_s": ctx.Vh * nrhs / (ms * 1e-3)}
+            log("hop %-6s x%-2d %.4f ms  %.0f GB/s algorithmic" % (prec, nrhs, ms,
+                                                                   rows["%s_x%d" % (prec, nrhs)]["alg_gbs"]))
+        nbytes = 2 << 30
+        rows["copy_gbs"] = 2 * nbytes / (ctx.bench_copy(nbytes, 10) * 1e-3) / 1e9
+        log("copy %.0f GB/s" % rows["copy_gbs"])
+        print(json.dumps({"hop_only": rows, "lattice": list(dims)}), flush=True)
+        ctx.close()
+        return
+
     src = ii.DeviceField(ctx, "full", 12)
     sol = ii.DeviceField(ctx, "full", 12)
     src.point_sources((0, 0, 0, 0))
@@ -140,8 +162,11 @@ def main():
     hop_gbs = FP64_HOP_BYTES_PER_SITE * Vh / (hop_ms * 1e-3) / 1e9
     hop_lp_ms = ctx.bench_hop(args.precision, 12, 20)
     lp_bytes = (104 + 192 / 12.0) * Vh * 12  # 16-bit spinor+norm in/out, r12 links shared
-    log("FP64 hop %.3f ms = %.0f GB/s (%.1f%% of HBM peak); %s 12-rhs hop %.3f ms"
-        % (hop_ms, hop_gbs, 100 * hop_gbs / HBM_PEAK_GBS, args.precision, hop_lp_ms))
+    copy_bytes = 2 << 30
+    copy_gbs = 2 * copy_bytes / (ctx.bench_copy(copy_bytes, 10) * 1e-3) / 1e9
+    log("FP64 hop %.3f ms = %.0f GB/s (%.1f%% of HBM peak); %s 12-rhs hop %.3f ms; "
+        "plain copy %.0f GB/s" % (hop_ms, hop_gbs, 100 * hop_gbs / HBM_PEAK_GBS, args.precision,
+                                  hop_lp_ms, copy_gbs))
 
     # ---- plain FP64 GPU CG, rhs after rhs (reference Qprop_EvenOdd shape)
     fp64 = None
@@ -207,7 +232,8 @@ def main():
                      "kernel": "k_hop FP64 even/odd Wilson hop, V/2 = %d sites/launch, "
                                "1536 B/site algorithmic" % Vh,
                      "avg_ms": hop_ms,
-                     "gflops": HOP_FLOPS_PER_SITE * Vh / (hop_ms * 1e-3) / 1e9},
+                     "gflops": HOP_FLOPS_PER_SITE * Vh / (hop_ms * 1e-3) / 1e9,
+                     "copy_kernel_gbs": copy_gbs},
         "roofline_batched_lowprec": {"kernel": "k_hop %s 12-rhs" % args.precision,
                                      "avg_ms": hop_lp_ms,
                                      "achieved": lp_bytes / (hop_lp_ms * 1e-3) / 1e9,

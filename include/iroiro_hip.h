@@ -181,6 +181,10 @@ ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_param
  * with HIP events on the context stream, over nrep launches after a
  * warm-up; nrhs > 1 times the fused multi-rhs hop.  avg_ms per launch. */
 ii_status ii_bench_hop(ii_context* ctx, int precision, int nrhs, int nrep, double* avg_ms);
+/* Times a plain device copy of `bytes` bytes (read + write) with HIP
+ * events: the achievable-bandwidth reference for the roofline and the
+ * calibration kernel of the FETCH_SIZE / WRITE_SIZE counters. */
+ii_status ii_bench_copy(ii_context* ctx, size_t bytes, int nrep, double* avg_ms);
 
 #ifdef __cplusplus
 }

@@ -142,6 +142,13 @@ class Context:
         self.check(_lib.load().ii_device_info(self._h, buf, 256, ctypes.byref(g)), "ii_device_info")
         return buf.value.decode(), g.value
 
+    def bench_copy(self, nbytes, nrep=20):
+        """Average ms of a plain device copy of nbytes (read + write)."""
+        ms = ctypes.c_double()
+        self.check(_lib.load().ii_bench_copy(self._h, int(nbytes), int(nrep), ctypes.byref(ms)),
+                   "ii_bench_copy")
+        return ms.value
+
     def bench_hop(self, precision="double", nrhs=1, nrep=50):
         ms = ctypes.c_double()
         self.check(_lib.load().ii_bench_hop(self._h, PRECISIONS[precision], nrhs, nrep,

@@ -27,7 +27,7 @@ EXPORTS = [
     "ii_dirac_mult", "ii_dirac_mult_dag", "ii_dirac_gamma5", "ii_fopr_ddagd_mult",
     "ii_dirac_full_mult", "ii_solver_cg", "ii_multishift_cg", "ii_eo_invert",
     "ii_field_create", "ii_field_destroy", "ii_field_upload", "ii_field_download",
-    "ii_field_point_sources", "ii_propagator_solve", "ii_bench_hop",
+    "ii_field_point_sources", "ii_propagator_solve", "ii_bench_hop", "ii_bench_copy",
 ]
 
 
@@ -107,6 +107,7 @@ def load():
         "ii_field_point_sources": [vp, ip],
         "ii_propagator_solve": [vp, cd, ctypes.POINTER(PropParamsC), vp, vp, so],
         "ii_bench_hop": [vp, ci, ci, ci, dp],
+        "ii_bench_copy": [vp, ctypes.c_size_t, ci, dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -58,6 +58,13 @@ ii_status ensure_scalars(ii_context* ctx, int n) {
   return II_OK;
 }
 
+ii_status enter(ii_context* ctx) {
+  if (!ctx) return II_ERR_BAD_ARG;
+  II_HIP(ctx, hipSetDevice(ctx->device));
+  (void)hipGetLastError();
+  return II_OK;
+}
+
 ii_status require_gauge(ii_context* ctx) {
   if (!ctx) return II_ERR_BAD_ARG;
   if (!ctx->gauge_valid)
@@ -123,6 +130,9 @@ ii_status ii_context_create(ii_context** out, const int dims[4], int device) {
   g.sZ = g.Lxh * dims[1];
   g.sT = g.Lxh * dims[1] * dims[2];
   g.stride = (g.Vh + 63) / 64 * 64;
+  // kernels address each array with 32-bit buffer offsets; the largest
+  // array indexed that way is the FP64 gauge field, 72 components
+  if ((unsigned long long)72 * g.stride * sizeof(double2) >= (1ull << 32)) return II_ERR_BAD_ARG;
   // default-priority stream (no priority streams on this platform)
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return II_ERR_DEVICE;
@@ -140,6 +150,12 @@ ii_status ii_context_destroy(ii_context* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   learned_release(ctx);
+  for (ii_field* f : ctx->fields) {
+    (void)hipFree(f->data);
+    f->data = nullptr;
+    f->ctx = nullptr;  // detached: ii_field_destroy only frees the handle
+  }
+  ctx->fields.clear();
   for (auto& b : ctx->work)
     if (b.ptr) (void)hipFree(b.ptr);
   for (int i = 0; i < 4; ++i)
@@ -172,7 +188,7 @@ ii_status ii_device_info(ii_context* ctx, char* name, size_t name_len, double* h
 
 ii_status ii_gauge_upload(ii_context* ctx, const double* u) {
   if (!ctx || !u) return II_ERR_BAD_ARG;
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   double* d = nullptr;
   II_CHECK(stage_in(ctx, u, (size_t)18 * 4 * ctx->g.V, &d));
   II_CHECK(pack_gauge(ctx, d));
@@ -198,14 +214,14 @@ ii_status ii_gauge_invalidate(ii_context* ctx) {
 ii_status ii_gauge_plaquette(ii_context* ctx, double* plaq) {
   if (!plaq) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   return plaquette64(ctx, plaq);
 }
 
 ii_status ii_gauge_download(ii_context* ctx, double* u) {
   if (!u) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   double* d = nullptr;
   size_t n = (size_t)18 * 4 * ctx->g.V;
   II_CHECK(stage_buf(ctx, n, &d));
@@ -218,7 +234,7 @@ static ii_status single_hop(ii_context* ctx, double kappa, double* out, const do
                             int out_par, bool dag) {
   if (!out || !in) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   double *din = nullptr, *dout = nullptr;
   double2 *a = nullptr, *b = nullptr;
@@ -253,7 +269,7 @@ ii_status ii_dirac_mult_oe_dag(ii_context* ctx, double kappa, double* out, const
 static ii_status even_op(ii_context* ctx, double kappa, double* out, const double* in, int which) {
   if (!out || !in) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   double *din = nullptr, *dout = nullptr;
   double2 *a, *b, *t, *q;
@@ -297,7 +313,7 @@ ii_status ii_dirac_full_mult(ii_context* ctx, double kappa, double* out, const d
                              int dag) {
   if (!out || !in) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   double *din = nullptr, *dout = nullptr;
   double2 *e, *o, *we, *wo;
@@ -325,7 +341,7 @@ ii_status ii_dirac_full_mult(ii_context* ctx, double kappa, double* out, const d
 // ------------------------------------------------------------ fields
 ii_status ii_field_create(ii_context* ctx, int subset, int nrhs, ii_field** f) {
   if (!ctx || !f || nrhs < 1 || subset < II_EVEN || subset > II_FULL) return II_ERR_BAD_ARG;
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   std::unique_ptr<ii_field> fl(new ii_field());
   fl->ctx = ctx;
   fl->subset = subset;
@@ -334,23 +350,27 @@ ii_status ii_field_create(ii_context* ctx, int subset, int nrhs, ii_field** f) {
   II_HIP(ctx, hipMalloc(&fl->data, n * sizeof(double2)));
   II_HIP(ctx, hipMemsetAsync(fl->data, 0, n * sizeof(double2), ctx->stream));
   II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->fields.insert(fl.get());
   *f = fl.release();
   return II_OK;
 }
 
 ii_status ii_field_destroy(ii_field* f) {
   if (!f) return II_ERR_BAD_ARG;
-  (void)hipSetDevice(f->ctx->device);
-  (void)hipStreamSynchronize(f->ctx->stream);
-  (void)hipFree(f->data);
+  if (f->ctx) {
+    (void)hipSetDevice(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    (void)hipFree(f->data);
+    f->ctx->fields.erase(f);
+  }
   delete f;
   return II_OK;
 }
 
 ii_status ii_field_upload(ii_field* f, int rhs, const double* host) {
-  if (!f || !host || rhs < 0 || rhs >= f->nrhs) return II_ERR_BAD_ARG;
+  if (!f || !f->ctx || !host || rhs < 0 || rhs >= f->nrhs) return II_ERR_BAD_ARG;
   ii_context* ctx = f->ctx;
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   size_t he = half_elems(g);
   double* d = nullptr;
@@ -367,9 +387,9 @@ ii_status ii_field_upload(ii_field* f, int rhs, const double* host) {
 }
 
 ii_status ii_field_download(const ii_field* f, int rhs, double* host) {
-  if (!f || !host || rhs < 0 || rhs >= f->nrhs) return II_ERR_BAD_ARG;
+  if (!f || !f->ctx || !host || rhs < 0 || rhs >= f->nrhs) return II_ERR_BAD_ARG;
   ii_context* ctx = f->ctx;
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   size_t he = half_elems(g);
   double* d = nullptr;
@@ -385,12 +405,12 @@ ii_status ii_field_download(const ii_field* f, int rhs, double* host) {
 }
 
 ii_status ii_field_point_sources(ii_field* f, const int site[4]) {
-  if (!f || !site || f->subset != II_FULL || f->nrhs != 12) return II_ERR_BAD_ARG;
+  if (!f || !f->ctx || !site || f->subset != II_FULL || f->nrhs != 12) return II_ERR_BAD_ARG;
   ii_context* ctx = f->ctx;
   const Geom& g = ctx->g;
   for (int m = 0; m < 4; ++m)
     if (site[m] < 0 || site[m] >= g.L[m]) return II_ERR_BAD_ARG;
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   size_t he = half_elems(g);
   II_HIP(ctx, hipMemsetAsync(f->data, 0, he * 2 * 12 * sizeof(double2), ctx->stream));
   int par = (site[0] + site[1] + site[2] + site[3]) & 1;

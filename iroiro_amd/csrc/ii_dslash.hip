@@ -64,7 +64,126 @@ k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
   }
 }
 
+// Multi-rhs hop.  A block is TS sites x blockDim.y right-hand sides.  All
+// threads first stage the tile's neighbour indices, time-boundary flags
+// and its 8 links per site in LDS, so each link is read from HBM once per
+// site for all rhs; then thread (tx, ty) applies the stencil to site tx of
+// rhs ty (one wavefront per rhs for TS = 64).  Partial dot products are
+// per (rhs, tile): partials[rhs * gridDim.x + blockIdx.x].
+template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Gauge,
+          typename StOut, typename SpX, int TS, int MAXT>
+__global__ void __launch_bounds__(MAXT)
+k_hop_mrhs(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
+           typename SpIn::R a, bool tbc_flip, double* __restrict__ partials,
+           const int* __restrict__ skip, int skip_stride, int nrhs) {
+  using C = typename SpIn::C;
+  using UC = typename Gauge::C;
+  __shared__ UC lnk[8 * 9 * TS];  // [direction][row*3+col][site], full matrices
+  __shared__ int nbi[8][TS];      // neighbour half index per direction
+  __shared__ int flg[TS];         // bit 0: t+ link flipped, bit 1: t- link flipped
+  const int ntile = gridDim.x;
+  const int tile = xcd_remap(blockIdx.x, ntile);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = tx + ty * blockDim.x;
+  const int hs0 = tile * TS;
+  const int op = 1 - out_par;
+  for (int i = tid; i < TS; i += nthr) {
+    const int hs = hs0 + i;
+    if (hs < g.Vh) {
+      const SiteCoord c = decode_site(g, hs, out_par);
+#pragma unroll
+      for (int mu = 0; mu < 4; ++mu) {
+        nbi[2 * mu][i] = neighbour(g, c, hs, mu, true);
+        nbi[2 * mu + 1][i] = neighbour(g, c, hs, mu, false);
+      }
+      flg[i] = (tbc_flip && c.t == g.L[3] - 1 ? 1 : 0) | (tbc_flip && c.t == 0 ? 2 : 0);
+    }
+  }
+  __syncthreads();
+  // one thread per (direction, site): load the stored link, rebuild the full
+  // 3x3 matrix once (reconstruct-12, time-boundary sign) in the compute
+  // precision, so the rhs threads only read it
+  for (int i = tid; i < 8 * TS; i += nthr) {
+    const int s = i % TS;
+    const int d = __builtin_amdgcn_readfirstlane(i / TS);  // one direction per wavefront
+    const int hs = hs0 + s;
+    if (hs < g.Vh) {
+      UC U[3][3];
+      const bool fl = (d == 6 && (flg[s] & 1)) || (d == 7 && (flg[s] & 2));
+      gauge.ld((d & 1) ? op : out_par, d >> 1, (d & 1) ? nbi[d][s] : hs, fl, U);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) lnk[(d * 9 + k) * TS + s] = U[k / 3][k % 3];
+    }
+  }
+  __syncthreads();
+  const int hs = hs0 + tx;
+  // TS = 64 = one wavefront per rhs, so rhs is wave-uniform and the rhs
+  // base pointers live in scalar registers
+  static_assert(TS == 64, "one wavefront per rhs row");
+  const int rhs = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + ty);
+  const bool live = rhs < nrhs && !(skip && skip[rhs * skip_stride]);
+  double dsum = 0.0;
+  if (live && hs < g.Vh) {
+    const SpIn inr = in.rhs(rhs);
+    C acc[4][3];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[s][c] = cmk<C>(0, 0);
+#define II_MDIR(MU, FWD, D)                                                                 \
+  hop_dir_with<MU, FWD, G5IN, UC>(acc, inr, nbi[D][tx], [&](UC U[3][3]) {                  \
+    _Pragma("unroll") for (int k = 0; k < 9; ++k) U[k / 3][k % 3] = lnk[(D * 9 + k) * TS + tx]; \
+  });                                                                                       \
+  pin_acc(acc);
+    II_MDIR(0, true, 0)
+    II_MDIR(0, false, 1)
+    II_MDIR(1, true, 2)
+    II_MDIR(1, false, 3)
+    II_MDIR(2, true, 4)
+    II_MDIR(2, false, 5)
+    II_MDIR(3, true, 6)
+    II_MDIR(3, false, 7)
+#undef II_MDIR
+    C o[12];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int so = G5OUT ? (s ^ 2) : s;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[3 * so + c] = cscale(acc[s][c], a);
+    }
+    if (XPAY) {
+      const SpX xr = x.rhs(rhs);
+      const auto xs = xr.scale(hs);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        auto xv = xr.ld(hs, k);
+        o[k].x += (typename SpIn::R)(xv.x * xs);
+        o[k].y += (typename SpIn::R)(xv.y * xs);
+      }
+    }
+    if (DOT) {
+      const SpX dr = dv.rhs(rhs);
+      const double ds = (double)dr.scale(hs);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        auto dvv = dr.ld(hs, k);
+        dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
+      }
+    }
+    out.rhs(rhs).st(hs, o);
+  }
+  if (DOT) {
+    for (int off = TS / 2; off > 0; off >>= 1) dsum += __shfl_down(dsum, off, TS);
+    if (tx == 0 && live) partials[(size_t)rhs * ntile + blockIdx.x] = dsum;
+  }
+}
+
 namespace {
+
+// sites per tile of the multi-rhs kernel and its thread limit
+template <typename Gauge> struct MrhsShape { static constexpr int TS = 64, MAXT = 768; };
+template <> struct MrhsShape<GaugeF64> { static constexpr int TS = 64, MAXT = 768; };
 
 template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Gauge,
           typename StOut, typename SpX>
@@ -72,11 +191,20 @@ ii_status launch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, const 
                      const SpX& x, const SpX& dv, int out_par, typename SpIn::R a, int nrhs,
                      double* partials, const int* skip, int skip_stride) {
   const Geom& g = ctx->g;
-  dim3 grid(blocks_for(g.Vh), nrhs);
   bool flip = ctx->tbc_sign < 0;
-  hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX>), grid,
-                     dim3(kBlock), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a, flip,
-                     partials, skip, skip_stride);
+  if (nrhs == 1) {
+    dim3 grid(blocks_for(g.Vh), 1);
+    hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX>), grid,
+                       dim3(kBlock), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a, flip,
+                       partials, skip, skip_stride);
+  } else {
+    constexpr int TS = MrhsShape<Gauge>::TS, MAXT = MrhsShape<Gauge>::MAXT;
+    const int rb = std::min(nrhs, MAXT / TS);
+    dim3 grid((g.Vh + TS - 1) / TS, (nrhs + rb - 1) / rb);
+    hipLaunchKernelGGL((k_hop_mrhs<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX, TS, MAXT>),
+                       grid, dim3(TS, rb), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a,
+                       flip, partials, skip, skip_stride, nrhs);
+  }
   II_HIP(ctx, hipGetLastError());
   return II_OK;
 }
@@ -109,6 +237,13 @@ ii_status dispatch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, cons
 }
 
 }  // namespace
+
+int hop_dot_parts(const Geom& g, int nrhs, int precision) {
+  if (nrhs == 1) return blocks_for(g.Vh);
+  const int ts = precision == II_PREC_DOUBLE ? MrhsShape<GaugeF64>::TS
+                                             : MrhsShape<GaugeF32r12>::TS;
+  return (g.Vh + ts - 1) / ts;
+}
 
 ii_status hop64(ii_context* ctx, const HopSpec& h, const double2* in, double2* out,
                 const double2* x, const double2* dv) {

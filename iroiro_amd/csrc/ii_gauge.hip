@@ -272,7 +272,7 @@ extern "C" {
 
 ii_status ii_gauge_unit(ii_context* ctx) {
   if (!ctx) return II_ERR_BAD_ARG;
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   hipLaunchKernelGGL(k_unit_gauge, dim3(blocks_for(g.Vh), 8), dim3(kBlock), 0, ctx->stream,
                      ctx->gauge64, g);
@@ -287,7 +287,7 @@ ii_status ii_gauge_heatbath(ii_context* ctx, double beta, unsigned long long see
                             int first_sweep, int nsweeps) {
   if (!ctx || beta <= 0.0 || nsweeps < 0 || first_sweep < 0) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   for (int sw = first_sweep; sw < first_sweep + nsweeps; ++sw)
     for (int mu = 0; mu < 4; ++mu)
@@ -303,7 +303,7 @@ ii_status ii_gauge_heatbath(ii_context* ctx, double beta, unsigned long long see
 
 ii_status ii_gauge_apply_apbc(ii_context* ctx) {
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   hipLaunchKernelGGL(k_apbc, dim3(blocks_for(g.Vh), 2), dim3(kBlock), 0, ctx->stream,
                      ctx->gauge64, g);

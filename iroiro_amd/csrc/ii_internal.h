@@ -19,6 +19,7 @@
 #include <hip/hip_bf16.h>
 
 #include <cstdint>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -83,6 +84,9 @@ struct ii_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // learned preconditioner state (owned by ii_learned.hip)
   void* learned = nullptr;
+  // live device fields of this context: destroying the context releases
+  // their memory and detaches them, so a later ii_field_destroy is safe
+  std::set<ii_field*> fields;
 };
 
 struct ii_field {
@@ -115,6 +119,9 @@ ii_status workspace(ii_context* ctx, int slot, size_t bytes, void** ptr);
 ii_status ensure_partials(ii_context* ctx, size_t n);
 ii_status ensure_scalars(ii_context* ctx, int n);
 ii_status require_gauge(ii_context* ctx);
+// Entry of every API call on a context: selects its device and clears a
+// stale HIP error left by an earlier, already reported failure.
+ii_status enter(ii_context* ctx);
 
 inline int blocks_for(long n, int block = kBlock) {
   return (int)((n + block - 1) / block);

@@ -40,6 +40,8 @@ ii_status hop_bf16(ii_context* ctx, const HopSpec& h, const LoField& in, const L
                    const LoField* x, const LoField* dv);
 
 inline int hop_blocks(const Geom& g) { return blocks_for(g.Vh); }
+// number of per-rhs partial sums a DOT hop writes for nrhs right-hand sides
+int hop_dot_parts(const Geom& g, int nrhs, int precision);
 
 // ------------------------------------------------------------- FP64 operators
 // Dhat f = f - Deo Doe f (dirac_wilson_EvenOdd.hpp:79-84); t: odd temp.

@@ -868,7 +868,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
   const int maxiter = prm->maxiter > 0 ? prm->maxiter : 100000;
   const size_t he = half_elems(g);
   const int nb = hop_blocks(g);
-  II_CHECK(ensure_partials(ctx, (size_t)nb * batch + 16));
+  II_CHECK(ensure_partials(ctx, (size_t)std::max(nb, hop_dot_parts(g, batch, P)) * batch + 16));
   II_CHECK(ensure_ru_scalars(ctx, batch));
   double* part = ctx->partials;
   RUScal* sc = dev_scal(ctx);
@@ -961,7 +961,8 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
     while (its < maxiter && !all_done) {
       for (int i = 0; i < chunk; ++i) {
         II_CHECK(ddagdP(ctx, P, kappa, ps, ss_, qs, ts, &ps, part, nr, skip_it, ss));
-        hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc);
+        hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
+                           hop_dot_parts(g, nr, P), sc);
         II_CHECK(cg_blas_P(ctx, P, ps, ss_, rs, ys, ytype, sc, nr, part, nb));
         hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc, 1);
         II_CHECK(cg_p_P(ctx, P, rs, ps, sc, nr));
@@ -1113,6 +1114,14 @@ ii_status full_residual(ii_context* ctx, double kappa, const ii_field* src, cons
   return II_OK;
 }
 
+// plain streaming copy (bandwidth reference), grid-stride
+__global__ void __launch_bounds__(256)
+k_copy_stream(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 }  // namespace ii
 
 using namespace ii;
@@ -1123,7 +1132,7 @@ ii_status ii_solver_cg(ii_context* ctx, double kappa, double* x, const double* b
                        int maxiter, ii_solver_output* out) {
   if (!x || !b || maxiter < 1) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   ii_field *fb = nullptr, *fx = nullptr;
   II_CHECK(ii_field_create(ctx, II_EVEN, 1, &fb));
@@ -1145,7 +1154,7 @@ ii_status ii_multishift_cg(ii_context* ctx, double kappa, double* x, const doubl
                            ii_solver_output* out) {
   if (!x || !b || !shifts || nshift < 1 || maxiter < 1) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   ii_field *fb = nullptr, *fx = nullptr;
   II_CHECK(ii_field_create(ctx, II_EVEN, 1, &fb));
@@ -1166,7 +1175,7 @@ ii_status ii_eo_invert(ii_context* ctx, double kappa, double* sol, const double*
                        int maxiter, ii_solver_output* out) {
   if (!sol || !src || maxiter < 1) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   const size_t he = half_elems(g);
   ii_field *fs = nullptr, *fx = nullptr, *fb = nullptr, *fy = nullptr;
@@ -1211,10 +1220,11 @@ ii_status ii_eo_invert(ii_context* ctx, double kappa, double* sol, const double*
 ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_params* p,
                               const ii_field* src, ii_field* sol, ii_solver_output* out) {
   if (!p || !src || !sol || !out) return II_ERR_BAD_ARG;
-  if (src->subset != II_FULL || sol->subset != II_FULL || src->nrhs != sol->nrhs)
+  if (src->ctx != ctx || sol->ctx != ctx || src->subset != II_FULL || sol->subset != II_FULL ||
+      src->nrhs != sol->nrhs)
     return set_error(ctx, II_ERR_BAD_ARG, "propagator fields must be full and of equal nrhs");
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   ii_status st;
   if (p->use_learned) {
     st = learned_propagator(ctx, kappa, p, src, sol, out);
@@ -1230,10 +1240,35 @@ ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_param
   return st;
 }
 
+ii_status ii_bench_copy(ii_context* ctx, size_t bytes, int nrep, double* avg_ms) {
+  if (!ctx || !avg_ms || nrep < 1 || bytes < 16) return II_ERR_BAD_ARG;
+  II_CHECK(enter(ctx));
+  void *a = nullptr, *b = nullptr;
+  II_CHECK(workspace(ctx, 60, bytes, &a));
+  II_CHECK(workspace(ctx, 61, bytes, &b));
+  size_t n = bytes / sizeof(double4);
+  const int block = 256;
+  const int grid = (int)std::min<size_t>((n + block - 1) / block, 1u << 20);
+  hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(block), 0, ctx->stream,
+                     (const double4*)a, (double4*)b, n);
+  II_HIP(ctx, hipGetLastError());
+  II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  for (int i = 0; i < nrep; ++i)
+    hipLaunchKernelGGL(k_copy_stream, dim3(grid), dim3(block), 0, ctx->stream,
+                       (const double4*)a, (double4*)b, n);
+  II_HIP(ctx, hipGetLastError());
+  II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  II_HIP(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  II_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  *avg_ms = ms / nrep;
+  return II_OK;
+}
+
 ii_status ii_bench_hop(ii_context* ctx, int precision, int nrhs, int nrep, double* avg_ms) {
   if (!avg_ms || nrep < 1 || nrhs < 1) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
-  (void)hipSetDevice(ctx->device);
+  II_CHECK(enter(ctx));
   const int P = precision;
   if (P != II_PREC_DOUBLE && !ctx->gauge_lp[P]) II_CHECK(build_lowprec_gauge(ctx, P));
   SF a, b;

@@ -15,20 +15,80 @@
 // costs no extra pass.  Storage is abstracted by accessor types so that the
 // same stencil runs on FP64, FP32 and FP16/BF16-with-norm fields and on
 // r18/r12 links.
+//
+// Addressing: every access is a raw buffer load/store with the array's
+// descriptor in scalar registers, the component offset as the scalar
+// offset and the site as a 32-bit per-lane offset; this keeps the per-load
+// integer work to one shift (64-bit per-lane address arithmetic cost ~20%
+// of the VALU instructions of the 16-bit multi-rhs hop).
 #pragma once
 
 #include "ii_internal.h"
 
 namespace ii {
 
+// ---------------------------------------------------------------- addressing
+// Raw buffer access: one descriptor per array (scalar registers), the
+// component offset as the scalar offset and the site as a 32-bit lane
+// offset.  Third descriptor word of the gfx9 family (raw, dword data).
+constexpr int kBufWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xFFFFFFFF,
+                                           kBufWord3);
+}
+
+// element `off + idx` of base (off uniform, idx per lane); the bytes are
+// moved with memcpy because __half2 / __hip_bfloat162 are not trivially
+// copyable
+using U32x2 = uint32_t __attribute__((ext_vector_type(2)));
+using U32x4 = uint32_t __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* __restrict__ base, uint32_t off, uint32_t idx) {
+  const uint32_t vo = idx * (uint32_t)sizeof(T), so = off * (uint32_t)sizeof(T);
+  T r;
+  if constexpr (sizeof(T) == 4) {
+    uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rsrc(base), vo, so, 0);
+    __builtin_memcpy((void*)&r, &v, 4);
+  } else if constexpr (sizeof(T) == 8) {
+    U32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), vo, so, 0);
+    __builtin_memcpy((void*)&r, &v, 8);
+  } else {
+    static_assert(sizeof(T) == 16, "4, 8 or 16-byte elements");
+    U32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), vo, so, 0);
+    __builtin_memcpy((void*)&r, &v, 16);
+  }
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T* __restrict__ base, uint32_t off, uint32_t idx, T v) {
+  const uint32_t vo = idx * (uint32_t)sizeof(T), so = off * (uint32_t)sizeof(T);
+  if constexpr (sizeof(T) == 4) {
+    uint32_t w;
+    __builtin_memcpy(&w, &v, 4);
+    __builtin_amdgcn_raw_buffer_store_b32(w, rsrc(base), vo, so, 0);
+  } else if constexpr (sizeof(T) == 8) {
+    U32x2 w;
+    __builtin_memcpy(&w, &v, 8);
+    __builtin_amdgcn_raw_buffer_store_b64(w, rsrc(base), vo, so, 0);
+  } else {
+    U32x4 w;
+    __builtin_memcpy(&w, &v, 16);
+    __builtin_amdgcn_raw_buffer_store_b128(w, rsrc(base), vo, so, 0);
+  }
+}
+
 // ---------------------------------------------------------------- accessors
 // FP64 half spinor, [12][stride] double2.
 struct SpF64 {
   using R = double;
   using C = double2;
+  static constexpr bool kScaled = false;
   const double2* __restrict__ p;
   int stride;
-  __device__ __forceinline__ C ld(int site, int comp) const { return p[comp * stride + site]; }
+  __device__ __forceinline__ C ld(int site, int comp) const {
+    return ld_at(p, (uint32_t)(comp * stride), (uint32_t)site);
+  }
   __device__ __forceinline__ R scale(int) const { return 1.0; }
   __device__ __forceinline__ SpF64 rhs(int r) const { return SpF64{p + (size_t)r * 12 * stride, stride}; }
 };
@@ -39,7 +99,7 @@ struct StF64 {
   int stride;
   __device__ __forceinline__ void st(int site, const C v[12]) const {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) p[k * stride + site] = v[k];
+    for (int k = 0; k < 12; ++k) st_at(p, (uint32_t)(k * stride), (uint32_t)site,v[k]);
   }
   __device__ __forceinline__ StF64 rhs(int r) const { return StF64{p + (size_t)r * 12 * stride, stride}; }
 };
@@ -48,9 +108,12 @@ struct StF64 {
 struct SpF32 {
   using R = float;
   using C = float2;
+  static constexpr bool kScaled = false;
   const float2* __restrict__ p;
   int stride;
-  __device__ __forceinline__ C ld(int site, int comp) const { return p[comp * stride + site]; }
+  __device__ __forceinline__ C ld(int site, int comp) const {
+    return ld_at(p, (uint32_t)(comp * stride), (uint32_t)site);
+  }
   __device__ __forceinline__ R scale(int) const { return 1.0f; }
   __device__ __forceinline__ SpF32 rhs(int r) const { return SpF32{p + (size_t)r * 12 * stride, stride}; }
 };
@@ -61,17 +124,25 @@ struct StF32 {
   int stride;
   __device__ __forceinline__ void st(int site, const C v[12]) const {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) p[k * stride + site] = v[k];
+    for (int k = 0; k < 12; ++k) st_at(p, (uint32_t)(k * stride), (uint32_t)site,v[k]);
   }
   __device__ __forceinline__ StF32 rhs(int r) const { return StF32{p + (size_t)r * 12 * stride, stride}; }
 };
 
 // 16-bit half spinor with a per-site FP32 norm: value = stored * norm[site],
 // stored components in [-1, 1].  T = __half2 or __hip_bfloat162.
-template <typename T> __device__ __forceinline__ float2 lo_to_f2(T v);
-template <> __device__ __forceinline__ float2 lo_to_f2<__half2>(__half2 v) { return __half22float2(v); }
-template <> __device__ __forceinline__ float2 lo_to_f2<__hip_bfloat162>(__hip_bfloat162 v) {
-  return make_float2(__bfloat162float(v.x), __bfloat162float(v.y));
+template <typename T> __device__ __forceinline__ float lo_re(T v);
+template <typename T> __device__ __forceinline__ float lo_im(T v);
+template <> __device__ __forceinline__ float lo_re<__half2>(__half2 v) { return __low2float(v); }
+template <> __device__ __forceinline__ float lo_im<__half2>(__half2 v) { return __high2float(v); }
+template <> __device__ __forceinline__ float lo_re<__hip_bfloat162>(__hip_bfloat162 v) {
+  return __bfloat162float(v.x);
+}
+template <> __device__ __forceinline__ float lo_im<__hip_bfloat162>(__hip_bfloat162 v) {
+  return __bfloat162float(v.y);
+}
+template <typename T> __device__ __forceinline__ float2 lo_to_f2(T v) {
+  return make_float2(lo_re<T>(v), lo_im<T>(v));
 }
 template <typename T> __device__ __forceinline__ T f2_to_lo(float2 v);
 template <> __device__ __forceinline__ __half2 f2_to_lo<__half2>(float2 v) { return __float22half2_rn(v); }
@@ -86,11 +157,16 @@ template <typename T>
 struct SpLo {
   using R = float;
   using C = float2;
+  using Raw = T;
+  static constexpr bool kScaled = true;
   const T* __restrict__ p;
   const float* __restrict__ nrm;
   int stride;
-  __device__ __forceinline__ C ld(int site, int comp) const { return lo_to_f2<T>(p[comp * stride + site]); }
-  __device__ __forceinline__ R scale(int site) const { return nrm[site]; }
+  __device__ __forceinline__ T raw(int site, int comp) const {
+    return ld_at(p, (uint32_t)(comp * stride), (uint32_t)site);
+  }
+  __device__ __forceinline__ C ld(int site, int comp) const { return lo_to_f2<T>(raw(site, comp)); }
+  __device__ __forceinline__ R scale(int site) const { return ld_at(nrm, 0u, (uint32_t)site); }
   __device__ __forceinline__ SpLo rhs(int r) const {
     return SpLo{p + (size_t)r * 12 * stride, nrm + (size_t)r * stride, stride};
   }
@@ -106,32 +182,25 @@ struct StLo {
     float m = 0.0f;
 #pragma unroll
     for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].x), fabsf(v[k].y)));
-    float inv = m > 0.0f ? 1.0f / m : 0.0f;
-    nrm[site] = m;
+    // Sites below 1e-30 are stored as zero: near the front of a point
+    // source's Krylov vectors the values underflow to FP32 denormals, whose
+    // reciprocal overflows.  They are far below the 16-bit resolution of
+    // any field the solver stores.
+    if (!(m > 1e-30f)) m = 0.0f;
+    const float inv = m > 0.0f ? 1.0f / m : 0.0f;
+    st_at(nrm, 0u, (uint32_t)site, m);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) p[k * stride + site] = f2_to_lo<T>(make_float2(v[k].x * inv, v[k].y * inv));
+    for (int k = 0; k < 12; ++k)
+      st_at(p, (uint32_t)(k * stride), (uint32_t)site,f2_to_lo<T>(make_float2(v[k].x * inv, v[k].y * inv)));
   }
   __device__ __forceinline__ StLo rhs(int r) const {
     return StLo{p + (size_t)r * 12 * stride, nrm + (size_t)r * stride, stride};
   }
 };
 
-// Links.  ld() returns the full 3x3 matrix U[row][col].
-struct GaugeF64 {
-  using R = double;
-  using C = double2;
-  const double2* __restrict__ p;  // [2][4][9][stride]
-  int stride;
-  __device__ __forceinline__ void ld(int par, int mu, int site, bool /*tflip*/, C U[3][3]) const {
-    const double2* b = p + (size_t)((par * 4 + mu) * 9) * stride + site;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) U[k / 3][k % 3] = b[(size_t)k * stride];
-  }
-};
-
 // Reconstruct-12: rows 0 and 1 stored, row 2 = conj(row0 x row1) * sign,
 // exact for SU(3) (unitary rows, det 1).  The antiperiodic time boundary
-// makes the last-slice t-links -SU(3); their sign is restored by tflip.
+// makes the last-slice t-links -SU(3); their sign is restored by flip.
 template <typename C>
 __device__ __forceinline__ void r12_third_row(C U[3][3], bool flip) {
 #pragma unroll
@@ -145,29 +214,44 @@ __device__ __forceinline__ void r12_third_row(C U[3][3], bool flip) {
   }
 }
 
+// Links.  ld() returns the full 3x3 matrix U[row][col] of the link that
+// starts at half site `site` of parity `par` in direction mu.
+struct GaugeF64 {
+  using R = double;
+  using C = double2;
+  const double2* __restrict__ p;  // [2][4][9][stride]
+  int stride;
+  __device__ __forceinline__ void ld(int par, int mu, int site, bool /*tflip*/, C U[3][3]) const {
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      U[k / 3][k % 3] = ld_at(p, (uint32_t)(((par * 4 + mu) * 9 + k) * stride), (uint32_t)site);
+  }
+};
+
 struct GaugeF32r12 {
   using R = float;
   using C = float2;
   const float2* __restrict__ p;  // [2][4][6][stride]
   int stride;
   __device__ __forceinline__ void ld(int par, int mu, int site, bool tflip, C U[3][3]) const {
-    const float2* b = p + (size_t)((par * 4 + mu) * 6) * stride + site;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) U[k / 3][k % 3] = b[(size_t)k * stride];
+    for (int k = 0; k < 6; ++k)
+      U[k / 3][k % 3] = ld_at(p, (uint32_t)(((par * 4 + mu) * 6 + k) * stride), (uint32_t)site);
     r12_third_row(U, tflip);
   }
 };
 
-template <typename T>
+template <typename TT>
 struct GaugeLoR12 {
   using R = float;
   using C = float2;
-  const T* __restrict__ p;  // [2][4][6][stride]
+  const TT* __restrict__ p;  // [2][4][6][stride]
   int stride;
   __device__ __forceinline__ void ld(int par, int mu, int site, bool tflip, C U[3][3]) const {
-    const T* b = p + (size_t)((par * 4 + mu) * 6) * stride + site;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) U[k / 3][k % 3] = lo_to_f2<T>(b[(size_t)k * stride]);
+    for (int k = 0; k < 6; ++k)
+      U[k / 3][k % 3] =
+          lo_to_f2<TT>(ld_at(p, (uint32_t)(((par * 4 + mu) * 6 + k) * stride), (uint32_t)site));
     r12_third_row(U, tflip);
   }
 };
@@ -187,23 +271,57 @@ __device__ __forceinline__ void pin_acc(C acc[4][3]) {
 }
 #define II_DIR_BARRIER() pin_acc(acc)
 
-// ---------------------------------------------------------------- stencil
-template <int MU, bool FWD, typename C>
-__device__ __forceinline__ void project(const C v[4][3], C w1[3], C w2[3]) {
+// ---------------------------------------------------------------- projection
+// w1 = v[A1] + c1 v[B1], w2 = v[A2] + c2 v[B2], with c in {+1, -1, +i, -i}
+// (codes 0..3), or w = 2 v[A] (code 4, the time direction, where
+// 1 -/+ gamma_t = 2 diag(0,0,1,1) / 2 diag(1,1,0,0)).
+template <int MU, bool FWD> struct Proj;
+template <> struct Proj<0, true>  { static constexpr int A1 = 0, B1 = 3, C1 = 2, A2 = 1, B2 = 2, C2 = 2; };
+template <> struct Proj<0, false> { static constexpr int A1 = 0, B1 = 3, C1 = 3, A2 = 1, B2 = 2, C2 = 3; };
+template <> struct Proj<1, true>  { static constexpr int A1 = 0, B1 = 3, C1 = 0, A2 = 1, B2 = 2, C2 = 1; };
+template <> struct Proj<1, false> { static constexpr int A1 = 0, B1 = 3, C1 = 1, A2 = 1, B2 = 2, C2 = 0; };
+template <> struct Proj<2, true>  { static constexpr int A1 = 0, B1 = 2, C1 = 2, A2 = 1, B2 = 3, C2 = 3; };
+template <> struct Proj<2, false> { static constexpr int A1 = 0, B1 = 2, C1 = 3, A2 = 1, B2 = 3, C2 = 2; };
+template <> struct Proj<3, true>  { static constexpr int A1 = 2, B1 = 2, C1 = 4, A2 = 3, B2 = 3, C2 = 4; };
+template <> struct Proj<3, false> { static constexpr int A1 = 0, B1 = 0, C1 = 4, A2 = 1, B2 = 1, C2 = 4; };
+
+template <int CODE, typename R>
+__device__ __forceinline__ void combine(R ax, R ay, R bx, R by, R n, R& wx, R& wy) {
+  // (a + c b) * n, written as one product and one fused multiply-add per
+  // real component so that 16-bit operands convert inside the FMA
+  if (CODE == 0) { wx = fma(ax, n, bx * n); wy = fma(ay, n, by * n); }
+  else if (CODE == 1) { wx = fma(ax, n, -(bx * n)); wy = fma(ay, n, -(by * n)); }
+  else if (CODE == 2) { wx = fma(ax, n, -(by * n)); wy = fma(ay, n, bx * n); }
+  else if (CODE == 3) { wx = fma(ax, n, by * n); wy = fma(ay, n, -(bx * n)); }
+  else { wx = ax * (n + n); wy = ay * (n + n); }
+}
+
+// Loads the neighbour spinor components the projector needs and forms the
+// two projected colour vectors (scaled by the per-site norm for 16-bit
+// storage).
+template <int MU, bool FWD, bool G5IN, typename SpIn>
+__device__ __forceinline__ void load_project(const SpIn& in, int nb, typename SpIn::C w1[3],
+                                             typename SpIn::C w2[3]) {
+  using R = typename SpIn::R;
+  using P = Proj<MU, FWD>;
+  constexpr int a1 = G5IN ? (P::A1 ^ 2) : P::A1, b1 = G5IN ? (P::B1 ^ 2) : P::B1;
+  constexpr int a2 = G5IN ? (P::A2 ^ 2) : P::A2, b2 = G5IN ? (P::B2 ^ 2) : P::B2;
+  const R n = in.scale(nb);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    if (MU == 0) {
-      if (FWD) { w1[c] = cadd_i(v[0][c], v[3][c]); w2[c] = cadd_i(v[1][c], v[2][c]); }
-      else     { w1[c] = csub_i(v[0][c], v[3][c]); w2[c] = csub_i(v[1][c], v[2][c]); }
-    } else if (MU == 1) {
-      if (FWD) { w1[c] = cadd(v[0][c], v[3][c]); w2[c] = csub(v[1][c], v[2][c]); }
-      else     { w1[c] = csub(v[0][c], v[3][c]); w2[c] = cadd(v[1][c], v[2][c]); }
-    } else if (MU == 2) {
-      if (FWD) { w1[c] = cadd_i(v[0][c], v[2][c]); w2[c] = csub_i(v[1][c], v[3][c]); }
-      else     { w1[c] = csub_i(v[0][c], v[2][c]); w2[c] = cadd_i(v[1][c], v[3][c]); }
+    auto va = in.ld(nb, 3 * a1 + c);
+    if (P::C1 == 4) {
+      combine<4, R>(va.x, va.y, R(0), R(0), n, w1[c].x, w1[c].y);
     } else {
-      if (FWD) { w1[c] = cadd(v[2][c], v[2][c]); w2[c] = cadd(v[3][c], v[3][c]); }
-      else     { w1[c] = cadd(v[0][c], v[0][c]); w2[c] = cadd(v[1][c], v[1][c]); }
+      auto vb = in.ld(nb, 3 * b1 + c);
+      combine<P::C1, R>(va.x, va.y, vb.x, vb.y, n, w1[c].x, w1[c].y);
+    }
+    auto vc = in.ld(nb, 3 * a2 + c);
+    if (P::C2 == 4) {
+      combine<4, R>(vc.x, vc.y, R(0), R(0), n, w2[c].x, w2[c].y);
+    } else {
+      auto vd = in.ld(nb, 3 * b2 + c);
+      combine<P::C2, R>(vc.x, vc.y, vd.x, vd.y, n, w2[c].x, w2[c].y);
     }
   }
 }
@@ -232,34 +350,18 @@ __device__ __forceinline__ void reconstruct(C acc[4][3], const C h1[3], const C 
   }
 }
 
-// One direction: load neighbour spinor (only the spins the projector
-// needs), project, multiply by U or U^dag, reconstruct into acc.
-template <int MU, bool FWD, bool G5IN, typename SpIn, typename Gauge>
-__device__ __forceinline__ void hop_dir(typename SpIn::C acc[4][3], const SpIn& in,
-                                        const Gauge& gauge, int nb, int lpar, int lsite,
-                                        bool tflip) {
+// One direction: project the neighbour spinor, multiply by U (forward) or
+// U^dag (backward), reconstruct into acc.  The link comes from getU(U)
+// (global memory or LDS).
+template <int MU, bool FWD, bool G5IN, typename UC, typename SpIn, typename GetU>
+__device__ __forceinline__ void hop_dir_with(typename SpIn::C acc[4][3], const SpIn& in, int nb,
+                                             GetU&& getU) {
   using C = typename SpIn::C;
   using R = typename SpIn::R;
-  C v[4][3];
-  // spins needed: t+ uses 2,3; t- uses 0,1; others all four
-  constexpr int s_lo = (MU == 3) ? (FWD ? 2 : 0) : 0;
-  constexpr int s_hi = (MU == 3) ? (FWD ? 4 : 2) : 4;
-#pragma unroll
-  for (int s = 0; s < 4; ++s)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (s >= s_lo && s < s_hi) {
-        const int sp = G5IN ? (s ^ 2) : s;
-        v[s][c] = in.ld(nb, 3 * sp + c);
-      } else {
-        v[s][c] = cmk<C>(R(0), R(0));
-      }
-    }
-  const R sc = in.scale(nb);
   C w1[3], w2[3];
-  project<MU, FWD>(v, w1, w2);
-  typename Gauge::C U[3][3];
-  gauge.ld(lpar, MU, lsite, tflip, U);
+  load_project<MU, FWD, G5IN>(in, nb, w1, w2);
+  UC U[3][3];
+  getU(U);
   C h1[3], h2[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -271,10 +373,16 @@ __device__ __forceinline__ void hop_dir(typename SpIn::C acc[4][3], const SpIn& 
       if (FWD) { cmac(h1[a], u, w1[b]); cmac(h2[a], u, w2[b]); }
       else     { cmacc(h1[a], u, w1[b]); cmacc(h2[a], u, w2[b]); }
     }
-    h1[a] = cscale(h1[a], sc);
-    h2[a] = cscale(h2[a], sc);
   }
   reconstruct<MU, FWD>(acc, h1, h2);
+}
+
+template <int MU, bool FWD, bool G5IN, typename SpIn, typename Gauge>
+__device__ __forceinline__ void hop_dir(typename SpIn::C acc[4][3], const SpIn& in,
+                                        const Gauge& gauge, int nb, int lpar, int lsite,
+                                        bool tflip) {
+  hop_dir_with<MU, FWD, G5IN, typename Gauge::C>(
+      acc, in, nb, [&](typename Gauge::C U[3][3]) { gauge.ld(lpar, MU, lsite, tflip, U); });
 }
 
 // The full 8-direction hop at half site hs of parity out_par.
@@ -293,9 +401,6 @@ __device__ __forceinline__ void wilson_hop_site(typename SpIn::C acc[4][3], cons
   const int op = 1 - out_par;
   const bool tf_fw = tbc_flip && sc.t == g.L[3] - 1;
   const bool tf_bw = tbc_flip && sc.t == 0;
-  // The scheduling barriers keep the compiler from hoisting the loads of
-  // all eight directions at once, which would need ~500 registers (FP64)
-  // and drop occupancy to one wave per SIMD.
   hop_dir<0, true, G5IN>(acc, in, gauge, neighbour(g, sc, hs, 0, true), out_par, hs, false);
   II_DIR_BARRIER();
   { int nb = neighbour(g, sc, hs, 0, false); hop_dir<0, false, G5IN>(acc, in, gauge, nb, op, nb, false); }
