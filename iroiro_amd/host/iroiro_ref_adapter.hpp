@@ -1,0 +1,225 @@
+// iroiro_ref_adapter.hpp -- drop-in classes for the IroIro++ source tree
+// that run the Wilson even/odd solve path on an MI355X through the C ABI of
+// libiroiro_hip.so (include/iroiro_hip.h).
+//
+// Include this header in the reference's build, after its own headers are
+// reachable (-I<tree> -I<tree>/include -I<tree>/lib), and link
+// -liroiro_hip.  The classes implement the reference's own abstract
+// interfaces, so its drivers call them unchanged:
+//
+//   Dirac_Wilson_EvenOdd_HIP : DiracWilsonLike_EvenOdd
+//       (lib/Dirac_ops/dirac_WilsonLike.hpp:66-82; replaces
+//        Dirac_Wilson_EvenOdd<Dirac_Wilson>, dirac_wilson_EvenOdd.hpp:12-92)
+//   Fopr_DdagD_HIP : Fopr_Herm          (lib/Fopr/fopr.h:72-77, 146-153)
+//   Solver_CG_HIP : Solver              (lib/Solver/solver.hpp:24-30;
+//                                        replaces Solver_CG, solver_CG.cpp:29-160)
+//   MultiShiftSolver_CG_HIP : MultiShiftSolver
+//                                       (lib/Solver/multiShiftSolver.hpp:15-24;
+//                                        replaces multiShiftSolver_CG.cpp:10-185)
+//
+// Inverter_WilsonLike (eoUtils.cpp:43-80), Qprop_EvenOdd
+// (qprop_EvenOdd.cpp:11-43), Action_Nf2 and RationalSolver_CG take these
+// through their base-class pointers.  Error behaviour follows the
+// reference: CG non-convergence calls Errors::ConvergenceErr (which exits),
+// the multishift solver throws "Not converged.", device failures abort with
+// the library's message.  The MD force, which is not on the device path,
+// is delegated to the reference's own CPU Dirac_Wilson_EvenOdd.
+#ifndef IROIRO_REF_ADAPTER_HPP
+#define IROIRO_REF_ADAPTER_HPP
+
+#include <cstdio>
+#include <cstdlib>
+#include <sstream>
+#include <typeinfo>
+#include <vector>
+
+#include "include/commonPrms.hpp"
+#include "include/errors.hpp"
+#include "include/field.h"
+#include "Dirac_ops/dirac_WilsonLike.hpp"
+#include "Dirac_ops/dirac_wilson.hpp"
+#include "Dirac_ops/dirac_wilson_EvenOdd.hpp"
+#include "Fopr/fopr.h"
+#include "Solver/solver.hpp"
+#include "Solver/multiShiftSolver.hpp"
+
+#include "iroiro_hip.h"
+
+namespace iroiro_hip_detail {
+
+inline void die(ii_context* ctx, ii_status st, const char* what) {
+  std::fprintf(stderr, "iroiro_hip: %s failed (status %d): %s\n", what, (int)st,
+               ctx ? ii_last_error(ctx) : "");
+  std::abort();
+}
+inline void check(ii_context* ctx, ii_status st, const char* what) {
+  if (st != II_OK) die(ctx, st, what);
+}
+inline double* addr(const Field& f) { return const_cast<Field&>(f).getaddr(0); }
+
+// One device context per process and lattice (the GPU side of CommonPrms).
+inline ii_context* context() {
+  static ii_context* ctx = 0;
+  if (!ctx) {
+    int dims[4] = {CommonPrms::Nx(), CommonPrms::Ny(), CommonPrms::Nz(), CommonPrms::Nt()};
+    ii_status st = ii_context_create(&ctx, dims, 0);
+    if (st != II_OK) die(0, st, "ii_context_create");
+  }
+  return ctx;
+}
+
+}  // namespace iroiro_hip_detail
+
+class Dirac_Wilson_EvenOdd_HIP : public DiracWilsonLike_EvenOdd {
+ private:
+  double kpp_;
+  const Field* u_;
+  ii_context* ctx_;
+  Dirac_Wilson_EvenOdd<Dirac_Wilson> cpu_;  // MD force only
+  Dirac_Wilson_EvenOdd_HIP(const Dirac_Wilson_EvenOdd_HIP&);
+
+  const Field hop(ii_status (*fn)(ii_context*, double, double*, const double*),
+                  const Field& f, const char* what) const {
+    Field w(fsize());
+    iroiro_hip_detail::check(ctx_, fn(ctx_, kpp_, w.getaddr(0), iroiro_hip_detail::addr(f)), what);
+    return w;
+  }
+
+ public:
+  Dirac_Wilson_EvenOdd_HIP(double mass, const Field* u)
+      : kpp_(0.5 / (4.0 + mass)), u_(u), ctx_(iroiro_hip_detail::context()), cpu_(mass, u) {
+    update_internal_state();
+  }
+
+  size_t fsize() const { return (size_t)CommonPrms::Nvol() / 2 * 24; }
+  size_t gsize() const { return (size_t)CommonPrms::Nvol() * 18 * 4; }
+  double getMass() const { return 0.5 / kpp_ - 4.0; }
+  double getKappa() const { return kpp_; }
+  const Field* getGaugeField_ptr() const { return u_; }
+  ii_context* context() const { return ctx_; }
+
+  // Dirac::update_internal_state (dirac.hpp:28): the caller changed U.
+  void update_internal_state() {
+    iroiro_hip_detail::check(ctx_, ii_gauge_upload(ctx_, iroiro_hip_detail::addr(*u_)),
+                             "ii_gauge_upload");
+  }
+
+  const Field mult(const Field& f) const { return hop(ii_dirac_mult, f, "mult"); }
+  const Field mult_dag(const Field& f) const { return hop(ii_dirac_mult_dag, f, "mult_dag"); }
+  const Field mult_eo(const Field& f) const { return hop(ii_dirac_mult_eo, f, "mult_eo"); }
+  const Field mult_oe(const Field& f) const { return hop(ii_dirac_mult_oe, f, "mult_oe"); }
+  const Field mult_eo_dag(const Field& f) const {
+    return hop(ii_dirac_mult_eo_dag, f, "mult_eo_dag");
+  }
+  const Field mult_oe_dag(const Field& f) const {
+    return hop(ii_dirac_mult_oe_dag, f, "mult_oe_dag");
+  }
+  const Field mult_ee(const Field& f) const { return f; }
+  const Field mult_oo(const Field& f) const { return f; }
+  const Field mult_ee_inv(const Field& f) const { return f; }
+  const Field mult_oo_inv(const Field& f) const { return f; }
+  const Field gamma5(const Field& f) const {
+    Field w(f.size());
+    iroiro_hip_detail::check(ctx_, ii_dirac_gamma5(ctx_, w.getaddr(0), iroiro_hip_detail::addr(f),
+                                                   (long)(f.size() / 24)),
+                             "gamma5");
+    return w;
+  }
+  const DiracWilsonLike* getDeo() const { return cpu_.getDeo(); }
+  const DiracWilsonLike* getDoe() const { return cpu_.getDoe(); }
+  const Field md_force(const Field& eta, const Field& zeta) const {
+    return cpu_.md_force(eta, zeta);
+  }
+};
+
+// Fopr_DdagD on the device (fopr.h:146-153).
+class Fopr_DdagD_HIP : public Fopr_Herm {
+  const Dirac_Wilson_EvenOdd_HIP* D_;
+
+ public:
+  explicit Fopr_DdagD_HIP(const Dirac_Wilson_EvenOdd_HIP* D) : D_(D) {}
+  double func(double x) const { return x; }
+  const Field mult(const Field& f) const {
+    Field w(fsize());
+    iroiro_hip_detail::check(D_->context(),
+                             ii_fopr_ddagd_mult(D_->context(), D_->getKappa(), w.getaddr(0),
+                                                iroiro_hip_detail::addr(f)),
+                             "Fopr_DdagD::mult");
+    return w;
+  }
+  size_t fsize() const { return D_->fsize(); }
+  const Dirac_Wilson_EvenOdd_HIP* dirac() const { return D_; }
+};
+
+// Solver_CG with the whole iteration on the device (solver_CG.cpp:29-160):
+// same parameters (Precision on |r|^2/|b|^2, MaxIter), x0 = b, same
+// Iterations / diff / timing in SolverOutput.
+class Solver_CG_HIP : public Solver {
+  const Fopr_DdagD_HIP* opr_;
+  double prec_;
+  int maxiter_;
+
+ public:
+  Solver_CG_HIP(double prec, int maxiter, const Fopr_DdagD_HIP* fopr)
+      : opr_(fopr), prec_(prec), maxiter_(maxiter) {}
+
+  SolverOutput solve(Field& xq, const Field& b) const {
+    const Dirac_Wilson_EvenOdd_HIP* D = opr_->dirac();
+    ii_context* ctx = D->context();
+    if (xq.size() != b.size()) xq.resize(b.size());
+    ii_solver_output o;
+    ii_status st = ii_solver_cg(ctx, D->getKappa(), xq.getaddr(0), iroiro_hip_detail::addr(b),
+                                prec_, maxiter_, &o);
+    if (st == II_ERR_NOT_CONVERGED) {
+      std::ostringstream msg;
+      msg << "CG solver not converged. " << ii_last_error(ctx);
+      Errors::ConvergenceErr(msg);
+    }
+    iroiro_hip_detail::check(ctx, st, "Solver_CG::solve");
+    SolverOutput out;
+    out.Msg = "CG solver";
+    out.Iterations = o.iterations;
+    out.diff = o.diff;
+    out.timing = o.timing_ms;
+    return out;
+  }
+  bool check_DdagD() const { return true; }
+};
+
+// MultiShiftSolver_CG on the device (multiShiftSolver_CG.cpp:95-185).
+class MultiShiftSolver_CG_HIP : public MultiShiftSolver {
+  const Fopr_DdagD_HIP* opr_;
+  double prec_;
+  int maxiter_;
+
+ public:
+  MultiShiftSolver_CG_HIP(const Fopr_DdagD_HIP* fopr, double stop_cond, int niter)
+      : opr_(fopr), prec_(stop_cond), maxiter_(niter) {}
+
+  SolverOutput solve(vector_Field& xq, const Field& b, const vector_double& sigma, double& diff,
+                     int& Nconv) const {
+    const Dirac_Wilson_EvenOdd_HIP* D = opr_->dirac();
+    ii_context* ctx = D->context();
+    const size_t n = b.size();
+    std::vector<double> x(n * sigma.size());
+    ii_solver_output o;
+    ii_status st = ii_multishift_cg(ctx, D->getKappa(), &x[0], iroiro_hip_detail::addr(b),
+                                    &sigma[0], (int)sigma.size(), prec_, maxiter_, &o);
+    if (st == II_ERR_NOT_CONVERGED) throw "Not converged.";
+    iroiro_hip_detail::check(ctx, st, "MultiShiftSolver_CG::solve");
+    for (size_t i = 0; i < sigma.size(); ++i) {
+      if (xq[i].size() != n) xq[i].resize(n);
+      for (size_t k = 0; k < n; ++k) xq[i].set(k, x[i * n + k]);
+    }
+    diff = o.diff;
+    Nconv = o.iterations;
+    SolverOutput out;
+    out.Msg = "Multishift CG solver";
+    out.Iterations = o.iterations;
+    out.diff = o.true_residual;
+    out.timing = o.timing_ms;
+    return out;
+  }
+};
+
+#endif

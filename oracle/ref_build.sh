@@ -51,3 +51,21 @@ done
 g++ $CXXFLAGS $INC -c "$HERE/ref_driver.cpp" -o "$OUT/obj/ref_driver.o"
 g++ -fopenmp -o "$OUT/iroiro_ref" "$OUT/obj/ref_driver.o" $OBJS
 echo "ref_build: built $OUT/iroiro_ref"
+
+# Drop-in test binary: the reference's own Inverter_WilsonLike / Qprop_EvenOdd
+# driving the device classes of iroiro_amd/host/iroiro_ref_adapter.hpp
+# (needs the HIP library; links it with an rpath relative to this folder).
+REPO=$(cd "$HERE/.." && pwd)
+HIPLIB="$REPO/iroiro_amd/libiroiro_hip.so"
+if [ -f "$HIPLIB" ]; then
+  QSRC=lib/Measurements/FermionicM/qprop_EvenOdd.cpp
+  QOBJ="$OUT/obj/$(echo "$QSRC" | tr '/' '_' | sed 's/\.cpp$/.o/')"
+  if [ ! -f "$QOBJ" ] || [ "$REF/$QSRC" -nt "$QOBJ" ]; then
+    g++ $CXXFLAGS $INC -c "$REF/$QSRC" -o "$QOBJ"
+  fi
+  g++ $CXXFLAGS $INC -I"$REPO/include" -I"$REPO/iroiro_amd/host" \
+      -c "$REPO/tests/cpp/ref_adapter_driver.cpp" -o "$OUT/obj/ref_adapter_driver.o"
+  g++ -fopenmp -o "$OUT/iroiro_ref_hip" "$OUT/obj/ref_adapter_driver.o" $OBJS "$QOBJ" \
+      -L"$REPO/iroiro_amd" -liroiro_hip -Wl,-rpath,'$ORIGIN/../../iroiro_amd'
+  echo "ref_build: built $OUT/iroiro_ref_hip"
+fi

@@ -1,0 +1,143 @@
+// Drop-in test: the reference's own drivers (Inverter_WilsonLike,
+// Qprop_EvenOdd with Source_local) run once with the reference CPU classes
+// and once with the device classes of iroiro_amd/host/iroiro_ref_adapter.hpp,
+// in one process, on the same gauge field.  Compiled by oracle/ref_build.sh
+// against the reference sources (test infrastructure, like the reference's
+// own tests/test_wilson_EvenOdd.cpp); run by tests/test_gpu_host.py.
+//
+// usage: iroiro_ref_hip Lx Ly Lz Lt mass gauge.bin src_full.bin
+// prints one line "IIHIP {json}" with relative differences and counts.
+#include "include/commonPrms.hpp"
+#include "Communicator/communicator.hpp"
+#include "Geometry/siteIndex_EvenOdd.hpp"
+#include "Dirac_ops/dirac_wilson.hpp"
+#include "Dirac_ops/dirac_wilson_EvenOdd.hpp"
+#include "Dirac_ops/eoUtils.hpp"
+#include "Fopr/fopr.h"
+#include "Solver/solver_CG.hpp"
+#include "Solver/multiShiftSolver_CG.hpp"
+#include "Measurements/FermionicM/qprop_EvenOdd.hpp"
+#include "Measurements/FermionicM/source_types.hpp"
+
+#include "iroiro_ref_adapter.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+static Field read_field(const char* path, size_t n) {
+  Field f(n);
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp || std::fread(f.getaddr(0), sizeof(double), n, fp) != n) {
+    std::fprintf(stderr, "cannot read %s\n", path);
+    std::exit(2);
+  }
+  std::fclose(fp);
+  return f;
+}
+
+static double rel(const Field& a, const Field& b) {
+  double d = 0.0, n = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    double x = a[i] - b[i];
+    d += x * x;
+    n += b[i] * b[i];
+  }
+  return std::sqrt(d / n);
+}
+
+int main(int argc, char** argv) {
+  if (argc < 8) {
+    std::fprintf(stderr, "usage: iroiro_ref_hip Lx Ly Lz Lt mass gauge src\n");
+    return 2;
+  }
+  Lattice latt;
+  latt.stdinput = false;
+  latt.Lx = std::atoi(argv[1]);
+  latt.Ly = std::atoi(argv[2]);
+  latt.Lz = std::atoi(argv[3]);
+  latt.Lt = std::atoi(argv[4]);
+  latt.NPEx = latt.NPEy = latt.NPEz = latt.NPEt = 1;
+  CommonPrms::instance(latt);
+  Communicator::instance();
+  const int V = CommonPrms::Nvol();
+  const double mass = std::atof(argv[5]);
+  Field u = read_field(argv[6], (size_t)18 * 4 * V);
+  Field src = read_field(argv[7], (size_t)24 * V);
+
+  Dirac_Wilson_EvenOdd<Dirac_Wilson> Dcpu(mass, &u);
+  Dirac_Wilson_EvenOdd_HIP Dgpu(mass, &u);
+  Format::Format_F fmt(V);
+  SiteIndex_EvenOdd* eo = SiteIndex_EvenOdd::instance();
+  Field fe(src[fmt.get_sub(eo->esec())]);
+  Field fo(src[fmt.get_sub(eo->osec())]);
+
+  double r_ops = 0.0;
+  r_ops = std::max(r_ops, rel(Dgpu.mult(fe), Dcpu.mult(fe)));
+  r_ops = std::max(r_ops, rel(Dgpu.mult_dag(fe), Dcpu.mult_dag(fe)));
+  r_ops = std::max(r_ops, rel(Dgpu.mult_eo(fo), Dcpu.mult_eo(fo)));
+  r_ops = std::max(r_ops, rel(Dgpu.mult_oe(fe), Dcpu.mult_oe(fe)));
+  r_ops = std::max(r_ops, rel(Dgpu.mult_eo_dag(fe), Dcpu.mult_eo_dag(fe)));
+  r_ops = std::max(r_ops, rel(Dgpu.mult_oe_dag(fo), Dcpu.mult_oe_dag(fo)));
+  r_ops = std::max(r_ops, rel(Dgpu.gamma5(fe), Dcpu.gamma5(fe)));
+
+  Fopr_DdagD Acpu(&Dcpu);
+  Fopr_DdagD_HIP Agpu(&Dgpu);
+  r_ops = std::max(r_ops, rel(Agpu.mult(fe), Acpu.mult(fe)));
+
+  // Solver_CG through its base class, as Inverter_WilsonLike uses it
+  const double prec = 1e-24;
+  Solver_CG Scpu(prec, 2000, &Acpu);
+  Solver_CG_HIP Sgpu(prec, 2000, &Agpu);
+  const Solver* sc = &Scpu;
+  const Solver* sg = &Sgpu;
+  Field xc(fe.size()), xg(fe.size());
+  SolverOutput oc = sc->solve(xc, fe);
+  SolverOutput og = sg->solve(xg, fe);
+  double r_cg = rel(xg, xc);
+
+  // multishift through its base class
+  MultiShiftSolver_CG Mcpu(&Acpu, prec, 2000);
+  MultiShiftSolver_CG_HIP Mgpu(&Agpu, prec, 2000);
+  std::vector<double> shifts;
+  shifts.push_back(0.0);
+  shifts.push_back(0.05);
+  shifts.push_back(0.2);
+  vector_Field mc(shifts.size(), Field(fe.size())), mg(shifts.size(), Field(fe.size()));
+  double dc = 0, dg = 0;
+  int nc = -1, ng = -1;
+  const MultiShiftSolver* msc = &Mcpu;
+  const MultiShiftSolver* msg = &Mgpu;
+  msc->solve(mc, fe, shifts, dc, nc);
+  msg->solve(mg, fe, shifts, dg, ng);
+  double r_ms = 0.0;
+  for (size_t i = 0; i < shifts.size(); ++i) r_ms = std::max(r_ms, rel(mg[i], mc[i]));
+
+  // the reference's Inverter_WilsonLike and Qprop_EvenOdd drive both
+  EvenOddUtils::Inverter_WilsonLike Icpu(&Dcpu, &Scpu);
+  EvenOddUtils::Inverter_WilsonLike Igpu(&Dgpu, &Sgpu);
+  Field sol_c(src.size()), sol_g(src.size());
+  Icpu.invert(sol_c, src);
+  Igpu.invert(sol_g, src);
+  double r_inv = rel(sol_g, sol_c);
+
+  std::vector<int> origin(4, 0);
+  Source_local<Format::Format_F> point(origin, V);
+  Qprop_EvenOdd Qcpu(&Dcpu, &Scpu);
+  Qprop_EvenOdd Qgpu(&Dgpu, &Sgpu);
+  prop_t pc, pg;
+  Qcpu.calc(pc, point);
+  Qgpu.calc(pg, point);
+  double r_prop = 0.0;
+  for (size_t k = 0; k < pc.size(); ++k) r_prop = std::max(r_prop, rel(pg[k], pc[k]));
+
+  std::printf("IIHIP {\"ops_rel\": %.3e, \"cg_rel\": %.3e, \"cg_iter_cpu\": %d, "
+              "\"cg_iter_gpu\": %d, \"ms_rel\": %.3e, \"ms_nconv_cpu\": %d, "
+              "\"ms_nconv_gpu\": %d, \"inv_rel\": %.3e, \"prop_rel\": %.3e, \"ncol\": %d}\n",
+              r_ops, r_cg, oc.Iterations, og.Iterations, r_ms, nc, ng, r_inv, r_prop,
+              (int)pg.size());
+  return 0;
+}
