@@ -176,6 +176,79 @@ typedef struct {
 ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_params* p,
                               const ii_field* src, ii_field* sol, ii_solver_output* out);
 
+/* ------------------------------------------- learned preconditioner
+ * No counterpart in the reference (its solvers are unpreconditioned CG):
+ * these calls add a trained, gauge-covariant preconditioner M for the
+ * even/odd Schur operator Dhat and a flexible GCR that uses it.
+ *
+ * The network is linear: 4 layers of widths 1 -> 8 -> 8 -> 8 -> 1 spinor
+ * channels on the full lattice.  Layer l maps its input channels by
+ *   out[o](x) = sum_{d=0..8} sum_i W_l[d][o][i] (T_d in[i])(x),
+ * T_0 = 1, T_{2mu+1} psi(x) = U_mu(x) psi(x+mu),
+ * T_{2mu+2} psi(x) = U_mu(x-mu)^dag psi(x-mu), each W a complex 4x4 spin
+ * matrix (acting alike on colour, so the map is gauge covariant).  M r is
+ * the even part of N applied to r placed on the even sites.
+ * Weights of layer l: float32 [9][cout][cin][4][4] complex, re/im
+ * interleaved (9*cout*cin*32 floats).  Device arithmetic: bfloat16 weights
+ * and bfloat16-rounded transported activations on the matrix cores, float32
+ * accumulation and float32 activation storage.
+ * Host full fields of nch channels: [nch][V][24] doubles, Format_F order
+ * per channel (include/format_F.h:36-42), lexicographic sites. */
+#define II_NET_LAYERS 4
+typedef struct {
+  int npairs;          /* training pairs (Gaussian source, plain-CG solution) */
+  int epochs;          /* passes over the pairs */
+  int batch;           /* pairs per optimizer step */
+  double lr;           /* Adam step size */
+  double pair_tol;     /* |bhat - Dhat y|/|bhat| of the plain-CG solutions */
+  unsigned long long seed;
+  int keep_weights;    /* nonzero: start from the current weights */
+} ii_train_params;
+typedef struct {
+  double loss_first;   /* mean relative loss |M b - y|^2/|y|^2, first epoch */
+  double loss_last;    /* ... last epoch */
+  double pairs_ms;     /* time to produce the pairs with the plain solver */
+  double train_ms;     /* time of the optimizer loop */
+  int steps;           /* optimizer steps taken */
+} ii_train_report;
+
+/* Initialises the weights to the truncated Neumann series
+ * sum_{k=0..4} (kappa H)^k of D^-1 (Horner form in channels 0 and 1), plus
+ * noise of relative size `noise` on every weight (seeded). */
+ii_status ii_net_init(ii_context* ctx, double kappa, double noise, unsigned long long seed);
+ii_status ii_net_set_weights(ii_context* ctx, int layer, const float* w);
+ii_status ii_net_get_weights(ii_context* ctx, int layer, float* w);
+/* One layer (adjoint != 0: its adjoint, cout -> cin channels) on host full
+ * fields; in has cin (cout) channels, out cout (cin). */
+ii_status ii_net_layer_apply(ii_context* ctx, int layer, int adjoint, const double* in,
+                             double* out);
+/* Gradient of a real loss with respect to W_l (dL/dRe W + i dL/dIm W) given
+ * the input of the layer and dL/dRe out + i dL/dIm out; gw as the weights. */
+ii_status ii_net_layer_wgrad(ii_context* ctx, int layer, const double* in,
+                             const double* grad_out, float* gw);
+/* Loss sum_k |M b_k - y_k|^2/|y_k|^2 over npairs even half fields and its
+ * gradient with respect to every layer's weights (concatenated). */
+ii_status ii_net_loss_grad(ii_context* ctx, int npairs, const double* b_even,
+                           const double* y_even, double* loss, float* grads);
+/* z = M r on even half fields (esec order, 24*V/2 doubles). */
+ii_status ii_net_apply(ii_context* ctx, const double* r_even, double* z_even);
+/* Produces npairs Gaussian sources bhat and their plain FP64 CG solutions
+ * y = Dhat^-1 bhat on the device, then trains with Adam on the relative
+ * loss.  Everything runs on the device. */
+ii_status ii_net_train(ii_context* ctx, double kappa, const ii_train_params* p,
+                       ii_train_report* rep);
+ii_status ii_net_save(ii_context* ctx, const char* path);
+ii_status ii_net_load(ii_context* ctx, const char* path);
+/* Flexible GCR on Dhat x = b (even half fields), FP64 vectors and Dhat,
+ * right preconditioner M (use_net != 0) or none, restarted every `nkrylov`
+ * directions with an FP64 true residual; stops when |b - Dhat x| <= tol |b|.
+ * Falls back to CG on Dhat^dag Dhat if the flexible iteration stalls.
+ * out->iterations counts outer directions, out->restarts restarts,
+ * out->inner_iterations fallback CG iterations. */
+ii_status ii_fgcr_solve(ii_context* ctx, double kappa, double* x_even, const double* b_even,
+                        double tol, int maxiter, int nkrylov, int use_net,
+                        ii_solver_output* out);
+
 /* ------------------------------------------------------ measurement */
 /* Times one even/odd hop (Doe, V/2 output sites) in the given precision
  * with HIP events on the context stream, over nrep launches after a

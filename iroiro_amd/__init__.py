@@ -29,7 +29,7 @@ __all__ = [
     "Context", "Dirac_Wilson_EvenOdd", "Dirac_Wilson", "Fopr_DdagD", "Solver_CG",
     "MultiShiftSolver_CG", "Inverter_WilsonLike", "Qprop_EvenOdd", "Source_local",
     "SolverOutput", "DeviceField", "PropagatorSolver", "ConvergenceErr", "NotConverged",
-    "IIError", "kappa_from_mass", "esec_osec",
+    "IIError", "kappa_from_mass", "esec_osec", "LearnedPreconditioner", "Solver_FGCR",
 ]
 
 
@@ -463,3 +463,125 @@ class PropagatorSolver:
         res = [SolverOutput(o) for o in outs]
         self.ctx.check(st, "ii_propagator_solve")
         return res
+
+
+class LearnedPreconditioner:
+    """The trained gauge-covariant preconditioner M of the even/odd Schur
+    operator (no reference counterpart): a linear network of 4 layers of
+    widths 1 -> 8 -> 8 -> 8 -> 1 spinor channels, each
+      out[o](x) = sum_d sum_i W[d][o][i] (T_d in[i])(x)
+    over the 9 nearest-neighbour parallel-transport paths d, W complex 4x4
+    spin matrices.  Weights: complex numpy arrays [9][cout][cin][4][4]."""
+
+    WIDTHS = _lib.NET_WIDTHS
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def _L(self):
+        return _lib.load()
+
+    def shape(self, layer):
+        return (9, self.WIDTHS[layer + 1], self.WIDTHS[layer], 4, 4)
+
+    def init_neumann(self, kappa, noise=0.0, seed=1):
+        """sum_{k<=4} (kappa H)^k, the truncated Neumann series of D^-1."""
+        self.ctx.check(self._L().ii_net_init(self.ctx.handle, float(kappa), float(noise),
+                                             int(seed)), "ii_net_init")
+
+    def set_weights(self, layer, w):
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.complex64).reshape(self.shape(layer)))
+        f = w.view(np.float32).ravel()
+        self.ctx.check(self._L().ii_net_set_weights(self.ctx.handle, int(layer), _lib.fptr(f)),
+                       "ii_net_set_weights")
+
+    def get_weights(self, layer):
+        shp = self.shape(layer)
+        f = np.empty(2 * int(np.prod(shp)), np.float32)
+        self.ctx.check(self._L().ii_net_get_weights(self.ctx.handle, int(layer), _lib.fptr(f)),
+                       "ii_net_get_weights")
+        return f.view(np.complex64).reshape(shp)
+
+    def layer_apply(self, layer, x, adjoint=False):
+        cin, cout = self.WIDTHS[layer], self.WIDTHS[layer + 1]
+        if adjoint:
+            cin, cout = cout, cin
+        x = _lib.as_f64(x, cin * 24 * self.ctx.V)
+        out = np.empty(cout * 24 * self.ctx.V)
+        self.ctx.check(self._L().ii_net_layer_apply(self.ctx.handle, int(layer), int(adjoint),
+                                                    _lib.dptr(x), _lib.dptr(out)),
+                       "ii_net_layer_apply")
+        return out
+
+    def layer_wgrad(self, layer, x, g):
+        cin, cout = self.WIDTHS[layer], self.WIDTHS[layer + 1]
+        x = _lib.as_f64(x, cin * 24 * self.ctx.V)
+        g = _lib.as_f64(g, cout * 24 * self.ctx.V)
+        shp = self.shape(layer)
+        f = np.empty(2 * int(np.prod(shp)), np.float32)
+        self.ctx.check(self._L().ii_net_layer_wgrad(self.ctx.handle, int(layer), _lib.dptr(x),
+                                                    _lib.dptr(g), _lib.fptr(f)),
+                       "ii_net_layer_wgrad")
+        return f.view(np.complex64).reshape(shp)
+
+    def loss_grad(self, bs, ys):
+        """Loss sum_k |M b_k - y_k|^2/|y_k|^2 and its gradient per layer."""
+        nh = 24 * self.ctx.V // 2
+        b = _lib.as_f64(np.concatenate([np.ravel(v) for v in bs]), nh * len(bs))
+        y = _lib.as_f64(np.concatenate([np.ravel(v) for v in ys]), nh * len(ys))
+        sizes = [int(np.prod(self.shape(l))) for l in range(_lib.NET_LAYERS)]
+        f = np.empty(2 * sum(sizes), np.float32)
+        loss = ctypes.c_double()
+        self.ctx.check(self._L().ii_net_loss_grad(self.ctx.handle, len(bs), _lib.dptr(b),
+                                                  _lib.dptr(y), ctypes.byref(loss), _lib.fptr(f)),
+                       "ii_net_loss_grad")
+        c = f.view(np.complex64)
+        grads, off = [], 0
+        for l, n in enumerate(sizes):
+            grads.append(c[off:off + n].reshape(self.shape(l)))
+            off += n
+        return loss.value, grads
+
+    def apply(self, r):
+        r = _lib.as_f64(r, 24 * self.ctx.V // 2)
+        z = np.empty_like(r)
+        self.ctx.check(self._L().ii_net_apply(self.ctx.handle, _lib.dptr(r), _lib.dptr(z)),
+                       "ii_net_apply")
+        return z
+
+    def train(self, kappa, npairs=64, epochs=20, batch=4, lr=1e-3, pair_tol=1e-8, seed=1,
+              keep_weights=False):
+        p = _lib.TrainParamsC(int(npairs), int(epochs), int(batch), float(lr), float(pair_tol),
+                              int(seed), int(keep_weights))
+        rep = _lib.TrainReportC()
+        self.ctx.check(self._L().ii_net_train(self.ctx.handle, float(kappa), ctypes.byref(p),
+                                              ctypes.byref(rep)), "ii_net_train")
+        return {"loss_first": rep.loss_first, "loss_last": rep.loss_last,
+                "pairs_ms": rep.pairs_ms, "train_ms": rep.train_ms, "steps": rep.steps}
+
+    def save(self, path):
+        self.ctx.check(self._L().ii_net_save(self.ctx.handle, str(path).encode()), "ii_net_save")
+
+    def load(self, path):
+        self.ctx.check(self._L().ii_net_load(self.ctx.handle, str(path).encode()), "ii_net_load")
+
+
+class Solver_FGCR:
+    """Flexible GCR on Dhat x = b with the learned preconditioner (or none),
+    FP64 vectors, restart every nkrylov directions; plain CG on
+    Dhat^dag Dhat takes over if the flexible iteration stalls."""
+
+    def __init__(self, ctx, kappa, tol=1e-10, maxiter=1000, nkrylov=16, use_net=True):
+        self.ctx, self.kappa, self.tol = ctx, float(kappa), float(tol)
+        self.maxiter, self.nkrylov, self.use_net = int(maxiter), int(nkrylov), bool(use_net)
+
+    def solve(self, x, b):
+        b = _lib.as_f64(b, 24 * self.ctx.V // 2)
+        out = _lib.SolverOutputC()
+        xx = np.empty_like(b)
+        st = _lib.load().ii_fgcr_solve(self.ctx.handle, self.kappa, _lib.dptr(xx), _lib.dptr(b),
+                                       self.tol, self.maxiter, self.nkrylov, int(self.use_net),
+                                       ctypes.byref(out))
+        self.ctx.check(st, "ii_fgcr_solve")
+        x[:] = xx
+        return SolverOutput(out)

@@ -27,8 +27,15 @@ EXPORTS = [
     "ii_dirac_mult", "ii_dirac_mult_dag", "ii_dirac_gamma5", "ii_fopr_ddagd_mult",
     "ii_dirac_full_mult", "ii_solver_cg", "ii_multishift_cg", "ii_eo_invert",
     "ii_field_create", "ii_field_destroy", "ii_field_upload", "ii_field_download",
-    "ii_field_point_sources", "ii_propagator_solve", "ii_bench_hop", "ii_bench_copy",
+    "ii_field_point_sources", "ii_propagator_solve",
+    "ii_net_init", "ii_net_set_weights", "ii_net_get_weights", "ii_net_layer_apply",
+    "ii_net_layer_wgrad", "ii_net_loss_grad", "ii_net_apply", "ii_net_train", "ii_net_save",
+    "ii_net_load", "ii_fgcr_solve",
+    "ii_bench_hop", "ii_bench_copy",
 ]
+
+NET_LAYERS = 4
+NET_WIDTHS = (1, 8, 8, 8, 1)
 
 
 class IIError(RuntimeError):
@@ -61,6 +68,18 @@ class PropParamsC(ctypes.Structure):
                 ("use_graph", ctypes.c_int), ("use_learned", ctypes.c_int)]
 
 
+class TrainParamsC(ctypes.Structure):
+    _fields_ = [("npairs", ctypes.c_int), ("epochs", ctypes.c_int), ("batch", ctypes.c_int),
+                ("lr", ctypes.c_double), ("pair_tol", ctypes.c_double),
+                ("seed", ctypes.c_ulonglong), ("keep_weights", ctypes.c_int)]
+
+
+class TrainReportC(ctypes.Structure):
+    _fields_ = [("loss_first", ctypes.c_double), ("loss_last", ctypes.c_double),
+                ("pairs_ms", ctypes.c_double), ("train_ms", ctypes.c_double),
+                ("steps", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -75,6 +94,7 @@ def load():
     L = ctypes.CDLL(LIB_PATH)
     vp, ci, cd, cl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
     dp = ctypes.POINTER(ctypes.c_double)
+    fp = ctypes.POINTER(ctypes.c_float)
     ip = ctypes.POINTER(ctypes.c_int)
     so = ctypes.POINTER(SolverOutputC)
     sig = {
@@ -106,6 +126,17 @@ def load():
         "ii_field_download": [vp, ci, dp],
         "ii_field_point_sources": [vp, ip],
         "ii_propagator_solve": [vp, cd, ctypes.POINTER(PropParamsC), vp, vp, so],
+        "ii_net_init": [vp, cd, cd, ctypes.c_ulonglong],
+        "ii_net_set_weights": [vp, ci, fp],
+        "ii_net_get_weights": [vp, ci, fp],
+        "ii_net_layer_apply": [vp, ci, ci, dp, dp],
+        "ii_net_layer_wgrad": [vp, ci, dp, dp, fp],
+        "ii_net_loss_grad": [vp, ci, dp, dp, dp, fp],
+        "ii_net_apply": [vp, dp, dp],
+        "ii_net_train": [vp, cd, ctypes.POINTER(TrainParamsC), ctypes.POINTER(TrainReportC)],
+        "ii_net_save": [vp, ctypes.c_char_p],
+        "ii_net_load": [vp, ctypes.c_char_p],
+        "ii_fgcr_solve": [vp, cd, dp, dp, cd, ci, ci, ci, so],
         "ii_bench_hop": [vp, ci, ci, ci, dp],
         "ii_bench_copy": [vp, ctypes.c_size_t, ci, dp],
     }
@@ -121,6 +152,10 @@ def load():
 
 def dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def as_f64(a, n=None):

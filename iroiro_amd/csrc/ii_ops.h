@@ -77,6 +77,15 @@ ii_status unpack_gauge(ii_context* ctx, double* host_dev);
 ii_status build_lowprec_gauge(ii_context* ctx, int prec);
 ii_status plaquette64(ii_context* ctx, double* plaq);
 
+// solvers (ii_solvers.hip)
+// Solver_CG::solve semantics on Dhat^dag Dhat x = b: x0 = b, stop on
+// |r|^2/|b|^2 < prec (solver_CG.cpp:29-160).
+ii_status cg_reference(ii_context* ctx, double kappa, double2* x, const double2* b, double prec,
+                       int maxiter, ii_solver_output* out);
+// Full-field propagator solve through the even/odd Schur system.
+ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
+                     const ii_field* src, ii_field* sol, ii_solver_output* outs);
+
 // learned preconditioner (ii_learned.hip)
 void learned_release(ii_context* ctx);
 ii_status learned_propagator(ii_context* ctx, double kappa, const ii_prop_params* p,

@@ -55,6 +55,9 @@ def lib():
             "or_plaquette": (c_double, [vp, _dp]),
             "or_heatbath_sweep": (None, [vp, _dp, c_double, c_u64, c_int]),
             "or_uniform": (c_double, [c_u64, c_u64, c_u64]),
+            "or_net_layer": (None, [vp, _dp, c_int, c_int, _dp, _dp, _dp, c_int]),
+            "or_net_layer_adj": (None, [vp, _dp, c_int, c_int, _dp, _dp, _dp, c_int]),
+            "or_net_layer_wgrad": (None, [vp, _dp, c_int, c_int, _dp, _dp, _dp, c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -188,10 +191,84 @@ class Lattice:
     def heatbath_sweep(self, u, beta, seed, sweep):
         lib().or_heatbath_sweep(self._g, u, beta, seed, sweep)
 
+    # ---- learned gauge-covariant layers (weights: complex [9][cout][cin][4][4])
+    def net_layer(self, u, w, x, round_bf16=False):
+        cout, cin = w.shape[1], w.shape[2]
+        out = np.empty(cout * self.full_size)
+        lib().or_net_layer(self._g, u, cin, cout, _wflat(w), np.ascontiguousarray(x, np.float64),
+                           out, int(round_bf16))
+        return out
+
+    def net_layer_adj(self, u, w, g, round_bf16=False):
+        cout, cin = w.shape[1], w.shape[2]
+        out = np.empty(cin * self.full_size)
+        lib().or_net_layer_adj(self._g, u, cin, cout, _wflat(w),
+                               np.ascontiguousarray(g, np.float64), out, int(round_bf16))
+        return out
+
+    def net_layer_wgrad(self, u, cin, cout, x, g, round_bf16=False):
+        gw = np.zeros(9 * cout * cin * 32)
+        lib().or_net_layer_wgrad(self._g, u, cin, cout, np.ascontiguousarray(x, np.float64),
+                                 np.ascontiguousarray(g, np.float64), gw, int(round_bf16))
+        return gw.view(np.complex128).reshape(9, cout, cin, 4, 4)
+
     def unit_gauge(self):
         u = np.zeros((4 * self.V, 18))
         u[:, 0] = u[:, 8] = u[:, 16] = 1.0
         return u.ravel()
+
+
+def _wflat(w):
+    return np.ascontiguousarray(np.asarray(w, dtype=np.complex128)).view(np.float64).ravel()
+
+
+def bf16_round(a):
+    """Round float64 values to the nearest bfloat16 (ties to even)."""
+    f = np.asarray(a, dtype=np.float32).copy()
+    b = f.view(np.uint32)
+    b += np.uint32(0x7FFF) + ((b >> 16) & np.uint32(1))
+    b &= np.uint32(0xFFFF0000)
+    return b.view(np.float32).astype(np.float64)
+
+
+class Net:
+    """Composition of gauge-covariant layers (linear), widths e.g. (1, 8, 8, 8, 1).
+    The preconditioner acts on an even half field b: it is placed on the even
+    sites of a full field (odd sites zero), the layers run on the full
+    lattice, and the even sites of the output are returned.  Loss of a batch
+    of (b_k, y_k) pairs: sum_k |N(b_k) - y_k|^2 / |y_k|^2."""
+
+    def __init__(self, lat, u, weights, round_bf16=False):
+        self.lat, self.u, self.w, self.rb = lat, u, list(weights), round_bf16
+
+    def embed(self, b):
+        return self.lat.merge(b, np.zeros(self.lat.half_size))
+
+    def forward(self, b):
+        acts = [self.embed(b)]
+        for w in self.w:
+            acts.append(self.lat.net_layer(self.u, w, acts[-1], self.rb))
+        return acts
+
+    def apply(self, b):
+        return self.lat.split(self.forward(b)[-1])[0]
+
+    def loss_and_grad(self, pairs):
+        loss = 0.0
+        grads = [np.zeros_like(w) for w in self.w]
+        for b, y in pairs:
+            acts = self.forward(b)
+            e = self.lat.split(acts[-1])[0] - y
+            ny = float(np.dot(y, y))
+            loss += float(np.dot(e, e)) / ny
+            g = self.lat.merge(2.0 * e / ny, np.zeros(self.lat.half_size))
+            for li in range(len(self.w) - 1, -1, -1):
+                w = self.w[li]
+                grads[li] += self.lat.net_layer_wgrad(self.u, w.shape[2], w.shape[1], acts[li], g,
+                                                      self.rb)
+                if li:
+                    g = self.lat.net_layer_adj(self.u, w, g, self.rb)
+        return loss, grads
 
 
 def gamma5(f):

@@ -762,3 +762,148 @@ void or_heatbath_sweep(const or_geom* g, double* u, double beta,
         or_heatbath_link(u + 18 * (s + g->V * mu), A, beta, seed, stream);
       }
 }
+
+/* ------------------------------------------------------------------ */
+/* Learned gauge-covariant layer (no reference counterpart).  Path d of a
+ * site x: d = 0 identity; d = 2mu+1 reads x+mu through U_mu(x); d = 2mu+2
+ * reads x-mu through U_mu(x-mu)^dag. */
+
+/* bfloat16 round to nearest, ties to even (finite inputs). */
+static double bf16_round(double v) {
+  float f = (float)v;
+  uint32_t b;
+  memcpy(&b, &f, 4);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  b &= 0xFFFF0000u;
+  memcpy(&f, &b, 4);
+  return (double)f;
+}
+
+/* Colour transport of one 24-double spinor: r[t][a] = sum_b M[a][b] v[t][b]
+ * (dag = 0) or conj(M[b][a]) v[t][b] (dag = 1). */
+static void colour_mul(double* r, const double* M, const double* v, int dag) {
+  for (int t = 0; t < 4; ++t)
+    for (int a = 0; a < 3; ++a) {
+      double re = 0.0, im = 0.0;
+      for (int b = 0; b < 3; ++b) {
+        double mr, mi;
+        if (dag) { mr = M[2 * (3 * b + a)]; mi = -M[2 * (3 * b + a) + 1]; }
+        else     { mr = M[2 * (3 * a + b)]; mi =  M[2 * (3 * a + b) + 1]; }
+        double vr = v[RE(t, b)], vi = v[IM(t, b)];
+        re += mr * vr - mi * vi;
+        im += mr * vi + mi * vr;
+      }
+      r[RE(t, a)] = re;
+      r[IM(t, a)] = im;
+    }
+}
+
+/* Neighbour site n_d(x) and the matrix of path d (dag flag) at site s. */
+static long net_path(const or_geom* g, const double* u, int d, long s,
+                     const double** M, int* dag) {
+  if (d == 0) { *M = NULL; *dag = 0; return s; }
+  int mu = (d - 1) / 2;
+  if ((d - 1) % 2 == 0) {
+    *M = u + 18 * (s + g->V * mu);
+    *dag = 0;
+    return g->nb_fw[mu * g->V + s];
+  }
+  long n = g->nb_bw[mu * g->V + s];
+  *M = u + 18 * (n + g->V * mu);
+  *dag = 1;
+  return n;
+}
+
+static void net_transport(const or_geom* g, const double* u, int d, long s,
+                          const double* field, double* r, int round_bf16) {
+  const double* M;
+  int dag;
+  long n = net_path(g, u, d, s, &M, &dag);
+  if (!M) memcpy(r, field + 24 * n, 24 * sizeof(double));
+  else colour_mul(r, M, field + 24 * n, dag);
+  if (round_bf16)
+    for (int k = 0; k < 24; ++k) r[k] = bf16_round(r[k]);
+}
+
+/* acc[s][c] += W[s][t] v[t][c]  (dag = 0)  or  conj(W[t][s]) v[t][c]  (dag = 1) */
+static void spin_mul_add(double* acc, const double* W, const double* v, int dag) {
+  for (int s = 0; s < 4; ++s)
+    for (int t = 0; t < 4; ++t) {
+      double wr, wi;
+      if (dag) { wr = W[2 * (4 * t + s)]; wi = -W[2 * (4 * t + s) + 1]; }
+      else     { wr = W[2 * (4 * s + t)]; wi =  W[2 * (4 * s + t) + 1]; }
+      for (int c = 0; c < 3; ++c) {
+        double vr = v[RE(t, c)], vi = v[IM(t, c)];
+        acc[RE(s, c)] += wr * vr - wi * vi;
+        acc[IM(s, c)] += wr * vi + wi * vr;
+      }
+    }
+}
+
+void or_net_layer(const or_geom* g, const double* u, int cin, int cout, const double* w,
+                  const double* in, double* out, int round_bf16) {
+  const long V = g->V;
+  double tv[24];
+  for (int o = 0; o < cout; ++o) memset(out + (long)o * 24 * V, 0, 24 * V * sizeof(double));
+  for (long s = 0; s < V; ++s)
+    for (int d = 0; d < 9; ++d)
+      for (int i = 0; i < cin; ++i) {
+        net_transport(g, u, d, s, in + (long)i * 24 * V, tv, round_bf16);
+        for (int o = 0; o < cout; ++o)
+          spin_mul_add(out + (long)o * 24 * V + 24 * s,
+                       w + 32 * ((long)(d * cout + o) * cin + i), tv, 0);
+      }
+}
+
+void or_net_layer_adj(const or_geom* g, const double* u, int cin, int cout, const double* w,
+                      const double* out_grad, double* in_grad, int round_bf16) {
+  const long V = g->V;
+  double tv[24];
+  for (int i = 0; i < cin; ++i) memset(in_grad + (long)i * 24 * V, 0, 24 * V * sizeof(double));
+  for (long s = 0; s < V; ++s)
+    for (int d = 0; d < 9; ++d) {
+      const double* M;
+      int dag;
+      long n = net_path(g, u, d, s, &M, &dag);
+      for (int o = 0; o < cout; ++o) {
+        const double* gx = out_grad + (long)o * 24 * V + 24 * s;
+        /* transpose of "read n through M": carry g(x) back to n through M^dag */
+        if (!M) memcpy(tv, gx, sizeof tv);
+        else colour_mul(tv, M, gx, !dag);
+        if (round_bf16)
+          for (int k = 0; k < 24; ++k) tv[k] = bf16_round(tv[k]);
+        for (int i = 0; i < cin; ++i)
+          spin_mul_add(in_grad + (long)i * 24 * V + 24 * n,
+                       w + 32 * ((long)(d * cout + o) * cin + i), tv, 1);
+      }
+    }
+}
+
+void or_net_layer_wgrad(const or_geom* g, const double* u, int cin, int cout,
+                        const double* in, const double* grad_out, double* gw, int round_bf16) {
+  const long V = g->V;
+  double tv[24], gv[24];
+  for (long s = 0; s < V; ++s)
+    for (int d = 0; d < 9; ++d)
+      for (int i = 0; i < cin; ++i) {
+        net_transport(g, u, d, s, in + (long)i * 24 * V, tv, round_bf16);
+        for (int o = 0; o < cout; ++o) {
+          memcpy(gv, grad_out + (long)o * 24 * V + 24 * s, sizeof gv);
+          if (round_bf16)
+            for (int k = 0; k < 24; ++k) gv[k] = bf16_round(gv[k]);
+          double* G = gw + 32 * ((long)(d * cout + o) * cin + i);
+          for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+              double re = 0.0, im = 0.0;
+              for (int c = 0; c < 3; ++c) {
+                double xr = gv[RE(a, c)], xi = gv[IM(a, c)];
+                double yr = tv[RE(b, c)], yi = tv[IM(b, c)];
+                re += xr * yr + xi * yi;
+                im += xi * yr - xr * yi;
+              }
+              G[2 * (4 * a + b)] += re;
+              G[2 * (4 * a + b) + 1] += im;
+            }
+        }
+      }
+}

@@ -110,6 +110,34 @@ void or_heatbath_sweep(const or_geom* g, double* u, double beta,
 /* Counter-based uniform deviate in (0,1], shared with the GPU heat-bath. */
 double or_uniform(uint64_t seed, uint64_t stream, uint64_t counter);
 
+/* ---- learned gauge-covariant preconditioner (no reference counterpart) --
+ * One linear layer maps cin channels of full-lattice spinor fields to cout
+ * channels:
+ *   out[o](x) = sum_{d=0..8} sum_i W[d][o][i] (T_d in[i])(x)
+ * T_0 = identity, T_{2mu+1} psi(x) = U_mu(x) psi(x+mu),
+ * T_{2mu+2} psi(x) = U_mu(x-mu)^dag psi(x-mu); W[d][o][i] is a complex 4x4
+ * spin matrix acting identically on colour (gauge covariant).
+ * Fields: [channel][site][24] doubles (Format_F per channel, lexicographic).
+ * Weights: [9][cout][cin][4][4] complex = interleaved re,im doubles.
+ * round_bf16 != 0 rounds every transported value (T_d in[i])(x) to the
+ * nearest bfloat16 (ties to even) before it meets W, as a device layer that
+ * feeds bf16 matrix cores does.  Each of the three functions is pinned by
+ * tests/test_oracle_net.py: reduction to or_dirac_full for Wilson weights,
+ * the adjoint identity, finite differences of a loss, gauge covariance. */
+void or_net_layer(const or_geom* g, const double* u, int cin, int cout, const double* w,
+                  const double* in, double* out, int round_bf16);
+/* Adjoint layer, written as the scatter transpose of or_net_layer:
+ * in_grad(n_d(x)) += U_d(x)^dag W_d^dag out_grad(x); with round_bf16 the
+ * colour-transported U_d(x)^dag out_grad(x) is rounded before W_d^dag. */
+void or_net_layer_adj(const or_geom* g, const double* u, int cin, int cout, const double* w,
+                      const double* out_grad, double* in_grad, int round_bf16);
+/* Gradient of a real loss L with respect to W (dL/dRe W + i dL/dIm W), given
+ * grad_out = dL/dRe out + i dL/dIm out:
+ * gw[d][o][i][s][t] += sum_{x,c} grad_out[o](x)[s][c] * conj((T_d in[i])(x)[t][c]).
+ * With round_bf16 both factors are rounded to bfloat16 first. */
+void or_net_layer_wgrad(const or_geom* g, const double* u, int cin, int cout,
+                        const double* in, const double* grad_out, double* gw, int round_bf16);
+
 #ifdef __cplusplus
 }
 #endif

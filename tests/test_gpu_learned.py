@@ -1,0 +1,219 @@
+"""Learned gauge-covariant preconditioner on the GPU, through the C ABI,
+against the oracle's layers (oracle/wilson_oracle.c or_net_layer*).
+
+Bars (relative 2-norm):
+  * layer forward / adjoint vs the oracle that rounds the transported values
+    to bfloat16 exactly as the matrix-core operands are: 1e-4.  The residue
+    is the transported values whose FP32 (device) and FP64 (oracle) forms
+    round to different bf16 neighbours: probability ~1e-7/2^-8 each, error
+    2^-8 each, ~3e-5 in the 2-norm over 288 products per output;
+  * the same vs the unrounded FP64 oracle: 1e-2 (bf16 has 8 significant bits);
+  * weight gradient (FP32 VALU) vs the FP64 oracle: 1e-5;
+  * network gradient vs central finite differences of the FP64 oracle loss
+    along random weight directions: 3e-2 (bf16 forward and adjoint);
+  * gauge covariance of M: 2e-2 (different bf16 roundings of rotated fields).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import iroiro_amd as ii
+from oracle import oracle as orc
+from tests.wilson_math import GAMMAS, random_su3, shift_index
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (4, 4, 4, 8)
+WIDTHS = (1, 8, 8, 8, 1)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
+
+
+def bf16_weights(rng, shape, scale=0.3):
+    w = scale * (rng.normal(size=shape) + 1j * rng.normal(size=shape))
+    return orc.bf16_round(w.real) + 1j * orc.bf16_round(w.imag)
+
+
+@pytest.fixture(scope="module")
+def env():
+    lat = orc.Lattice(DIMS)
+    u = lat.apbc(lat.gauge_expH(0.4, 2024))
+    ctx = ii.Context(DIMS)
+    ctx.upload_gauge(u)
+    net = ii.LearnedPreconditioner(ctx)
+    yield lat, u, ctx, net
+    ctx.close()
+
+
+@pytest.mark.parametrize("layer", [0, 1, 3])
+def test_layer_forward_vs_oracle(env, layer):
+    lat, u, ctx, net = env
+    rng = np.random.default_rng(10 + layer)
+    cin, cout = WIDTHS[layer], WIDTHS[layer + 1]
+    w = bf16_weights(rng, (9, cout, cin, 4, 4))
+    net.set_weights(layer, w)
+    x = rng.normal(size=cin * lat.full_size).astype(np.float32).astype(np.float64)
+    got = net.layer_apply(layer, x)
+    assert rel(got, lat.net_layer(u, w, x, round_bf16=True)) < 1e-4
+    assert rel(got, lat.net_layer(u, w, x)) < 1e-2
+
+
+@pytest.mark.parametrize("layer", [1, 3])
+def test_layer_adjoint_vs_oracle(env, layer):
+    lat, u, ctx, net = env
+    rng = np.random.default_rng(20 + layer)
+    cin, cout = WIDTHS[layer], WIDTHS[layer + 1]
+    w = bf16_weights(rng, (9, cout, cin, 4, 4))
+    net.set_weights(layer, w)
+    g = rng.normal(size=cout * lat.full_size).astype(np.float32).astype(np.float64)
+    got = net.layer_apply(layer, g, adjoint=True)
+    assert rel(got, lat.net_layer_adj(u, w, g, round_bf16=True)) < 1e-4
+    assert rel(got, lat.net_layer_adj(u, w, g)) < 1e-2
+
+
+@pytest.mark.parametrize("layer", [0, 1, 3])
+def test_layer_weight_gradient_vs_oracle(env, layer):
+    lat, u, ctx, net = env
+    rng = np.random.default_rng(30 + layer)
+    cin, cout = WIDTHS[layer], WIDTHS[layer + 1]
+    x = rng.normal(size=cin * lat.full_size).astype(np.float32).astype(np.float64)
+    g = rng.normal(size=cout * lat.full_size).astype(np.float32).astype(np.float64)
+    got = net.layer_wgrad(layer, x, g)
+    ref = lat.net_layer_wgrad(u, cin, cout, x, g)
+    assert rel(got, ref) < 1e-5
+
+
+def test_network_gradient_vs_oracle_finite_differences(env):
+    lat, u, ctx, net = env
+    rng = np.random.default_rng(40)
+    ws = [bf16_weights(rng, (9, WIDTHS[l + 1], WIDTHS[l], 4, 4), 0.15) for l in range(4)]
+    for l, w in enumerate(ws):
+        net.set_weights(l, w)
+    pairs = [(rng.normal(size=lat.half_size), rng.normal(size=lat.half_size)) for _ in range(2)]
+    pairs = [(b.astype(np.float32).astype(np.float64), y.astype(np.float32).astype(np.float64))
+             for b, y in pairs]
+    loss, grads = net.loss_grad([p[0] for p in pairs], [p[1] for p in pairs])
+    ref_loss, ref_grads = orc.Net(lat, u, ws).loss_and_grad(pairs)
+    assert abs(loss - ref_loss) < 2e-2 * ref_loss
+    for l in range(4):
+        assert rel(grads[l], ref_grads[l]) < 3e-2, l
+    # directional finite differences of the FP64 oracle loss
+    h = 1e-4
+    for trial in range(3):
+        dirs = [rng.normal(size=w.shape) + 1j * rng.normal(size=w.shape) for w in ws]
+        wp = [w + h * d for w, d in zip(ws, dirs)]
+        wm = [w - h * d for w, d in zip(ws, dirs)]
+        fd = (orc.Net(lat, u, wp).loss_and_grad(pairs)[0] -
+              orc.Net(lat, u, wm).loss_and_grad(pairs)[0]) / (2 * h)
+        an = sum(float(np.sum(g.real * d.real + g.imag * d.imag)) for g, d in zip(grads, dirs))
+        assert abs(an - fd) < 3e-2 * abs(fd), (trial, an, fd)
+
+
+def test_neumann_init_weights_and_series(env):
+    lat, u, ctx, net = env
+    kappa = 0.125
+    net.init_neumann(kappa, noise=0.0)
+    w0 = net.get_weights(0)
+    for mu in range(4):
+        np.testing.assert_allclose(w0[2 * mu + 1, 1, 0], kappa * (np.eye(4) - GAMMAS[mu]),
+                                   atol=1e-7)
+        np.testing.assert_allclose(w0[2 * mu + 2, 1, 0], kappa * (np.eye(4) + GAMMAS[mu]),
+                                   atol=1e-7)
+    # M r = even part of sum_{k<=4} (kappa H)^k (r, 0), kappa H = 1 - D
+    r = orc.gauss_field(lat.half_size, 7)
+    f = lat.merge(r, np.zeros(lat.half_size))
+    acc = f.copy()
+    term = f.copy()
+    for _ in range(4):
+        term = term - lat.dirac_full(u, term, 0, kappa)
+        acc += term
+    ref = lat.split(acc)[0]
+    assert rel(net.apply(r), ref) < 1e-2
+
+
+def test_preconditioner_gauge_covariance(env):
+    lat, u, ctx, net = env
+    net.init_neumann(0.125, noise=0.5, seed=3)
+    rng = np.random.default_rng(50)
+    V = lat.V
+    gt = random_su3(V, rng)
+    U = u.reshape(4, V, 3, 3, 2)
+    Uc = U[..., 0] + 1j * U[..., 1]
+    Ug = np.empty_like(Uc)
+    for mu in range(4):
+        fw = shift_index(DIMS, mu, +1)
+        Ug[mu] = gt @ Uc[mu] @ np.conj(np.transpose(gt[fw], (0, 2, 1)))
+    ug = np.stack([Ug.real, Ug.imag], axis=-1).ravel()
+    e, o = lat.eo_tables()
+
+    def rot_even(h):
+        c = h.reshape(-1, 4, 3, 2)
+        c = c[..., 0] + 1j * c[..., 1]
+        r = np.einsum("xab,xsb->xsa", gt[e], c)
+        return np.stack([r.real, r.imag], axis=-1).ravel()
+
+    r = orc.gauss_field(lat.half_size, 9)
+    z = net.apply(r)
+    ctx2 = ii.Context(DIMS)
+    try:
+        ctx2.upload_gauge(ug)
+        net2 = ii.LearnedPreconditioner(ctx2)
+        for l in range(4):
+            net2.set_weights(l, net.get_weights(l))
+        z2 = net2.apply(rot_even(r))
+    finally:
+        ctx2.close()
+    assert rel(z2, rot_even(z)) < 2e-2
+
+
+def test_fgcr_plain_and_preconditioned(env):
+    lat, u, ctx, net = env
+    kappa = 0.125
+    b = orc.gauss_field(lat.half_size, 11)
+    res = {}
+    for use_net in (False, True):
+        if use_net:
+            net.init_neumann(kappa, noise=0.0)
+        x = np.empty_like(b)
+        out = ii.Solver_FGCR(ctx, kappa, tol=1e-10, maxiter=500, nkrylov=16,
+                             use_net=use_net).solve(x, b)
+        true = rel(lat.dirac_eo(u, x, 0, kappa), b)
+        assert true < 1e-10, (use_net, true)
+        res[use_net] = out.Iterations
+    assert res[True] < res[False], res
+
+
+def test_training_reduces_loss_and_iterations(tmp_path):
+    """Ablation on an 8^3x16 heat-bath field near kappa_c: the trained
+    network lowers the loss below the Neumann start and the flexible GCR
+    needs fewer directions with it than with no preconditioner."""
+    dims = (8, 8, 8, 16)
+    kappa = 0.1355
+    ctx = ii.Context(dims)
+    try:
+        ctx.unit_gauge()
+        ctx.heatbath(5.8, 99, 60)
+        ctx.apply_apbc()
+        net = ii.LearnedPreconditioner(ctx)
+        rep = net.train(kappa, npairs=16, epochs=6, batch=4, lr=2e-3, pair_tol=1e-8, seed=5)
+        assert rep["loss_last"] < rep["loss_first"], rep
+        path = os.path.join(str(tmp_path), "net.bin")
+        net.save(path)
+        lat = orc.Lattice(dims)
+        b = orc.gauss_field(lat.half_size, 12345)
+        its = {}
+        for use_net in (False, True):
+            x = np.empty_like(b)
+            out = ii.Solver_FGCR(ctx, kappa, tol=1e-10, maxiter=2000, nkrylov=16,
+                                 use_net=use_net).solve(x, b)
+            its[use_net] = out.Iterations
+        assert its[True] < its[False], its
+        z1 = net.apply(b)
+        net.init_neumann(kappa)
+        net.load(path)
+        assert np.array_equal(net.apply(b), z1)
+    finally:
+        ctx.close()
