@@ -140,6 +140,14 @@ ii_status ii_solver_cg(ii_context* ctx, double kappa, double* x, const double* b
 ii_status ii_multishift_cg(ii_context* ctx, double kappa, double* x, const double* b,
                            const double* shifts, int nshift, double prec, int maxiter,
                            ii_solver_output* out);
+/* MultiShiftSolver_CG::solve on Fopr_DdagD of the full-index Dirac_Wilson
+ * (D^dag D + sigma_i, multiShiftSolver_CG.cpp:10-185; the operator of
+ * Qprop_MultiShift, qprop_MultiShift.cpp:10-31).  b: one full field
+ * (24*V doubles, lexicographic); x: nshift full fields back to back.
+ * Same outputs and errors as ii_multishift_cg. */
+ii_status ii_multishift_cg_full(ii_context* ctx, double kappa, double* x, const double* b,
+                                const double* shifts, int nshift, double prec, int maxiter,
+                                ii_solver_output* out);
 /* Inverter_WilsonLike::invert with a Solver_CG (eoUtils.cpp:43-80): full
  * lexicographic solve of D x = b. */
 ii_status ii_eo_invert(ii_context* ctx, double kappa, double* sol, const double* src,
@@ -249,11 +257,23 @@ ii_status ii_fgcr_solve(ii_context* ctx, double kappa, double* x_even, const dou
                         double tol, int maxiter, int nkrylov, int use_net,
                         ii_solver_output* out);
 
+/* Qprop_MultiShift::calc (qprop_MultiShift.cpp:10-31) with
+ * Source_local at `site`: for each column 3s+c a full-operator multishift
+ * solve, then xq[(3s+c)*nshift + m] = D x_m.  xq: full field of 12*nshift
+ * rhs on the device; out: 12 entries (one per column). */
+ii_status ii_qprop_multishift(ii_context* ctx, double kappa, const int site[4],
+                              const double* shifts, int nshift, double prec, int maxiter,
+                              ii_field* xq, ii_solver_output* out);
+
 /* ------------------------------------------------------ measurement */
 /* Times one even/odd hop (Doe, V/2 output sites) in the given precision
  * with HIP events on the context stream, over nrep launches after a
  * warm-up; nrhs > 1 times the fused multi-rhs hop.  avg_ms per launch. */
 ii_status ii_bench_hop(ii_context* ctx, int precision, int nrhs, int nrep, double* avg_ms);
+/* Launch-shape autotuner: times the admissible site traversals of the
+ * single-rhs hop on this lattice (run automatically before the first FP64
+ * single-rhs hop) and reports the timings and the choice as text. */
+ii_status ii_autotune(ii_context* ctx, char* report, size_t len);
 /* Times a plain device copy of `bytes` bytes (read + write) with HIP
  * events: the achievable-bandwidth reference for the roofline and the
  * calibration kernel of the FETCH_SIZE / WRITE_SIZE counters. */

@@ -10,6 +10,7 @@ by the C ABI of libiroiro_hip.so (include/iroiro_hip.h):
   MultiShiftSolver_CG    lib/Solver/multiShiftSolver_CG.cpp:10-185
   Inverter_WilsonLike    lib/Dirac_ops/eoUtils.cpp:43-80
   Qprop_EvenOdd          lib/Measurements/FermionicM/qprop_EvenOdd.cpp:11-43
+  Qprop_MultiShift       lib/Measurements/FermionicM/qprop_MultiShift.cpp:10-31
   Source_local           lib/Measurements/FermionicM/source_types.hpp:23-62
 
 Fields are float64 numpy arrays in the reference layouts (Format_F
@@ -27,7 +28,8 @@ from ._lib import (ConvergenceErr, IIError, NotConverged, II_EVEN, II_ODD, II_FU
 
 __all__ = [
     "Context", "Dirac_Wilson_EvenOdd", "Dirac_Wilson", "Fopr_DdagD", "Solver_CG",
-    "MultiShiftSolver_CG", "Inverter_WilsonLike", "Qprop_EvenOdd", "Source_local",
+    "MultiShiftSolver_CG", "Inverter_WilsonLike", "Qprop_EvenOdd", "Qprop_MultiShift",
+    "Source_local",
     "SolverOutput", "DeviceField", "PropagatorSolver", "ConvergenceErr", "NotConverged",
     "IIError", "kappa_from_mass", "esec_osec", "LearnedPreconditioner", "Solver_FGCR",
 ]
@@ -141,6 +143,12 @@ class Context:
         g = ctypes.c_double()
         self.check(_lib.load().ii_device_info(self._h, buf, 256, ctypes.byref(g)), "ii_device_info")
         return buf.value.decode(), g.value
+
+    def autotune(self):
+        """Re-runs the launch-shape autotuner; returns its report."""
+        buf = ctypes.create_string_buffer(1024)
+        self.check(_lib.load().ii_autotune(self._h, buf, 1024), "ii_autotune")
+        return buf.value.decode()
 
     def bench_copy(self, nbytes, nrep=20):
         """Average ms of a plain device copy of nbytes (read + write)."""
@@ -322,9 +330,11 @@ class MultiShiftSolver_CG:
         sh = _lib.as_f64(shifts)
         x = np.empty(len(sh) * D.fsize())
         out = _lib.SolverOutputC()
-        st = _lib.load().ii_multishift_cg(D.ctx.handle, D.kpp, _lib.dptr(x), _lib.dptr(b),
-                                          _lib.dptr(sh), len(sh), self.prec, self.MaxIter,
-                                          ctypes.byref(out))
+        # the even/odd Schur operator or the full-index one (Qprop_MultiShift)
+        fn = (_lib.load().ii_multishift_cg if isinstance(D, Dirac_Wilson_EvenOdd)
+              else _lib.load().ii_multishift_cg_full)
+        st = fn(D.ctx.handle, D.kpp, _lib.dptr(x), _lib.dptr(b), _lib.dptr(sh), len(sh),
+                self.prec, self.MaxIter, ctypes.byref(out))
         xs = x.reshape(len(sh), D.fsize())
         for i in range(len(sh)):
             solution[i][...] = xs[i]
@@ -388,6 +398,39 @@ class Qprop_EvenOdd:
                 sol = np.empty(24 * self.inv.D.ctx.V)
                 self.inv.invert(sol, src.mksrc(s, c))
                 xq.append(sol)
+        return xq
+
+
+class Qprop_MultiShift:
+    """Qprop_MultiShift::calc (qprop_MultiShift.cpp:10-31): for each column
+    3*s + c a multishift solve of (D^dag D + sigma_m) x_m = src on the
+    full-index operator, then xq[(3s+c)*nshift + m] = D x_m.  Runs on the
+    device (one call for all 12 columns)."""
+
+    def __init__(self, D, solver):
+        if not isinstance(D, Dirac_Wilson):
+            raise TypeError("Qprop_MultiShift takes the full-index Dirac_Wilson")
+        self.D = D
+        self.slv = solver
+
+    def calc(self, source, mass_shifts):
+        ctx = self.D.ctx
+        sh = _lib.as_f64(mass_shifts)
+        n = len(sh)
+        xqf = DeviceField(ctx, "full", 12 * n)
+        outs = (_lib.SolverOutputC * 12)()
+        site = (ctypes.c_int * 4)(*source.pos)
+        try:
+            st = _lib.load().ii_qprop_multishift(ctx.handle, self.D.kpp, site, _lib.dptr(sh), n,
+                                                 self.slv.prec, self.slv.MaxIter, xqf.handle,
+                                                 outs)
+            if st == _lib.II_ERR_NOT_CONVERGED:
+                raise NotConverged(st, "Not converged.")
+            ctx.check(st, "Qprop_MultiShift")
+            xq = [xqf.download(k) for k in range(12 * n)]
+        finally:
+            xqf.close()
+        self.outputs = [SolverOutput(o) for o in outs]
         return xq
 
 

@@ -27,11 +27,12 @@ EXPORTS = [
     "ii_dirac_mult", "ii_dirac_mult_dag", "ii_dirac_gamma5", "ii_fopr_ddagd_mult",
     "ii_dirac_full_mult", "ii_solver_cg", "ii_multishift_cg", "ii_eo_invert",
     "ii_field_create", "ii_field_destroy", "ii_field_upload", "ii_field_download",
-    "ii_field_point_sources", "ii_propagator_solve",
+    "ii_field_point_sources", "ii_propagator_solve", "ii_multishift_cg_full",
+    "ii_qprop_multishift",
     "ii_net_init", "ii_net_set_weights", "ii_net_get_weights", "ii_net_layer_apply",
     "ii_net_layer_wgrad", "ii_net_loss_grad", "ii_net_apply", "ii_net_train", "ii_net_save",
     "ii_net_load", "ii_fgcr_solve",
-    "ii_bench_hop", "ii_bench_copy",
+    "ii_autotune", "ii_bench_hop", "ii_bench_copy",
 ]
 
 NET_LAYERS = 4
@@ -120,6 +121,8 @@ def load():
         "ii_solver_cg": [vp, cd, dp, dp, cd, ci, so],
         "ii_multishift_cg": [vp, cd, dp, dp, dp, ci, cd, ci, so],
         "ii_eo_invert": [vp, cd, dp, dp, cd, ci, so],
+        "ii_multishift_cg_full": [vp, cd, dp, dp, dp, ci, cd, ci, so],
+        "ii_qprop_multishift": [vp, cd, ip, dp, ci, cd, ci, vp, so],
         "ii_field_create": [vp, ci, ci, ctypes.POINTER(vp)],
         "ii_field_destroy": [vp],
         "ii_field_upload": [vp, ci, dp],
@@ -137,6 +140,7 @@ def load():
         "ii_net_save": [vp, ctypes.c_char_p],
         "ii_net_load": [vp, ctypes.c_char_p],
         "ii_fgcr_solve": [vp, cd, dp, dp, cd, ci, ci, ci, so],
+        "ii_autotune": [vp, ctypes.c_char_p, ctypes.c_size_t],
         "ii_bench_hop": [vp, ci, ci, ci, dp],
         "ii_bench_copy": [vp, ctypes.c_size_t, ci, dp],
     }

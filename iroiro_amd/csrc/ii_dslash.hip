@@ -11,6 +11,10 @@
 #include "ii_ops.h"
 #include "ii_stencil.h"
 
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
 namespace ii {
 
 template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Gauge,
@@ -18,11 +22,11 @@ template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Ga
 __global__ void __launch_bounds__(kBlock)
 k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
       typename SpIn::R a, bool tbc_flip, double* __restrict__ partials,
-      const int* __restrict__ skip, int skip_stride) {
+      const int* __restrict__ skip, int skip_stride, HopOrder order) {
   using C = typename SpIn::C;
   const int nblk = gridDim.x;
-  const int blk = xcd_remap(blockIdx.x, nblk);
-  const int hs = blk * blockDim.x + threadIdx.x;
+  const int base = hop_block_base(g, order, blockIdx.x, nblk, kBlock);
+  const int hs = base + threadIdx.x;
   const int rhs = blockIdx.y;
   if (skip && skip[rhs * skip_stride]) return;  // uniform per block
   double dsum = 0.0;
@@ -60,7 +64,8 @@ k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
   }
   if (DOT) {
     double s = block_sum(dsum);
-    if (threadIdx.x == 0) partials[(size_t)rhs * nblk + blockIdx.x] = s;
+    // indexed by the site range, so sums do not depend on the traversal
+    if (threadIdx.x == 0) partials[(size_t)rhs * nblk + base / kBlock] = s;
   }
 }
 
@@ -196,7 +201,7 @@ ii_status launch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, const 
     dim3 grid(blocks_for(g.Vh), 1);
     hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX>), grid,
                        dim3(kBlock), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a, flip,
-                       partials, skip, skip_stride);
+                       partials, skip, skip_stride, ctx->hop_order);
   } else {
     constexpr int TS = MrhsShape<Gauge>::TS, MAXT = MrhsShape<Gauge>::MAXT;
     const int rb = std::min(nrhs, MAXT / TS);
@@ -245,9 +250,88 @@ int hop_dot_parts(const Geom& g, int nrhs, int precision) {
   return (g.Vh + ts - 1) / ts;
 }
 
+// Autotuner of the single-rhs hop traversal (HopOrder): times every
+// admissible order on this lattice with the FP64 hop and keeps the fastest.
+// Runs once per context on the first single-rhs FP64 hop, never inside a
+// stream capture.  II_HOP_ORDER=0 or II_HOP_ORDER=1:W forces an order.
+ii_status tune_hop(ii_context* ctx) {
+  ctx->hop_tuned = true;
+  const Geom& g = ctx->g;
+  const int P = g.Lxh * g.L[1];
+  std::vector<HopOrder> cands(1);
+  const bool tiled = P % kBlock == 0 && g.L[3] % 8 == 0 && g.Vh >= 8 * kBlock;
+  if (tiled)
+    for (int W : {1, 2, 4, 8, 16})
+      if ((g.L[3] / 8) % W == 0) cands.push_back(HopOrder{1, W, P / kBlock});
+  if (const char* env = std::getenv("II_HOP_ORDER")) {
+    HopOrder o;
+    if (env[0] == '1' && tiled) {
+      o.mode = 1;
+      o.W = env[1] == ':' ? std::atoi(env + 2) : 4;
+      o.bpp = P / kBlock;
+      if (o.W < 1 || (g.L[3] / 8) % o.W) o = HopOrder();
+    }
+    ctx->hop_order = o;
+    ctx->tune_report = "hop order forced by II_HOP_ORDER";
+    return II_OK;
+  }
+  if (cands.size() == 1) {
+    ctx->tune_report = "hop order: contiguous per XCD (lattice not tileable)";
+    return II_OK;
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  II_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) {
+    ctx->hop_tuned = false;
+    return II_OK;
+  }
+  const size_t he = half_elems(g);
+  double2 *a = nullptr, *b = nullptr;
+  II_HIP(ctx, hipMalloc(&a, he * sizeof(double2)));
+  if (hipMalloc(&b, he * sizeof(double2)) != hipSuccess) {
+    (void)hipFree(a);
+    return set_error(ctx, II_ERR_DEVICE, "autotuner: out of device memory");
+  }
+  ii_status st = II_OK;
+  std::string rep = "hop order ms:";
+  float best = 1e30f;
+  HopOrder pick;
+  if (hipMemsetAsync(a, 0, he * sizeof(double2), ctx->stream) != hipSuccess)
+    st = set_error(ctx, II_ERR_DEVICE, "autotuner: memset failed");
+  HopSpec h;
+  h.a = -0.125;
+  for (size_t i = 0; i < cands.size() && st == II_OK; ++i) {
+    ctx->hop_order = cands[i];
+    for (int r = 0; r < 2 && st == II_OK; ++r) st = hop64(ctx, h, a, b, nullptr, nullptr);
+    if (st != II_OK) break;
+    float ms = 0.f;
+    if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) st = II_ERR_DEVICE;
+    for (int r = 0; r < 10 && st == II_OK; ++r) st = hop64(ctx, h, a, b, nullptr, nullptr);
+    if (st == II_OK && (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess ||
+                        hipEventSynchronize(ctx->ev1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess))
+      st = set_error(ctx, II_ERR_DEVICE, "autotuner: event timing failed");
+    ms /= 10.f;
+    char buf[64];
+    std::snprintf(buf, sizeof buf, " %s%d=%.4f", cands[i].mode ? "strip" : "contig",
+                  cands[i].mode ? cands[i].W : 0, ms);
+    rep += buf;
+    if (ms < best) {
+      best = ms;
+      pick = cands[i];
+    }
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  ctx->hop_order = pick;
+  ctx->tune_report = rep;
+  return st;
+}
+
 ii_status hop64(ii_context* ctx, const HopSpec& h, const double2* in, double2* out,
                 const double2* x, const double2* dv) {
   const Geom& g = ctx->g;
+  if (h.nrhs == 1 && !ctx->hop_tuned) II_CHECK(tune_hop(ctx));
   SpF64 sin{in, g.stride};
   GaugeF64 gf{ctx->gauge64, g.stride};
   StF64 so{out, g.stride};

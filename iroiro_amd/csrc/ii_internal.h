@@ -39,6 +39,18 @@ struct Geom {
   int stride;      // per-component stride of a half field (multiple of 64)
 };
 
+// Site traversal of the single-rhs hop, chosen per lattice by the
+// autotuner (ii_dslash.hip).  mode 0: the XCD that runs block b (b % 8)
+// takes a contiguous range of half sites.  mode 1: XCD x takes the time
+// slices [x*Lt/8, (x+1)*Lt/8) and walks them in strips of W slices, z-planes
+// outer and t inner, so that the t- and z-neighbour planes of a block were
+// read a few planes earlier into the same L2.  bpp: blocks per (z,t) plane.
+struct HopOrder {
+  int mode = 0;
+  int W = 4;
+  int bpp = 1;
+};
+
 // Solver scalars kept on the device so that iterations need no host sync.
 struct CGScalars {
   double rr;       // |r|^2 of the current iterate
@@ -82,6 +94,10 @@ struct ii_context {
   ii::CGScalars* scal_host = nullptr;  // pinned mirror
   int nscal = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // autotuned launch shapes
+  ii::HopOrder hop_order;
+  bool hop_tuned = false;
+  std::string tune_report;
   // learned preconditioner state (owned by ii_learned.hip)
   void* learned = nullptr;
   // live device fields of this context: destroying the context releases

@@ -40,6 +40,8 @@ ii_status hop_bf16(ii_context* ctx, const HopSpec& h, const LoField& in, const L
                    const LoField* x, const LoField* dv);
 
 inline int hop_blocks(const Geom& g) { return blocks_for(g.Vh); }
+// autotuner of the single-rhs hop traversal (ii_dslash.hip)
+ii_status tune_hop(ii_context* ctx);
 // number of per-rhs partial sums a DOT hop writes for nrhs right-hand sides
 int hop_dot_parts(const Geom& g, int nrhs, int precision);
 

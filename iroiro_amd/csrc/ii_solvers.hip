@@ -604,7 +604,11 @@ __global__ void k_ms_init(MSScal* S, const double* __restrict__ rrbb, const doub
   }
 }
 
+// The multishift kernels run on even half fields (NC = 12 complex
+// components per site) or on full fields of the full-index operator
+// (NC = 24: both parity blocks, [2][12][stride] = [24][stride]).
 // s += sigma0 p0 ; partial <s, p0>      (multiShiftSolver_CG.cpp:50-55)
+template <int NC>
 __global__ void __launch_bounds__(kBlock)
 k_ms_sp(Geom g, double2* __restrict__ s, const double2* __restrict__ p0, const MSScal* S,
         double* __restrict__ partials) {
@@ -614,7 +618,7 @@ k_ms_sp(Geom g, double2* __restrict__ s, const double2* __restrict__ p0, const M
   double acc = 0.0;
   if (hs < g.Vh) {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
+    for (int k = 0; k < NC; ++k) {
       size_t i = (size_t)k * g.stride + hs;
       double2 sv = s[i], pv = p0[i];
       sv.x = fma(sig, pv.x, sv.x);
@@ -640,6 +644,7 @@ __global__ void k_ms_fin_pap(const double* __restrict__ partials, int nb, MSScal
 }
 
 // x0 -= beta p0 ; r += beta s ; partial |r|^2   (:57-59)
+template <int NC>
 __global__ void __launch_bounds__(kBlock)
 k_ms_xr(Geom g, double2* __restrict__ x0, const double2* __restrict__ p0, double2* __restrict__ r,
         const double2* __restrict__ s, const MSScal* S, double* __restrict__ partials) {
@@ -649,7 +654,7 @@ k_ms_xr(Geom g, double2* __restrict__ x0, const double2* __restrict__ p0, double
   double acc = 0.0;
   if (hs < g.Vh) {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
+    for (int k = 0; k < NC; ++k) {
       size_t i = (size_t)k * g.stride + hs;
       double2 xv = x0[i], pv = p0[i], rv = r[i], sv = s[i];
       xv.x = fma(-b, pv.x, xv.x);
@@ -693,18 +698,19 @@ __global__ void k_ms_fin_rr(const double* __restrict__ partials, int nb, MSScal*
 
 // p0 = alpha p0 + r (:63-64) and, for ish in [1, nshift2):
 // x_i -= betas p_i ; p_i = alphas p_i + zeta r ; partial |p_i|^2 (:73-79)
+template <int NC>
 __global__ void __launch_bounds__(kBlock)
 k_ms_vec(Geom g, double2* __restrict__ x, double2* __restrict__ p, const double2* __restrict__ r,
          const MSScal* S, double* __restrict__ partials) {
   if (S->done) return;
   const int ish = blockIdx.y;
   const int hs = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t he = (size_t)12 * g.stride;
+  const size_t he = (size_t)NC * g.stride;
   if (ish == 0) {
     if (hs < g.Vh) {
       const double a = S->alpha;
 #pragma unroll
-      for (int k = 0; k < 12; ++k) {
+      for (int k = 0; k < NC; ++k) {
         size_t i = (size_t)k * g.stride + hs;
         double2 pv = p[i], rv = r[i];
         p[i] = make_double2(fma(pv.x, a, rv.x), fma(pv.y, a, rv.y));
@@ -719,7 +725,7 @@ k_ms_vec(Geom g, double2* __restrict__ x, double2* __restrict__ p, const double2
     double2* xi = x + ish * he;
     double2* pi = p + ish * he;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
+    for (int k = 0; k < NC; ++k) {
       size_t i = (size_t)k * g.stride + hs;
       double2 xv = xi[i], pv = pi[i], rv = r[i];
       xv.x = fma(-bs, pv.x, xv.x);
@@ -762,10 +768,44 @@ __global__ void k_ms_tail(const double* __restrict__ partials, int nb, MSScal* S
   }
 }
 
+// A = D^dag D of the full-index operator D = 1 - kappa H on full fields
+// [2][12][stride] (Fopr_DdagD over Dirac_Wilson, fopr.h:146-153,
+// dirac_wilson.cpp:28-36, 115-118); w: full-field temporary.
+ii_status ddagd_full64(ii_context* ctx, double kappa, const double2* f, double2* out, double2* w,
+                       const int* skip) {
+  const size_t he = half_elems(ctx->g);
+  HopSpec h;
+  h.a = -kappa;
+  h.xpay = true;
+  h.skip = skip;
+  for (int par = 0; par < 2; ++par) {
+    h.out_par = par;
+    II_CHECK(hop64(ctx, h, f + (1 - par) * he, w + par * he, f + par * he, nullptr));
+  }
+  h.g5in = h.g5out = true;
+  for (int par = 0; par < 2; ++par) {
+    h.out_par = par;
+    II_CHECK(hop64(ctx, h, w + (1 - par) * he, out + par * he, w + par * he, nullptr));
+  }
+  return II_OK;
+}
+
+__global__ void k_sum2(const double* __restrict__ in, int n, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += in[i];
+    out[0] = s;
+  }
+}
+
+// multishift on the even Schur system (full = false: x, b even half fields)
+// or on the full-index operator (full = true: full fields)
 ii_status multishift_reference(ii_context* ctx, double kappa, double2* x, const double2* b,
                                const double* shifts, int nshift, double prec, int maxiter,
-                               ii_solver_output* out, std::vector<double>* per_shift) {
+                               ii_solver_output* out, std::vector<double>* per_shift,
+                               bool full) {
   const Geom& g = ctx->g;
+  const int nh = full ? 2 : 1;  // half-field blocks per vector
   if (nshift < 1 || nshift > kMaxShift) return set_error(ctx, II_ERR_BAD_ARG, "nshift out of range");
   int nb = hop_blocks(g);
   II_CHECK(ensure_partials(ctx, (size_t)nb * nshift + 16));
@@ -778,21 +818,34 @@ ii_status multishift_reference(ii_context* ctx, double kappa, double2* x, const 
   II_HIP(ctx, hipHostMalloc(&Sh, sizeof(MSScal)));
   struct Free { MSScal* p; ~Free() { if (p) (void)hipHostFree(p); } } fr_{Sh};
   SF fr, fs, fq, ft, fp;
-  II_CHECK(fieldP(ctx, 10, 0, 1, &fr));
-  II_CHECK(fieldP(ctx, 12, 0, 1, &fs));
-  II_CHECK(fieldP(ctx, 13, 0, 1, &fq));
-  II_CHECK(fieldP(ctx, 14, 0, 1, &ft));
-  II_CHECK(fieldP(ctx, 15, 0, nshift, &fp));
-  size_t he = half_elems(g);
+  II_CHECK(fieldP(ctx, 10, 0, nh, &fr));
+  II_CHECK(fieldP(ctx, 12, 0, nh, &fs));
+  II_CHECK(fieldP(ctx, 13, 0, nh, &fq));
+  II_CHECK(fieldP(ctx, 14, 0, nh, &ft));
+  II_CHECK(fieldP(ctx, 15, 0, nshift * nh, &fp));
+  size_t he = half_elems(g) * nh;
   double2* p = (double2*)fp.v;
+  // |v|^2 over all nh blocks of a vector into dst (device)
+  auto norm2 = [&](const double2* v, double* dst) -> ii_status {
+    II_CHECK(dot64(ctx, v, v, aux + 8 + kMaxShift - 4, nh));
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(64), 0, ctx->stream, aux + 8 + kMaxShift - 4, nh,
+                       dst);
+    II_HIP(ctx, hipGetLastError());
+    return II_OK;
+  };
+  auto apply_A = [&](const double2* f, const int* skip) -> ii_status {
+    if (full) return ddagd_full64(ctx, kappa, f, (double2*)fs.v, (double2*)fq.v, skip);
+    return ddagdP(ctx, 0, kappa, SF{const_cast<double2*>(f)}, fs, fq, ft, nullptr, nullptr, 1,
+                  skip, 0);
+  };
   II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   for (int i = 0; i < nshift; ++i) {
-    II_CHECK(copy64(ctx, p + i * he, b));
-    II_CHECK(zero64(ctx, x + i * he));
+    II_CHECK(copy64(ctx, p + i * he, b, nh));
+    II_CHECK(zero64(ctx, x + i * he, nh));
   }
-  II_CHECK(copy64(ctx, (double2*)fr.v, b));
-  II_CHECK(dot64(ctx, b, b, aux));
-  II_CHECK(dot64(ctx, b, b, aux + 1));
+  II_CHECK(copy64(ctx, (double2*)fr.v, b, nh));
+  II_CHECK(norm2(b, aux));
+  II_CHECK(norm2(b, aux + 1));
   II_HIP(ctx, hipMemcpyAsync(aux + 2, shifts, nshift * sizeof(double), hipMemcpyHostToDevice,
                              ctx->stream));
   hipLaunchKernelGGL(k_ms_init, dim3(1), dim3(64), 0, ctx->stream, S, aux, aux + 2, nshift, prec);
@@ -803,15 +856,26 @@ ii_status multishift_reference(ii_context* ctx, double kappa, double2* x, const 
   while (queued < maxiter) {
     int n = std::min(chunk, maxiter - queued);
     for (int it = 0; it < n; ++it) {
-      II_CHECK(ddagdP(ctx, 0, kappa, SF{p}, fs, fq, ft, nullptr, nullptr, 1, skip, 0));
-      hipLaunchKernelGGL(k_ms_sp, dim3(nb), dim3(kBlock), 0, ctx->stream, g, (double2*)fs.v, p, S,
-                         part);
-      hipLaunchKernelGGL(k_ms_fin_pap, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
-      hipLaunchKernelGGL(k_ms_xr, dim3(nb), dim3(kBlock), 0, ctx->stream, g, x, p,
-                         (double2*)fr.v, (const double2*)fs.v, S, part);
-      hipLaunchKernelGGL(k_ms_fin_rr, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
-      hipLaunchKernelGGL(k_ms_vec, dim3(nb, nshift), dim3(kBlock), 0, ctx->stream, g, x, p,
-                         (const double2*)fr.v, S, part);
+      II_CHECK(apply_A(p, skip));
+      if (full) {
+        hipLaunchKernelGGL(k_ms_sp<24>, dim3(nb), dim3(kBlock), 0, ctx->stream, g,
+                           (double2*)fs.v, p, S, part);
+        hipLaunchKernelGGL(k_ms_fin_pap, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
+        hipLaunchKernelGGL(k_ms_xr<24>, dim3(nb), dim3(kBlock), 0, ctx->stream, g, x, p,
+                           (double2*)fr.v, (const double2*)fs.v, S, part);
+        hipLaunchKernelGGL(k_ms_fin_rr, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
+        hipLaunchKernelGGL(k_ms_vec<24>, dim3(nb, nshift), dim3(kBlock), 0, ctx->stream, g, x, p,
+                           (const double2*)fr.v, S, part);
+      } else {
+        hipLaunchKernelGGL(k_ms_sp<12>, dim3(nb), dim3(kBlock), 0, ctx->stream, g,
+                           (double2*)fs.v, p, S, part);
+        hipLaunchKernelGGL(k_ms_fin_pap, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
+        hipLaunchKernelGGL(k_ms_xr<12>, dim3(nb), dim3(kBlock), 0, ctx->stream, g, x, p,
+                           (double2*)fr.v, (const double2*)fs.v, S, part);
+        hipLaunchKernelGGL(k_ms_fin_rr, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
+        hipLaunchKernelGGL(k_ms_vec<12>, dim3(nb, nshift), dim3(kBlock), 0, ctx->stream, g, x, p,
+                           (const double2*)fr.v, S, part);
+      }
       hipLaunchKernelGGL(k_ms_tail, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, S);
       II_HIP(ctx, hipGetLastError());
     }
@@ -826,10 +890,10 @@ ii_status multishift_reference(ii_context* ctx, double kappa, double2* x, const 
   double dmax = -1.0;
   if (per_shift) per_shift->assign(nshift, 0.0);
   for (int i = 0; i < nshift; ++i) {
-    II_CHECK(ddagdP(ctx, 0, kappa, SF{x + i * he}, fs, fq, ft, nullptr, nullptr, 1, nullptr, 0));
-    II_CHECK(axpby64(ctx, shifts[i], x + i * he, 1.0, (double2*)fs.v));
-    II_CHECK(axpby64(ctx, -1.0, b, 1.0, (double2*)fs.v));
-    II_CHECK(dot64(ctx, (double2*)fs.v, (double2*)fs.v, aux));
+    II_CHECK(apply_A(x + i * he, nullptr));
+    II_CHECK(axpby64(ctx, shifts[i], x + i * he, 1.0, (double2*)fs.v, nh));
+    II_CHECK(axpby64(ctx, -1.0, b, 1.0, (double2*)fs.v, nh));
+    II_CHECK(norm2((double2*)fs.v, aux));
     double h = 0.0;
     II_HIP(ctx, hipMemcpyAsync(&h, aux, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     II_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1163,12 +1227,82 @@ ii_status ii_multishift_cg(ii_context* ctx, double kappa, double* x, const doubl
   Del d2{fx};
   II_CHECK(ii_field_upload(fb, 0, b));
   ii_status st = multishift_reference(ctx, kappa, fx->data, fb->data, shifts, nshift, prec,
-                                      maxiter, out, nullptr);
+                                      maxiter, out, nullptr, false);
   if (st != II_OK && st != II_ERR_NOT_CONVERGED) return st;
   std::string msg = ctx->err;
   for (int i = 0; i < nshift; ++i) II_CHECK(ii_field_download(fx, i, x + (size_t)24 * g.Vh * i));
   ctx->err = msg;
   return st;
+}
+
+ii_status ii_multishift_cg_full(ii_context* ctx, double kappa, double* x, const double* b,
+                                const double* shifts, int nshift, double prec, int maxiter,
+                                ii_solver_output* out) {
+  if (!x || !b || !shifts || nshift < 1 || maxiter < 1) return II_ERR_BAD_ARG;
+  II_CHECK(require_gauge(ctx));
+  II_CHECK(enter(ctx));
+  const Geom& g = ctx->g;
+  ii_field *fb = nullptr, *fx = nullptr;
+  II_CHECK(ii_field_create(ctx, II_FULL, 1, &fb));
+  struct Del { ii_field* f; ~Del() { if (f) ii_field_destroy(f); } } d1{fb};
+  II_CHECK(ii_field_create(ctx, II_FULL, nshift, &fx));
+  Del d2{fx};
+  II_CHECK(ii_field_upload(fb, 0, b));
+  ii_status st = multishift_reference(ctx, kappa, fx->data, fb->data, shifts, nshift, prec,
+                                      maxiter, out, nullptr, true);
+  if (st != II_OK && st != II_ERR_NOT_CONVERGED) return st;
+  std::string msg = ctx->err;
+  for (int i = 0; i < nshift; ++i) II_CHECK(ii_field_download(fx, i, x + (size_t)24 * g.V * i));
+  ctx->err = msg;
+  return st;
+}
+
+ii_status ii_qprop_multishift(ii_context* ctx, double kappa, const int site[4],
+                              const double* shifts, int nshift, double prec, int maxiter,
+                              ii_field* xq, ii_solver_output* out) {
+  if (!site || !shifts || !xq || !out || nshift < 1 || maxiter < 1) return II_ERR_BAD_ARG;
+  if (xq->ctx != ctx || xq->subset != II_FULL || xq->nrhs != 12 * nshift)
+    return set_error(ctx, II_ERR_BAD_ARG, "xq must be a full field of 12*nshift rhs");
+  II_CHECK(require_gauge(ctx));
+  II_CHECK(enter(ctx));
+  const size_t he = half_elems(ctx->g);
+  ii_field* src = nullptr;
+  II_CHECK(ii_field_create(ctx, II_FULL, 12, &src));
+  struct Del { ii_field* f; ~Del() { if (f) ii_field_destroy(f); } } d1{src};
+  II_CHECK(ii_field_point_sources(src, site));
+  ii_status worst = II_OK;
+  std::string msg;
+  for (int col = 0; col < 12; ++col) {
+    double2* x = xq->data + (size_t)col * nshift * 2 * he;
+    ii_status st = multishift_reference(ctx, kappa, x, src->data + (size_t)col * 2 * he, shifts,
+                                        nshift, prec, maxiter, &out[col], nullptr, true);
+    if (st == II_ERR_NOT_CONVERGED) {
+      worst = st;
+      msg = ctx->err;
+    } else if (st != II_OK) {
+      return st;
+    }
+    // xq[(3s+c)*nshift + m] = D x_m  (qprop_MultiShift.cpp:25-28)
+    double2* w = nullptr;
+    {
+      void* p;
+      II_CHECK(workspace(ctx, 21, 2 * he * sizeof(double2), &p));
+      w = (double2*)p;
+    }
+    for (int m = 0; m < nshift; ++m) {
+      double2* xm = x + (size_t)m * 2 * he;
+      HopSpec h;
+      h.a = -kappa;
+      h.xpay = true;
+      for (int par = 0; par < 2; ++par) {
+        h.out_par = par;
+        II_CHECK(hop64(ctx, h, xm + (1 - par) * he, w + par * he, xm + par * he, nullptr));
+      }
+      II_CHECK(copy64(ctx, xm, w, 2));
+    }
+  }
+  if (worst != II_OK) return set_error(ctx, worst, msg);
+  return II_OK;
 }
 
 ii_status ii_eo_invert(ii_context* ctx, double kappa, double* sol, const double* src, double prec,
@@ -1238,6 +1372,18 @@ ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_param
   for (int k = 0; k < src->nrhs; ++k) out[k].true_residual = res[k];
   ctx->err = msg;
   return st;
+}
+
+ii_status ii_autotune(ii_context* ctx, char* report, size_t len) {
+  if (!ctx) return II_ERR_BAD_ARG;
+  II_CHECK(require_gauge(ctx));
+  II_CHECK(enter(ctx));
+  ctx->hop_tuned = false;
+  II_CHECK(tune_hop(ctx));
+  if (report && len) {
+    std::snprintf(report, len, "%s", ctx->tune_report.c_str());
+  }
+  return II_OK;
 }
 
 ii_status ii_bench_copy(ii_context* ctx, size_t bytes, int nrep, double* avg_ms) {

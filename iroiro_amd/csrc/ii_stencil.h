@@ -427,4 +427,21 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
   return (b % nx) * (nblocks / nx) + b / nx;
 }
 
+// First half site of the logical block that physical block b handles.
+__device__ __forceinline__ int hop_block_base(const Geom& g, const HopOrder& o, int b, int nblk,
+                                              int bs) {
+  if (o.mode == 1) {
+    const int x = b & 7, j = b >> 3;
+    const int bb = j % o.bpp;
+    int r = j / o.bpp;
+    const int tt = r % o.W;
+    r /= o.W;
+    const int z = r % g.L[2];
+    const int strip = r / g.L[2];
+    const int t = x * (g.L[3] >> 3) + strip * o.W + tt;
+    return ((z + g.L[2] * t) * o.bpp + bb) * bs;
+  }
+  return xcd_remap(b, nblk) * bs;
+}
+
 }  // namespace ii

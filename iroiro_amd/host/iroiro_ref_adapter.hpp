@@ -16,9 +16,13 @@
 //   MultiShiftSolver_CG_HIP : MultiShiftSolver
 //                                       (lib/Solver/multiShiftSolver.hpp:15-24;
 //                                        replaces multiShiftSolver_CG.cpp:10-185)
+//   Dirac_Wilson_HIP : DiracWilsonLike  (the full-index Dirac_Wilson,
+//                                        dirac_wilson.hpp:117-130, of
+//                                        Qprop_MultiShift)
 //
 // Inverter_WilsonLike (eoUtils.cpp:43-80), Qprop_EvenOdd
-// (qprop_EvenOdd.cpp:11-43), Action_Nf2 and RationalSolver_CG take these
+// (qprop_EvenOdd.cpp:11-43), Qprop_MultiShift (qprop_MultiShift.cpp:10-31),
+// Action_Nf2 and RationalSolver_CG take these
 // through their base-class pointers.  Error behaviour follows the
 // reference: CG non-convergence calls Errors::ConvergenceErr (which exits),
 // the multishift solver throws "Not converged.", device failures abort with
@@ -68,9 +72,23 @@ inline ii_context* context() {
   return ctx;
 }
 
+// What the device Fopr and solvers need from either device operator.
+class DeviceOp {
+ public:
+  virtual ~DeviceOp() {}
+  virtual ii_context* context() const = 0;
+  virtual double getKappa() const = 0;
+  virtual size_t fsize() const = 0;
+  virtual bool full_index() const = 0;
+  // D f and D^dag f on host fields of fsize() doubles
+  virtual const Field mult(const Field& f) const = 0;
+  virtual const Field mult_dag(const Field& f) const = 0;
+};
+
 }  // namespace iroiro_hip_detail
 
-class Dirac_Wilson_EvenOdd_HIP : public DiracWilsonLike_EvenOdd {
+class Dirac_Wilson_EvenOdd_HIP : public DiracWilsonLike_EvenOdd,
+                                 public iroiro_hip_detail::DeviceOp {
  private:
   double kpp_;
   const Field* u_;
@@ -97,6 +115,7 @@ class Dirac_Wilson_EvenOdd_HIP : public DiracWilsonLike_EvenOdd {
   double getKappa() const { return kpp_; }
   const Field* getGaugeField_ptr() const { return u_; }
   ii_context* context() const { return ctx_; }
+  bool full_index() const { return false; }
 
   // Dirac::update_internal_state (dirac.hpp:28): the caller changed U.
   void update_internal_state() {
@@ -132,14 +151,62 @@ class Dirac_Wilson_EvenOdd_HIP : public DiracWilsonLike_EvenOdd {
   }
 };
 
-// Fopr_DdagD on the device (fopr.h:146-153).
-class Fopr_DdagD_HIP : public Fopr_Herm {
-  const Dirac_Wilson_EvenOdd_HIP* D_;
+// Full-index Dirac_Wilson (dirac_wilson.hpp:117-130): D = 1 - kappa H on
+// full lexicographic fields (mult_full, dirac_wilson.cpp:115-118).
+class Dirac_Wilson_HIP : public DiracWilsonLike, public iroiro_hip_detail::DeviceOp {
+ private:
+  double kpp_;
+  const Field* u_;
+  ii_context* ctx_;
+  Dirac_Wilson cpu_;  // MD force only
+  Dirac_Wilson_HIP(const Dirac_Wilson_HIP&);
+
+  const Field full(const Field& f, int dag, const char* what) const {
+    Field w(fsize());
+    iroiro_hip_detail::check(
+        ctx_, ii_dirac_full_mult(ctx_, kpp_, w.getaddr(0), iroiro_hip_detail::addr(f), dag), what);
+    return w;
+  }
 
  public:
-  explicit Fopr_DdagD_HIP(const Dirac_Wilson_EvenOdd_HIP* D) : D_(D) {}
+  Dirac_Wilson_HIP(double mass, const Field* u)
+      : kpp_(0.5 / (4.0 + mass)), u_(u), ctx_(iroiro_hip_detail::context()), cpu_(mass, u) {
+    update_internal_state();
+  }
+  size_t fsize() const { return (size_t)CommonPrms::Nvol() * 24; }
+  size_t gsize() const { return (size_t)CommonPrms::Nvol() * 18 * 4; }
+  double getMass() const { return 0.5 / kpp_ - 4.0; }
+  double getKappa() const { return kpp_; }
+  const Field* getGaugeField_ptr() const { return u_; }
+  ii_context* context() const { return ctx_; }
+  bool full_index() const { return true; }
+  void update_internal_state() {
+    iroiro_hip_detail::check(ctx_, ii_gauge_upload(ctx_, iroiro_hip_detail::addr(*u_)),
+                             "ii_gauge_upload");
+  }
+  const Field mult(const Field& f) const { return full(f, 0, "mult"); }
+  const Field mult_dag(const Field& f) const { return full(f, 1, "mult_dag"); }
+  const Field gamma5(const Field& f) const {
+    Field w(f.size());
+    iroiro_hip_detail::check(ctx_, ii_dirac_gamma5(ctx_, w.getaddr(0), iroiro_hip_detail::addr(f),
+                                                   (long)(f.size() / 24)),
+                             "gamma5");
+    return w;
+  }
+  const Field md_force(const Field& eta, const Field& zeta) const {
+    return cpu_.md_force(eta, zeta);
+  }
+};
+
+// Fopr_DdagD on the device (fopr.h:146-153), over either device operator.
+class Fopr_DdagD_HIP : public Fopr_Herm {
+  const iroiro_hip_detail::DeviceOp* D_;
+
+ public:
+  explicit Fopr_DdagD_HIP(const iroiro_hip_detail::DeviceOp* D) : D_(D) {}
   double func(double x) const { return x; }
   const Field mult(const Field& f) const {
+    if (D_->full_index()) return D_->mult_dag(D_->mult(f));
     Field w(fsize());
     iroiro_hip_detail::check(D_->context(),
                              ii_fopr_ddagd_mult(D_->context(), D_->getKappa(), w.getaddr(0),
@@ -148,7 +215,7 @@ class Fopr_DdagD_HIP : public Fopr_Herm {
     return w;
   }
   size_t fsize() const { return D_->fsize(); }
-  const Dirac_Wilson_EvenOdd_HIP* dirac() const { return D_; }
+  const iroiro_hip_detail::DeviceOp* dirac() const { return D_; }
 };
 
 // Solver_CG with the whole iteration on the device (solver_CG.cpp:29-160):
@@ -164,8 +231,9 @@ class Solver_CG_HIP : public Solver {
       : opr_(fopr), prec_(prec), maxiter_(maxiter) {}
 
   SolverOutput solve(Field& xq, const Field& b) const {
-    const Dirac_Wilson_EvenOdd_HIP* D = opr_->dirac();
+    const iroiro_hip_detail::DeviceOp* D = opr_->dirac();
     ii_context* ctx = D->context();
+    if (D->full_index()) iroiro_hip_detail::die(ctx, II_ERR_BAD_ARG, "Solver_CG on the full index");
     if (xq.size() != b.size()) xq.resize(b.size());
     ii_solver_output o;
     ii_status st = ii_solver_cg(ctx, D->getKappa(), xq.getaddr(0), iroiro_hip_detail::addr(b),
@@ -198,13 +266,14 @@ class MultiShiftSolver_CG_HIP : public MultiShiftSolver {
 
   SolverOutput solve(vector_Field& xq, const Field& b, const vector_double& sigma, double& diff,
                      int& Nconv) const {
-    const Dirac_Wilson_EvenOdd_HIP* D = opr_->dirac();
+    const iroiro_hip_detail::DeviceOp* D = opr_->dirac();
     ii_context* ctx = D->context();
     const size_t n = b.size();
     std::vector<double> x(n * sigma.size());
     ii_solver_output o;
-    ii_status st = ii_multishift_cg(ctx, D->getKappa(), &x[0], iroiro_hip_detail::addr(b),
-                                    &sigma[0], (int)sigma.size(), prec_, maxiter_, &o);
+    ii_status st = (D->full_index() ? ii_multishift_cg_full : ii_multishift_cg)(
+        ctx, D->getKappa(), &x[0], iroiro_hip_detail::addr(b), &sigma[0], (int)sigma.size(),
+        prec_, maxiter_, &o);
     if (st == II_ERR_NOT_CONVERGED) throw "Not converged.";
     iroiro_hip_detail::check(ctx, st, "MultiShiftSolver_CG::solve");
     for (size_t i = 0; i < sigma.size(); ++i) {

@@ -58,14 +58,18 @@ echo "ref_build: built $OUT/iroiro_ref"
 REPO=$(cd "$HERE/.." && pwd)
 HIPLIB="$REPO/iroiro_amd/libiroiro_hip.so"
 if [ -f "$HIPLIB" ]; then
-  QSRC=lib/Measurements/FermionicM/qprop_EvenOdd.cpp
-  QOBJ="$OUT/obj/$(echo "$QSRC" | tr '/' '_' | sed 's/\.cpp$/.o/')"
-  if [ ! -f "$QOBJ" ] || [ "$REF/$QSRC" -nt "$QOBJ" ]; then
-    g++ $CXXFLAGS $INC -c "$REF/$QSRC" -o "$QOBJ"
-  fi
+  QOBJ=""
+  for QSRC in lib/Measurements/FermionicM/qprop_EvenOdd.cpp \
+              lib/Measurements/FermionicM/qprop_MultiShift.cpp; do
+    o="$OUT/obj/$(echo "$QSRC" | tr '/' '_' | sed 's/\.cpp$/.o/')"
+    if [ ! -f "$o" ] || [ "$REF/$QSRC" -nt "$o" ]; then
+      g++ $CXXFLAGS $INC -c "$REF/$QSRC" -o "$o"
+    fi
+    QOBJ="$QOBJ $o"
+  done
   g++ $CXXFLAGS $INC -I"$REPO/include" -I"$REPO/iroiro_amd/host" \
       -c "$REPO/tests/cpp/ref_adapter_driver.cpp" -o "$OUT/obj/ref_adapter_driver.o"
-  g++ -fopenmp -o "$OUT/iroiro_ref_hip" "$OUT/obj/ref_adapter_driver.o" $OBJS "$QOBJ" \
+  g++ -fopenmp -o "$OUT/iroiro_ref_hip" "$OUT/obj/ref_adapter_driver.o" $OBJS $QOBJ \
       -L"$REPO/iroiro_amd" -liroiro_hip -Wl,-rpath,'$ORIGIN/../../iroiro_amd'
   echo "ref_build: built $OUT/iroiro_ref_hip"
 fi

@@ -17,6 +17,7 @@
 #include "Solver/solver_CG.hpp"
 #include "Solver/multiShiftSolver_CG.hpp"
 #include "Measurements/FermionicM/qprop_EvenOdd.hpp"
+#include "Measurements/FermionicM/qprop_MultiShift.hpp"
 #include "Measurements/FermionicM/source_types.hpp"
 
 #include "iroiro_ref_adapter.hpp"
@@ -134,10 +135,31 @@ int main(int argc, char** argv) {
   double r_prop = 0.0;
   for (size_t k = 0; k < pc.size(); ++k) r_prop = std::max(r_prop, rel(pg[k], pc[k]));
 
+  // the reference's Qprop_MultiShift on the full-index operator
+  Dirac_Wilson DFcpu(mass, &u);
+  Dirac_Wilson_HIP DFgpu(mass, &u);
+  double r_full = std::max(rel(DFgpu.mult(src), DFcpu.mult(src)),
+                           rel(DFgpu.mult_dag(src), DFcpu.mult_dag(src)));
+  Fopr_DdagD AFcpu(&DFcpu);
+  Fopr_DdagD_HIP AFgpu(&DFgpu);
+  MultiShiftSolver_CG MFcpu(&AFcpu, prec, 2000);
+  MultiShiftSolver_CG_HIP MFgpu(&AFgpu, prec, 2000);
+  Qprop_MultiShift QMcpu(&DFcpu, &MFcpu);
+  Qprop_MultiShift QMgpu(&DFgpu, &MFgpu);
+  std::vector<double> mshifts;
+  mshifts.push_back(0.01);
+  mshifts.push_back(0.3);
+  prop_t qc, qg;
+  QMcpu.calc(qc, point, mshifts);
+  QMgpu.calc(qg, point, mshifts);
+  double r_qms = 0.0;
+  for (size_t k = 0; k < qc.size(); ++k) r_qms = std::max(r_qms, rel(qg[k], qc[k]));
+
   std::printf("IIHIP {\"ops_rel\": %.3e, \"cg_rel\": %.3e, \"cg_iter_cpu\": %d, "
               "\"cg_iter_gpu\": %d, \"ms_rel\": %.3e, \"ms_nconv_cpu\": %d, "
-              "\"ms_nconv_gpu\": %d, \"inv_rel\": %.3e, \"prop_rel\": %.3e, \"ncol\": %d}\n",
+              "\"ms_nconv_gpu\": %d, \"inv_rel\": %.3e, \"prop_rel\": %.3e, \"ncol\": %d, "
+              "\"full_ops_rel\": %.3e, \"qprop_ms_rel\": %.3e, \"qprop_ms_n\": %d}\n",
               r_ops, r_cg, oc.Iterations, og.Iterations, r_ms, nc, ng, r_inv, r_prop,
-              (int)pg.size());
+              (int)pg.size(), r_full, r_qms, (int)qg.size());
   return 0;
 }

@@ -1,5 +1,5 @@
 """Drop-in test of the C++ host side: the reference's own
-Inverter_WilsonLike, Qprop_EvenOdd (with Source_local), Solver and
+Inverter_WilsonLike, Qprop_EvenOdd and Qprop_MultiShift (with Source_local), Solver and
 MultiShiftSolver base-class calls, run once with the reference CPU classes
 and once with the device classes of iroiro_amd/host/iroiro_ref_adapter.hpp,
 in one process (oracle/_ref/iroiro_ref_hip, built by oracle/ref_build.sh)."""
@@ -35,3 +35,6 @@ def test_reference_drivers_on_device_classes(golden, tmp_path):
     assert r["ms_rel"] < 1e-12
     assert r["inv_rel"] < 1e-12
     assert r["ncol"] == 12 and r["prop_rel"] < 1e-12
+    # Qprop_MultiShift (qprop_MultiShift.cpp:10-31) on the full-index operator
+    assert r["full_ops_rel"] < 1e-13
+    assert r["qprop_ms_n"] == 24 and r["qprop_ms_rel"] < 1e-10

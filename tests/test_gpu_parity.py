@@ -296,3 +296,58 @@ def test_point_source_heatbath_16c32(precision):
     r = lat.true_residual_full(u, 0.1355, sol.download(7), b)
     assert r <= 1e-10
     ctx.close()
+
+
+def test_multishift_full_operator_vs_oracle():
+    """MultiShiftSolver_CG on Fopr_DdagD of the full-index Dirac_Wilson (the
+    operator of Qprop_MultiShift): every shifted system is solved, checked
+    with the oracle's full operator."""
+    dims = (4, 4, 4, 8)
+    lat = oracle.Lattice(dims)
+    u = lat.apbc(lat.gauge_expH(0.3, 31))
+    mass = 1.0 / 6.0
+    kappa = oracle.kappa_from_mass(mass)
+    ctx = ii.Context(dims)
+    try:
+        Df = ii.Dirac_Wilson(mass, u, ctx)
+        b = oracle.gauss_field(lat.full_size, 5)
+        shifts = [0.0, 0.02, 0.1, 0.5]
+        xs = [np.empty(lat.full_size) for _ in shifts]
+        so, diff, nconv = ii.MultiShiftSolver_CG(ii.Fopr_DdagD(Df), 1e-22, 2000).solve(
+            xs, b, shifts)
+        assert nconv > 0
+        worst = 0.0
+        for x, s in zip(xs, shifts):
+            ax = lat.dirac_full(u, lat.dirac_full(u, x, 0, kappa), 1, kappa) + s * x
+            worst = max(worst, rel(ax, b) ** 2)
+            assert rel(ax, b) < 1e-10
+        assert abs(diff - worst) <= 1e-3 * worst + 1e-30
+    finally:
+        ctx.close()
+
+
+def test_qprop_multishift_columns():
+    """Qprop_MultiShift::calc: xq[(3s+c)n + m] = D x_m with
+    (D^dag D + sigma_m) x_m = src(s,c); checked as (D D^dag + sigma_m) xq = D src."""
+    dims = (4, 4, 4, 8)
+    lat = oracle.Lattice(dims)
+    u = lat.apbc(lat.gauge_expH(0.3, 32))
+    mass = 1.0 / 6.0
+    kappa = oracle.kappa_from_mass(mass)
+    ctx = ii.Context(dims)
+    try:
+        Df = ii.Dirac_Wilson(mass, u, ctx)
+        shifts = [0.01, 0.2]
+        src = ii.Source_local((1, 2, 3, 4), dims)
+        q = ii.Qprop_MultiShift(Df, ii.MultiShiftSolver_CG(ii.Fopr_DdagD(Df), 1e-22, 2000))
+        xq = q.calc(src, shifts)
+        assert len(xq) == 12 * len(shifts)
+        for s in range(4):
+            for c in range(3):
+                dsrc = lat.dirac_full(u, src.mksrc(s, c), 0, kappa)
+                for m, sig in enumerate(shifts):
+                    y = xq[(3 * s + c) * len(shifts) + m]
+                    lhs = lat.dirac_full(u, lat.dirac_full(u, y, 1, kappa), 0, kappa) + sig * y
+                    assert rel(lhs, dsrc) < 1e-9, (s, c, m)
+    finally:
+        ctx.close()
