@@ -110,6 +110,10 @@ def main():
     ap.add_argument("--seed", type=int, default=20260601)
     ap.add_argument("--hop-only", action="store_true",
                     help="only time the hop kernels and the copy reference")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip training and timing the learned preconditioner")
+    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--train-pairs", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (one MI355X per box); "
@@ -155,6 +159,8 @@ def main():
     src = ii.DeviceField(ctx, "full", 12)
     sol = ii.DeviceField(ctx, "full", 12)
     src.point_sources((0, 0, 0, 0))
+    tune = ctx.autotune()
+    log("autotune:", tune)
 
     # ---- Dslash roofline (FP64 hop, one launch = V/2 output sites)
     Vh = ctx.Vh
@@ -200,6 +206,29 @@ def main():
                                    [o.Iterations for o in outs], [o.restarts for o in outs],
                                    max_res))
 
+    # ---- learned preconditioner: trained on this gauge field, flexible GCR
+    learned = None
+    if not args.no_learned:
+        net = ii.LearnedPreconditioner(ctx)
+        t0 = time.time()
+        rep = net.train(kappa, npairs=args.train_pairs, epochs=args.train_epochs, batch=4,
+                        lr=2e-3, pair_tol=1e-8, seed=args.seed)
+        train_s = time.time() - t0
+        log("learned: trained on %d pairs, %d epochs in %.1f s (pairs %.1f s): loss %.4f -> %.4f"
+            % (args.train_pairs, args.train_epochs, train_s, rep["pairs_ms"] / 1e3,
+               rep["loss_first"], rep["loss_last"]))
+        ls = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision="double", batched=False,
+                                 use_learned=True)
+        t0 = time.time()
+        louts = ls.solve(src, sol)
+        l_s = time.time() - t0
+        learned = {"s": l_s, "directions_per_rhs": [o.Iterations for o in louts],
+                   "cg_fallback_iterations": [o.inner_iterations for o in louts],
+                   "max_true_residual": max(o.true_residual for o in louts),
+                   "train": rep, "train_s": train_s}
+        log("learned 12-rhs solve: %.3f s, directions %s, max true residual %.2e"
+            % (l_s, learned["directions_per_rhs"], learned["max_true_residual"]))
+
     cpu = None
     if not args.no_cpu and fp64 is not None:
         n_it = int(round(sum(fp64["iterations_per_rhs"]) / 12.0))
@@ -239,6 +268,8 @@ def main():
                                      "achieved": lp_bytes / (hop_lp_ms * 1e-3) / 1e9,
                                      "unit": "GB/s", "bytes_per_site_per_rhs": 104 + 16},
         "cpu_baseline": cpu,
+        "learned": learned,
+        "autotune": tune,
     }
     print(json.dumps(result), flush=True)
     ctx.close()

@@ -140,6 +140,19 @@ ii_status ii_solver_cg(ii_context* ctx, double kappa, double* x, const double* b
 ii_status ii_multishift_cg(ii_context* ctx, double kappa, double* x, const double* b,
                            const double* shifts, int nshift, double prec, int maxiter,
                            ii_solver_output* out);
+/* Mixed-precision multishift on Fopr_DdagD(Dirac_Wilson_EvenOdd) + sigma_i,
+ * no reference counterpart (the reference's multishift is FP64 only,
+ * multiShiftSolver_CG.cpp:10-185).  Stage 1 runs the same shifted
+ * recursion, all shifts fused per launch, with FP32 vectors and operator to
+ * |r|/|b| < max(tol, 1e-6); stage 2 refines every shift at once with a
+ * batched reliable-update CG (inner precision II_PREC_*, FP64 true
+ * residuals) until |b - (A + sigma_i) x_i| <= tol |b|.  x: nshift even half
+ * fields; out: nshift entries (iterations = stage 1 + stage 2 of that
+ * shift, inner_iterations = stage 1, restarts = reliable updates,
+ * true_residual per shift). */
+ii_status ii_multishift_mixed(ii_context* ctx, double kappa, double* x, const double* b,
+                              const double* shifts, int nshift, double tol, int precision,
+                              int maxiter, ii_solver_output* out);
 /* MultiShiftSolver_CG::solve on Fopr_DdagD of the full-index Dirac_Wilson
  * (D^dag D + sigma_i, multiShiftSolver_CG.cpp:10-185; the operator of
  * Qprop_MultiShift, qprop_MultiShift.cpp:10-31).  b: one full field

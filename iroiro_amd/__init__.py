@@ -28,7 +28,7 @@ from ._lib import (ConvergenceErr, IIError, NotConverged, II_EVEN, II_ODD, II_FU
 
 __all__ = [
     "Context", "Dirac_Wilson_EvenOdd", "Dirac_Wilson", "Fopr_DdagD", "Solver_CG",
-    "MultiShiftSolver_CG", "Inverter_WilsonLike", "Qprop_EvenOdd", "Qprop_MultiShift",
+    "MultiShiftSolver_CG", "MultiShiftSolver_Mixed", "Inverter_WilsonLike", "Qprop_EvenOdd", "Qprop_MultiShift",
     "Source_local",
     "SolverOutput", "DeviceField", "PropagatorSolver", "ConvergenceErr", "NotConverged",
     "IIError", "kappa_from_mass", "esec_osec", "LearnedPreconditioner", "Solver_FGCR",
@@ -345,6 +345,34 @@ class MultiShiftSolver_CG:
         so.Msg = "Multishift CG solver"
         so.diff = out.true_residual
         return so, out.diff, out.iterations
+
+
+class MultiShiftSolver_Mixed:
+    """Mixed-precision multishift (no reference counterpart): all shifts
+    fused in an FP32 shifted recursion, then a batched reliable-update CG
+    per shift (inner precision `precision`, FP64 true residuals) until
+    |b - (A + sigma_i) x_i| <= tol |b| for every shift."""
+
+    def __init__(self, fopr, tol=1e-10, precision="single", maxiter=100000):
+        self.opr = fopr
+        self.tol = float(tol)
+        self.precision = PRECISIONS[precision]
+        self.maxiter = int(maxiter)
+
+    def solve(self, solution, source, shifts):
+        D = self.opr.D
+        b = _lib.as_f64(source, D.fsize())
+        sh = _lib.as_f64(shifts)
+        x = np.empty(len(sh) * D.fsize())
+        outs = (_lib.SolverOutputC * len(sh))()
+        st = _lib.load().ii_multishift_mixed(D.ctx.handle, D.kpp, _lib.dptr(x), _lib.dptr(b),
+                                             _lib.dptr(sh), len(sh), self.tol, self.precision,
+                                             self.maxiter, outs)
+        xs = x.reshape(len(sh), D.fsize())
+        for i in range(len(sh)):
+            solution[i][...] = xs[i]
+        D.ctx.check(st, "MultiShiftSolver_Mixed")
+        return [SolverOutput(o) for o in outs]
 
 
 class Inverter_WilsonLike:

@@ -28,6 +28,7 @@ EXPORTS = [
     "ii_dirac_full_mult", "ii_solver_cg", "ii_multishift_cg", "ii_eo_invert",
     "ii_field_create", "ii_field_destroy", "ii_field_upload", "ii_field_download",
     "ii_field_point_sources", "ii_propagator_solve", "ii_multishift_cg_full",
+    "ii_multishift_mixed",
     "ii_qprop_multishift",
     "ii_net_init", "ii_net_set_weights", "ii_net_get_weights", "ii_net_layer_apply",
     "ii_net_layer_wgrad", "ii_net_loss_grad", "ii_net_apply", "ii_net_train", "ii_net_save",
@@ -122,6 +123,7 @@ def load():
         "ii_multishift_cg": [vp, cd, dp, dp, dp, ci, cd, ci, so],
         "ii_eo_invert": [vp, cd, dp, dp, cd, ci, so],
         "ii_multishift_cg_full": [vp, cd, dp, dp, dp, ci, cd, ci, so],
+        "ii_multishift_mixed": [vp, cd, dp, dp, dp, ci, cd, ci, ci, so],
         "ii_qprop_multishift": [vp, cd, ip, dp, ci, cd, ci, vp, so],
         "ii_field_create": [vp, ci, ci, ctypes.POINTER(vp)],
         "ii_field_destroy": [vp],
