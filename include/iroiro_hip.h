@@ -260,6 +260,10 @@ ii_status ii_net_train(ii_context* ctx, double kappa, const ii_train_params* p,
                        ii_train_report* rep);
 ii_status ii_net_save(ii_context* ctx, const char* path);
 ii_status ii_net_load(ii_context* ctx, const char* path);
+/* Times every forward layer of the network (layer_ms[4], average over
+ * nrep launches, HIP events on the context stream; the last layer on the
+ * even output sites only, as M applies it). */
+ii_status ii_net_bench(ii_context* ctx, int nrep, double* layer_ms);
 /* Flexible GCR on Dhat x = b (even half fields), FP64 vectors and Dhat,
  * right preconditioner M (use_net != 0) or none, restarted every `nkrylov`
  * directions with an FP64 true residual; stops when |b - Dhat x| <= tol |b|.

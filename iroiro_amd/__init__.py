@@ -630,6 +630,13 @@ class LearnedPreconditioner:
         return {"loss_first": rep.loss_first, "loss_last": rep.loss_last,
                 "pairs_ms": rep.pairs_ms, "train_ms": rep.train_ms, "steps": rep.steps}
 
+    def bench(self, nrep=10):
+        """Average device time (ms) of each forward layer."""
+        ms = np.zeros(_lib.NET_LAYERS)
+        self.ctx.check(self._L().ii_net_bench(self.ctx.handle, int(nrep), _lib.dptr(ms)),
+                       "ii_net_bench")
+        return ms
+
     def save(self, path):
         self.ctx.check(self._L().ii_net_save(self.ctx.handle, str(path).encode()), "ii_net_save")
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "ii_qprop_multishift",
     "ii_net_init", "ii_net_set_weights", "ii_net_get_weights", "ii_net_layer_apply",
     "ii_net_layer_wgrad", "ii_net_loss_grad", "ii_net_apply", "ii_net_train", "ii_net_save",
-    "ii_net_load", "ii_fgcr_solve",
+    "ii_net_load", "ii_net_bench", "ii_fgcr_solve",
     "ii_autotune", "ii_bench_hop", "ii_bench_copy",
 ]
 
@@ -141,6 +141,7 @@ def load():
         "ii_net_train": [vp, cd, ctypes.POINTER(TrainParamsC), ctypes.POINTER(TrainReportC)],
         "ii_net_save": [vp, ctypes.c_char_p],
         "ii_net_load": [vp, ctypes.c_char_p],
+        "ii_net_bench": [vp, ci, dp],
         "ii_fgcr_solve": [vp, cd, dp, dp, cd, ci, ci, ci, so],
         "ii_autotune": [vp, ctypes.c_char_p, ctypes.c_size_t],
         "ii_bench_hop": [vp, ci, ci, ci, dp],

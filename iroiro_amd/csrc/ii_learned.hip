@@ -172,16 +172,19 @@ struct LayerArgs {
   int tiles_per_block;
 };
 
-// Transported 4-spinor of channel field `src` (one parity block of a full
-// channel field) at half site nb: w = M v with M = 1, U or U^dag.
-__device__ __forceinline__ void net_transport(const float2* __restrict__ src, int stride, int nb,
-                                              bool has_link, bool dag, const float2 U[3][3],
-                                              float2 w[4][3]) {
+// Transported 4-spinor of a channel field at one site: w = M v with M = 1,
+// U or U^dag.  The buffer base is the whole activation array (uniform), the
+// scalar offset `soff` must be wave-uniform (component blocks), and the
+// per-lane element offset `loff` carries the channel and the site, so no
+// load needs a per-lane descriptor or scalar offset.
+__device__ __forceinline__ void net_transport(const float2* __restrict__ base, uint32_t soff,
+                                              int stride, uint32_t loff, bool has_link, bool dag,
+                                              const float2 U[3][3], float2 w[4][3]) {
   float2 v[4][3];
 #pragma unroll
   for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[t][c] = ld_at(src, (uint32_t)((3 * t + c) * stride), (uint32_t)nb);
+    for (int c = 0; c < 3; ++c) v[t][c] = ld_at(base, soff + (uint32_t)((3 * t + c) * stride), loff);
   if (!has_link) {
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -201,6 +204,16 @@ __device__ __forceinline__ void net_transport(const float2* __restrict__ src, in
       }
       w[t][a] = acc;
     }
+}
+
+// r12 FP32 link with every offset in the lane offset (for kernels whose
+// threads may hold sites of different parities)
+__device__ __forceinline__ void ld_link_lane(const float2* __restrict__ gp, int stride, int par,
+                                             int mu, int site, bool flip, float2 U[3][3]) {
+  const uint32_t loff = (uint32_t)((par * 4 + mu) * 6 * stride + site);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) U[k / 3][k % 3] = ld_at(gp, (uint32_t)(k * stride), loff);
+  r12_third_row(U, flip);
 }
 
 // Path d of half site s (parity par): neighbour half index (in parity
@@ -239,7 +252,8 @@ __global__ void __launch_bounds__(kLayerThreads) k_net_layer(LayerArgs a) {
   const int t_begin = blockIdx.x * a.tiles_per_block;
   const int t_end = min(t_begin + a.tiles_per_block, ntiles);
   for (int tile = t_begin + wave; tile < t_end; tile += kLayerThreads / 64) {
-    const int par = a.par0 + tile / tpp;
+    // wave-uniform: keeps the component offsets of every access scalar
+    const int par = __builtin_amdgcn_readfirstlane(a.par0 + tile / tpp);
     const int site = (tile % tpp) * 16 + n;
     const bool valid = site < g.Vh;
     const int s = valid ? site : g.Vh - 1;
@@ -256,7 +270,7 @@ __global__ void __launch_bounds__(kLayerThreads) k_net_layer(LayerArgs a) {
       float2 U[3][3];
       bool dag;
       const int nb = net_path(g, gauge, a.flip, d, par, s, sc, U, dag);
-      const int npar = d == 0 ? par : 1 - par;
+      const uint32_t soff = (uint32_t)((d == 0 ? par : 1 - par) * 12 * g.stride);
       bf16x8 B[3][KB];
 #pragma unroll
       for (int kb = 0; kb < KB; ++kb) {
@@ -268,8 +282,7 @@ __global__ void __launch_bounds__(kLayerThreads) k_net_layer(LayerArgs a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) w[t][c] = make_float2(0.f, 0.f);
         } else {
-          net_transport(a.in + ch * cs + (size_t)npar * 12 * g.stride, g.stride, nb, d != 0, dag,
-                        U, w);
+          net_transport(a.in, soff, g.stride, (uint32_t)(ch * cs + nb), d != 0, dag, U, w);
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -300,15 +313,15 @@ __global__ void __launch_bounds__(kLayerThreads) k_net_layer(LayerArgs a) {
       for (int mb = 0; mb < MB; ++mb) {
         const int o = 2 * mb + (q >> 1);
         if (o < COUT) {
-          float2* dst = a.out + o * cs + (size_t)par * 12 * g.stride;
+          // spin 2(q&1)+sp varies across lane groups: its offset goes with the lane
+          const uint32_t loff = (uint32_t)(o * cs + 6 * (q & 1) * g.stride + site);
+          const uint32_t pbase = (uint32_t)(par * 12 * g.stride);
 #pragma unroll
           for (int sp = 0; sp < 2; ++sp)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              const int comp = 3 * (2 * (q & 1) + sp) + c;
-              st_at(dst, (uint32_t)(comp * g.stride), (uint32_t)site,
+            for (int c = 0; c < 3; ++c)
+              st_at(a.out, pbase + (uint32_t)((3 * sp + c) * g.stride), loff,
                     make_float2(acc[c][mb][2 * sp], acc[c][mb][2 * sp + 1]));
-            }
         }
       }
     }
@@ -352,8 +365,7 @@ k_net_wgrad(Geom g, const float2* __restrict__ in, const float2* __restrict__ go
       float2 v = make_float2(0.f, 0.f);
       if (q < q1) {
         const int par = q >= g.Vh, hs = q - par * g.Vh;
-        v = ld_at(gout + o * cs + (size_t)par * 12 * g.stride, (uint32_t)(comp * g.stride),
-                  (uint32_t)hs);
+        v = ld_at(gout, 0u, (uint32_t)(o * cs + (par * 12 + comp) * g.stride + hs));
       }
       sg[x][comp % 3][4 * o + comp / 3] = v;
     }
@@ -368,11 +380,22 @@ k_net_wgrad(Geom g, const float2* __restrict__ in, const float2* __restrict__ go
           const int par = q >= g.Vh, hs = q - par * g.Vh;
           const SiteCoord sc = decode_site(g, hs, par);
           float2 U[3][3];
-          bool dag;
-          const int nb = net_path(g, gauge, flip, d, par, hs, sc, U, dag);
+          bool dag = false;
+          int nb = hs;
+          if (d > 0) {
+            const int mu = (d - 1) >> 1;
+            const bool fw = ((d - 1) & 1) == 0;
+            nb = neighbour(g, sc, hs, mu, fw);
+            if (fw) {
+              ld_link_lane(gaugep, g.stride, par, mu, hs, flip && mu == 3 && sc.t == g.L[3] - 1, U);
+            } else {
+              ld_link_lane(gaugep, g.stride, 1 - par, mu, nb, flip && mu == 3 && sc.t == 0, U);
+              dag = true;
+            }
+          }
           const int npar = d == 0 ? par : 1 - par;
-          net_transport(in + i * cs + (size_t)npar * 12 * g.stride, g.stride, nb, d != 0, dag, U,
-                        w);
+          net_transport(in, 0u, g.stride, (uint32_t)(i * cs + npar * 12 * g.stride + nb), d != 0,
+                        dag, U, w);
         } else {
 #pragma unroll
           for (int t = 0; t < 4; ++t)
@@ -1415,6 +1438,31 @@ ii_status ii_net_load(ii_context* ctx, const char* path) {
   II_CHECK(build_frags(ctx, N));
   II_HIP(ctx, hipStreamSynchronize(ctx->stream));
   N->init = true;
+  return II_OK;
+}
+
+ii_status ii_net_bench(ii_context* ctx, int nrep, double* layer_ms) {
+  if (!ctx || !layer_ms || nrep < 1) return II_ERR_BAD_ARG;
+  II_CHECK(enter(ctx));
+  Net* N = nullptr;
+  II_CHECK(net_get(ctx, &N));
+  if (!N->init) return set_error(ctx, II_ERR_STATE, "network not initialised");
+  const Geom& g = ctx->g;
+  const size_t ce = chan_elems(g);
+  for (int l = 0; l <= kNL; ++l)
+    II_HIP(ctx, hipMemsetAsync(N->act[l], 0, ce * kWidth[l] * sizeof(float2), ctx->stream));
+  for (int l = 0; l < kNL; ++l) {
+    const int npar = l == kNL - 1 ? 1 : 2;
+    II_CHECK(run_layer(ctx, N, l, false, N->act[l], N->act[l + 1], 0, npar));
+    II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int r = 0; r < nrep; ++r)
+      II_CHECK(run_layer(ctx, N, l, false, N->act[l], N->act[l + 1], 0, npar));
+    II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    II_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    II_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    layer_ms[l] = ms / nrep;
+  }
   return II_OK;
 }
 
