@@ -43,38 +43,44 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p) {
 // copyable
 using U32x2 = uint32_t __attribute__((ext_vector_type(2)));
 using U32x4 = uint32_t __attribute__((ext_vector_type(4)));
-template <typename T>
+// Cache policy of a buffer access (instruction bits of gfx950): 0 = default,
+// kNT = non-temporal.  Links (each read once per hop) and stencil outputs
+// (written once, read by the next kernel) are streamed with kNT so they do
+// not evict the neighbour spinors the stencil re-reads from L2; measured on
+// the FP64 hop at 32^3x64: 0.362 ms -> 0.289 ms.
+constexpr int kNT = 2;
+template <typename T, int CP = 0>
 __device__ __forceinline__ T ld_at(const T* __restrict__ base, uint32_t off, uint32_t idx) {
   const uint32_t vo = idx * (uint32_t)sizeof(T), so = off * (uint32_t)sizeof(T);
   T r;
   if constexpr (sizeof(T) == 4) {
-    uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rsrc(base), vo, so, 0);
+    uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rsrc(base), vo, so, CP);
     __builtin_memcpy((void*)&r, &v, 4);
   } else if constexpr (sizeof(T) == 8) {
-    U32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), vo, so, 0);
+    U32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), vo, so, CP);
     __builtin_memcpy((void*)&r, &v, 8);
   } else {
     static_assert(sizeof(T) == 16, "4, 8 or 16-byte elements");
-    U32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), vo, so, 0);
+    U32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), vo, so, CP);
     __builtin_memcpy((void*)&r, &v, 16);
   }
   return r;
 }
-template <typename T>
+template <typename T, int CP = 0>
 __device__ __forceinline__ void st_at(T* __restrict__ base, uint32_t off, uint32_t idx, T v) {
   const uint32_t vo = idx * (uint32_t)sizeof(T), so = off * (uint32_t)sizeof(T);
   if constexpr (sizeof(T) == 4) {
     uint32_t w;
     __builtin_memcpy(&w, &v, 4);
-    __builtin_amdgcn_raw_buffer_store_b32(w, rsrc(base), vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(w, rsrc(base), vo, so, CP);
   } else if constexpr (sizeof(T) == 8) {
     U32x2 w;
     __builtin_memcpy(&w, &v, 8);
-    __builtin_amdgcn_raw_buffer_store_b64(w, rsrc(base), vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(w, rsrc(base), vo, so, CP);
   } else {
     U32x4 w;
     __builtin_memcpy(&w, &v, 16);
-    __builtin_amdgcn_raw_buffer_store_b128(w, rsrc(base), vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(w, rsrc(base), vo, so, CP);
   }
 }
 
@@ -99,7 +105,7 @@ struct StF64 {
   int stride;
   __device__ __forceinline__ void st(int site, const C v[12]) const {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) st_at(p, (uint32_t)(k * stride), (uint32_t)site,v[k]);
+    for (int k = 0; k < 12; ++k) st_at<double2, kNT>(p, (uint32_t)(k * stride), (uint32_t)site, v[k]);
   }
   __device__ __forceinline__ StF64 rhs(int r) const { return StF64{p + (size_t)r * 12 * stride, stride}; }
 };
@@ -124,7 +130,7 @@ struct StF32 {
   int stride;
   __device__ __forceinline__ void st(int site, const C v[12]) const {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) st_at(p, (uint32_t)(k * stride), (uint32_t)site,v[k]);
+    for (int k = 0; k < 12; ++k) st_at<float2, kNT>(p, (uint32_t)(k * stride), (uint32_t)site, v[k]);
   }
   __device__ __forceinline__ StF32 rhs(int r) const { return StF32{p + (size_t)r * 12 * stride, stride}; }
 };
@@ -188,10 +194,11 @@ struct StLo {
     // any field the solver stores.
     if (!(m > 1e-30f)) m = 0.0f;
     const float inv = m > 0.0f ? 1.0f / m : 0.0f;
-    st_at(nrm, 0u, (uint32_t)site, m);
+    st_at<float, kNT>(nrm, 0u, (uint32_t)site, m);
 #pragma unroll
     for (int k = 0; k < 12; ++k)
-      st_at(p, (uint32_t)(k * stride), (uint32_t)site,f2_to_lo<T>(make_float2(v[k].x * inv, v[k].y * inv)));
+      st_at<T, kNT>(p, (uint32_t)(k * stride), (uint32_t)site,
+                    f2_to_lo<T>(make_float2(v[k].x * inv, v[k].y * inv)));
   }
   __device__ __forceinline__ StLo rhs(int r) const {
     return StLo{p + (size_t)r * 12 * stride, nrm + (size_t)r * stride, stride};
@@ -224,7 +231,8 @@ struct GaugeF64 {
   __device__ __forceinline__ void ld(int par, int mu, int site, bool /*tflip*/, C U[3][3]) const {
 #pragma unroll
     for (int k = 0; k < 9; ++k)
-      U[k / 3][k % 3] = ld_at(p, (uint32_t)(((par * 4 + mu) * 9 + k) * stride), (uint32_t)site);
+      U[k / 3][k % 3] =
+          ld_at<double2, kNT>(p, (uint32_t)(((par * 4 + mu) * 9 + k) * stride), (uint32_t)site);
   }
 };
 
@@ -236,7 +244,8 @@ struct GaugeF32r12 {
   __device__ __forceinline__ void ld(int par, int mu, int site, bool tflip, C U[3][3]) const {
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-      U[k / 3][k % 3] = ld_at(p, (uint32_t)(((par * 4 + mu) * 6 + k) * stride), (uint32_t)site);
+      U[k / 3][k % 3] =
+          ld_at<float2, kNT>(p, (uint32_t)(((par * 4 + mu) * 6 + k) * stride), (uint32_t)site);
     r12_third_row(U, tflip);
   }
 };
@@ -251,7 +260,8 @@ struct GaugeLoR12 {
 #pragma unroll
     for (int k = 0; k < 6; ++k)
       U[k / 3][k % 3] =
-          lo_to_f2<TT>(ld_at(p, (uint32_t)(((par * 4 + mu) * 6 + k) * stride), (uint32_t)site));
+          lo_to_f2<TT>(ld_at<TT, kNT>(p, (uint32_t)(((par * 4 + mu) * 6 + k) * stride),
+                                      (uint32_t)site));
     r12_third_row(U, tflip);
   }
 };
