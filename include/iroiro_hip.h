@@ -179,6 +179,12 @@ ii_status ii_field_download(const ii_field* f, int rhs, double* host);
  * Qprop_EvenOdd::calc, qprop_EvenOdd.cpp:11-23). */
 ii_status ii_field_point_sources(ii_field* f, const int site[4]);
 
+/* Complex Gaussian source (unit variance per real component, counter-based
+ * RNG keyed by seed and stream) on the even sites of rhs `rhs`; the rest of
+ * that rhs is zeroed.  The training sources of ii_net_train are these with
+ * stream = pair index. */
+ii_status ii_field_gaussian(ii_field* f, int rhs, unsigned long long seed, int stream);
+
 /* Propagator solve parameters. */
 typedef struct {
   double tol;            /* target |b - D x|/|b| per rhs on the full operator */

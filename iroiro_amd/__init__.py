@@ -496,6 +496,11 @@ class DeviceField:
                        "ii_field_download")
         return a
 
+    def gaussian(self, rhs, seed, stream):
+        """Complex Gaussian source on the even sites of rhs (odd sites zero)."""
+        self.ctx.check(_lib.load().ii_field_gaussian(self.handle, int(rhs), int(seed),
+                                                     int(stream)), "ii_field_gaussian")
+
     def point_sources(self, site):
         d = (ctypes.c_int * 4)(*site)
         self.ctx.check(_lib.load().ii_field_point_sources(self._h, d), "ii_field_point_sources")

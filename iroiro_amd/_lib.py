@@ -27,7 +27,8 @@ EXPORTS = [
     "ii_dirac_mult", "ii_dirac_mult_dag", "ii_dirac_gamma5", "ii_fopr_ddagd_mult",
     "ii_dirac_full_mult", "ii_solver_cg", "ii_multishift_cg", "ii_eo_invert",
     "ii_field_create", "ii_field_destroy", "ii_field_upload", "ii_field_download",
-    "ii_field_point_sources", "ii_propagator_solve", "ii_multishift_cg_full",
+    "ii_field_point_sources", "ii_field_gaussian", "ii_propagator_solve",
+    "ii_multishift_cg_full",
     "ii_multishift_mixed",
     "ii_qprop_multishift",
     "ii_net_init", "ii_net_set_weights", "ii_net_get_weights", "ii_net_layer_apply",
@@ -130,6 +131,7 @@ def load():
         "ii_field_upload": [vp, ci, dp],
         "ii_field_download": [vp, ci, dp],
         "ii_field_point_sources": [vp, ip],
+        "ii_field_gaussian": [vp, ci, ctypes.c_ulonglong, ci],
         "ii_propagator_solve": [vp, cd, ctypes.POINTER(PropParamsC), vp, vp, so],
         "ii_net_init": [vp, cd, cd, ctypes.c_ulonglong],
         "ii_net_set_weights": [vp, ci, fp],

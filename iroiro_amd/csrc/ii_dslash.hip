@@ -12,6 +12,7 @@
 #include "ii_stencil.h"
 
 #include <cstdio>
+#include <type_traits>
 #include <cstdlib>
 #include <vector>
 
@@ -53,11 +54,24 @@ k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
     }
     if (DOT) {
       const SpX dr = dv.rhs(rhs);
-      const double ds = (double)dr.scale(hs);
+      if constexpr (std::is_same<typename SpIn::R, double>::value) {
+        const double ds = (double)dr.scale(hs);
 #pragma unroll
-      for (int k = 0; k < 12; ++k) {
-        auto dvv = dr.ld(hs, k);
-        dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
+        for (int k = 0; k < 12; ++k) {
+          auto dvv = dr.ld(hs, k);
+          dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
+        }
+      } else {
+        // FP32 arithmetic: the 24-term site product in FP32, the sum over
+        // sites in FP64
+        float sd = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          auto dvv = dr.ld(hs, k);
+          sd = fmaf((float)dvv.x, o[k].x, sd);
+          sd = fmaf((float)dvv.y, o[k].y, sd);
+        }
+        dsum += (double)sd * (double)dr.scale(hs);
       }
     }
     out.rhs(rhs).st(hs, o);
@@ -169,11 +183,24 @@ k_hop_mrhs(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
     }
     if (DOT) {
       const SpX dr = dv.rhs(rhs);
-      const double ds = (double)dr.scale(hs);
+      if constexpr (std::is_same<typename SpIn::R, double>::value) {
+        const double ds = (double)dr.scale(hs);
 #pragma unroll
-      for (int k = 0; k < 12; ++k) {
-        auto dvv = dr.ld(hs, k);
-        dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
+        for (int k = 0; k < 12; ++k) {
+          auto dvv = dr.ld(hs, k);
+          dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
+        }
+      } else {
+        // FP32 arithmetic: the 24-term site product in FP32, the sum over
+        // sites in FP64
+        float sd = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          auto dvv = dr.ld(hs, k);
+          sd = fmaf((float)dvv.x, o[k].x, sd);
+          sd = fmaf((float)dvv.y, o[k].y, sd);
+        }
+        dsum += (double)sd * (double)dr.scale(hs);
       }
     }
     out.rhs(rhs).st(hs, o);

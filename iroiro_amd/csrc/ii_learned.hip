@@ -1331,6 +1331,9 @@ ii_status ii_net_train(ii_context* ctx, double kappa, const ii_train_params* p,
         hipLaunchKernelGGL(k_net_gauss, dim3(blocks_for(g.Vh)), dim3(kBlock), 0, ctx->stream, g,
                            (uint64_t)p->seed, k0 + k, src.data + (size_t)k * 2 * he);
       st = propagator(ctx, kappa, &pp, &src, &sol, outs.data());
+      if (st != II_OK)
+        st = set_error(ctx, st, "training pairs " + std::to_string(k0) + ".." +
+                                    std::to_string(k0 + nr - 1) + ": " + ctx->err);
       for (int k = 0; k < nr && st == II_OK; ++k) {
         const double2* b = src.data + (size_t)k * 2 * he;
         const double2* y = sol.data + (size_t)k * 2 * he;
@@ -1383,6 +1386,22 @@ ii_status ii_net_train(ii_context* ctx, double kappa, const ii_train_params* p,
     rep->train_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     rep->steps = steps;
   }
+  return II_OK;
+}
+
+ii_status ii_field_gaussian(ii_field* f, int rhs, unsigned long long seed, int stream) {
+  if (!f || !f->ctx || rhs < 0 || rhs >= f->nrhs) return II_ERR_BAD_ARG;
+  ii_context* ctx = f->ctx;
+  II_CHECK(enter(ctx));
+  const Geom& g = ctx->g;
+  const size_t he = half_elems(g);
+  const int nblk = f->subset == II_FULL ? 2 : 1;
+  double2* base = f->data + (size_t)rhs * nblk * he;
+  II_HIP(ctx, hipMemsetAsync(base, 0, nblk * he * sizeof(double2), ctx->stream));
+  hipLaunchKernelGGL(k_net_gauss, dim3(blocks_for(g.Vh)), dim3(kBlock), 0, ctx->stream, g,
+                     (uint64_t)seed, stream, base);
+  II_HIP(ctx, hipGetLastError());
+  II_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return II_OK;
 }
 

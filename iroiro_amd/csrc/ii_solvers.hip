@@ -39,7 +39,8 @@ struct RUScal {
   int iter;        // reference loop index at convergence
   int updates;     // reliable updates done
   int nan;
-  int pad[2];
+  int keep_p;      // 0 after a reliable update whose true residual broke with the recursion
+  int restarts;    // such search-direction restarts
 };
 constexpr int kScalInts = sizeof(RUScal) / sizeof(int);
 
@@ -77,6 +78,7 @@ __global__ void k_fin_rr(const double* __restrict__ partials, int nb, RUScal* __
     S.rr = s;
     S.beta = s / rrp;
     S.count += 1;
+    S.keep_p = 1;
     if (!finite(s) || !finite(S.beta)) { S.nan = 1; S.done = 1; S.skip_ru = 1; return; }
     if (mode == 0) {
       if (s * S.snorm < S.prec) { S.done = 1; S.iter = S.count - 1; }
@@ -270,6 +272,25 @@ k_norm2_masked(Geom g, const double2* __restrict__ a, const int* __restrict__ sk
   if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = acc;
 }
 
+// masked |a|^2 partials of a field in any precision (FP64 accumulation)
+template <typename Sp>
+__global__ void __launch_bounds__(kBlock)
+k_norm2_P(Geom g, Sp a, const int* __restrict__ skip, int skip_stride,
+          double* __restrict__ partials) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs * skip_stride]) return;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (hs < g.Vh) {
+    double v[24];
+    ld_site(a.rhs(rhs), hs, v);
+#pragma unroll
+    for (int k = 0; k < 24; ++k) acc = fma(v[k], v[k], acc);
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = acc;
+}
+
 // after the true residual: done test
 __global__ void k_fin_rho(const double* __restrict__ partials, int nb, RUScal* __restrict__ sc) {
   RUScal& S = sc[blockIdx.x];
@@ -294,6 +315,14 @@ __global__ void k_fin_ru(const double* __restrict__ partials, int nb, RUScal* __
   for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
   s = block_sum(s);
   if (threadIdx.x == 0) {
+    // A true residual far above the recursive one means the recursion was
+    // corrupted (in the sloppy arithmetic or otherwise); the old search
+    // direction is then useless and keeping it makes CG diverge, so the
+    // caller restarts it from the new residual (p = r).
+    if (s > 16.0 * S.rr) {
+      S.keep_p = 0;
+      S.restarts += 1;
+    }
     S.rr = s;
     S.rmax2 = s;
     // |rho| / |Dhat^dag rho| calibrates when to test the true residual next
@@ -416,6 +445,25 @@ ii_status convertP(ii_context* ctx, int Pa, int Pb, const SF& a, const SF& b, in
   return set_error(ctx, II_ERR_BAD_ARG, "unsupported conversion");
 }
 
+template <int P>
+ii_status norm2_t(ii_context* ctx, const SF& a, int nrhs, const int* skip, int skip_stride,
+                  double* partials) {
+  const Geom& g = ctx->g;
+  hipLaunchKernelGGL((k_norm2_P<typename Acc<P>::Sp>), dim3(blocks_for(g.Vh), nrhs), dim3(kBlock),
+                     0, ctx->stream, g, Acc<P>::sp(a, g.stride), skip, skip_stride, partials);
+  II_HIP(ctx, hipGetLastError());
+  return II_OK;
+}
+ii_status norm2P(ii_context* ctx, int P, const SF& a, int nrhs, const int* skip, int skip_stride,
+                 double* partials) {
+  switch (P) {
+    case 0: return norm2_t<0>(ctx, a, nrhs, skip, skip_stride, partials);
+    case 1: return norm2_t<1>(ctx, a, nrhs, skip, skip_stride, partials);
+    case 2: return norm2_t<2>(ctx, a, nrhs, skip, skip_stride, partials);
+    default: return norm2_t<3>(ctx, a, nrhs, skip, skip_stride, partials);
+  }
+}
+
 ii_status cg_blas_P(ii_context* ctx, int P, const SF& p, const SF& s, const SF& r, const SF& y,
                     int ytype, RUScal* sc, int nrhs, double* partials, int nb) {
   switch (P) {
@@ -485,6 +533,7 @@ __global__ void k_scal_init(RUScal* sc, const double* __restrict__ rr, const dou
   z.delta2 = delta * delta;
   z.iter = -1;
   z.skip_ru = 1;
+  z.keep_p = 1;
   if (!(z.rr > 0.0)) { z.done = 1; }
   if (mode == 0) z.skip_ru = 1;
   S = z;
@@ -1189,6 +1238,8 @@ ii_status shifted_refine(ii_context* ctx, double kappa, const double* shifts, in
       if (lo) II_CHECK(convertP(ctx, 0, P, rho, rs, nrhs, skip_ru, ss));
       hipLaunchKernelGGL(k_fin_ru, dim3(nrhs), dim3(kBlock), 0, ctx->stream, part, nb, sc);
       II_HIP(ctx, hipGetLastError());
+      // restart the search direction where the recursion broke (keep_p == 0)
+      II_CHECK(convertP(ctx, 0, P, rho, ps, nrhs, &sc[0].keep_p, ss));
       chunk = 1;
     } else {
       chunk = std::min(chunk * 2, 8);
@@ -1242,7 +1293,8 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
   const int* skip_it = &sc[0].done;
   const int* skip_ru = &sc[0].skip_ru;
   void* auxp = nullptr;
-  II_CHECK(workspace(ctx, 40, sizeof(double) * 2 * batch + 64, &auxp));
+  // aux: |b_e|^2 [0, batch), |b_o|^2 [batch, 2 batch), |c|^2 [2 batch, 3 batch)
+  II_CHECK(workspace(ctx, 40, sizeof(double) * 3 * batch + 64, &auxp));
   double* aux = (double*)auxp;
   // FP64 fields
   SF be, bo, bh, y, r64, q64, t64, rho;
@@ -1339,7 +1391,13 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       bool any_upd = false;
       all_done = true;
       for (int k = 0; k < nr; ++k) {
-        if (H[k].nan) return set_error(ctx, II_ERR_NAN, "non-finite solver scalar");
+        if (H[k].nan)
+          return set_error(ctx, II_ERR_NAN,
+                           "non-finite solver scalar (rhs " + std::to_string(r0 + k) +
+                               ", iteration " + std::to_string(H[k].count) +
+                               ", rr " + std::to_string(H[k].rr) + ", pap " +
+                               std::to_string(H[k].pap) + ", b2 " + std::to_string(H[k].b2) +
+                               ")");
         if (!H[k].done) all_done = false;
         if (!H[k].done && !H[k].skip_ru) any_upd = true;
       }
@@ -1369,7 +1427,9 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
                            (const double2*)bh.v, (const double2*)q64.v, (double2*)rho.v, skip_ru,
                            ss, part);
         hipLaunchKernelGGL(k_fin_rho, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc);
-        // r64 = Dhat^dag rho ; |r64|^2
+        // new CG residual r = Dhat^dag rho and |r|^2.  The true residual rho
+        // is FP64; r only restarts the sloppy recursion, so with a sloppy
+        // inner precision it is formed in that precision from rho.
         h = HopSpec();
         h.nrhs = nr;
         h.skip = skip_ru;
@@ -1377,23 +1437,33 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
         h.out_par = 1;
         h.a = 1.0;
         h.g5in = true;
-        II_CHECK(hop64(ctx, h, (double2*)rho.v, (double2*)t64.v, nullptr, nullptr));
-        h.out_par = 0;
-        h.a = -kappa * kappa;
-        h.xpay = true;
-        h.g5in = false;
-        h.g5out = true;
         if (lo) {
-          II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)r64.v, (double2*)rho.v, nullptr));
+          II_CHECK(convertP(ctx, 0, P, rho, qs, nr, skip_ru, ss));
+          II_CHECK(hopP(ctx, P, h, qs, ts, nullptr, nullptr));
+          h.out_par = 0;
+          h.a = -kappa * kappa;
+          h.xpay = true;
+          h.g5in = false;
+          h.g5out = true;
+          II_CHECK(hopP(ctx, P, h, ts, rs, &qs, nullptr));
+          II_CHECK(norm2P(ctx, P, rs, nr, skip_ru, ss, part));
         } else {
           // in FP64 the sloppy residual is r64 itself
+          II_CHECK(hop64(ctx, h, (double2*)rho.v, (double2*)t64.v, nullptr, nullptr));
+          h.out_par = 0;
+          h.a = -kappa * kappa;
+          h.xpay = true;
+          h.g5in = false;
+          h.g5out = true;
           II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)r64.v, (double2*)rho.v, nullptr));
+          hipLaunchKernelGGL(k_norm2_masked, dim3(nb, nr), dim3(kBlock), 0, ctx->stream, g,
+                             (const double2*)r64.v, skip_ru, ss, part);
         }
-        hipLaunchKernelGGL(k_norm2_masked, dim3(nb, nr), dim3(kBlock), 0, ctx->stream, g,
-                           (const double2*)r64.v, skip_ru, ss, part);
-        if (lo) II_CHECK(convertP(ctx, 0, P, r64, rs, nr, skip_ru, ss));
         hipLaunchKernelGGL(k_fin_ru, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc);
         II_HIP(ctx, hipGetLastError());
+        // restart the search direction where the recursion broke (keep_p == 0)
+        if (lo) II_CHECK(convertP(ctx, P, P, rs, ps, nr, &sc[0].keep_p, ss));
+        else II_CHECK(convertP(ctx, 0, 0, r64, ps, nr, &sc[0].keep_p, ss));
         chunk = 1;
       } else {
         chunk = std::min(chunk * 2, 8);
