@@ -11,8 +11,13 @@ updates), sources already resident in HBM.
 
 Reported beside it: the plain FP64 GPU CG (one rhs after the other, as the
 reference's Qprop_EvenOdd does) on the same problem, the FP64 even/odd hop
-(Dslash) bandwidth against the HBM roofline, and the reference CPU path
-(oracle/_ref, the compiled IroIro++) timed on this host.
+(Dslash) bandwidth against the HBM roofline with its measured HBM traffic, the
+learned gauge-covariant preconditioner (trained on this gauge field from 64
+plain-CG pairs, then the same 12 columns through its flexible GCR; training
+time reported apart), and the reference CPU path (oracle/_ref, the compiled
+IroIro++) timed on this host.  `value` is the time of the fastest full-accuracy
+method, the batched mixed-precision CG; the learned method's time is in
+"learned".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -41,6 +46,25 @@ CONFIGS = {
     "tiny": dict(dims=(8, 8, 8, 16), beta=6.0, kappa=0.1355, sweeps=20,
                  workload="wilson_eo_propagator_8c16_b6.0_k0.1355_12rhs"),
 }
+
+
+def measured_traffic(dims):
+    """HBM bytes per launch of the FP64 hop from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/wilson_hop_32c64.md (scaled by
+    the copy-kernel calibration stated there); None for other lattices."""
+    if tuple(dims) != (32, 32, 32, 64):
+        return None, None
+    path = os.path.join(ROOT, "profiles", "wilson_hop_32c64.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None, None
+    m = re.search(r"pmc k_hop_fp64_32c64: fetch_kib=(\d+) write_kib=(\d+) "
+                  r"fetch_scale=([0-9.]+) write_scale=([0-9.]+)", text)
+    if not m:
+        return None, None
+    fk, wk, fs, ws = int(m.group(1)), int(m.group(2)), float(m.group(3)), float(m.group(4))
+    return (fk * fs + wk * ws) * 1024.0, "profiles/wilson_hop_32c64.md"
 
 
 def log(*a):
@@ -110,6 +134,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20260601)
     ap.add_argument("--hop-only", action="store_true",
                     help="only time the hop kernels and the copy reference")
+    ap.add_argument("--net-only", action="store_true",
+                    help="only time the layers of the learned network")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training and timing the learned preconditioner")
     ap.add_argument("--train-epochs", type=int, default=8)
@@ -156,6 +182,29 @@ def main():
         ctx.close()
         return
 
+    if args.net_only:
+        # kernel-level measurement of the learned network's layers (under
+        # rocprofv3): FP32 activations, bf16 matrix-core channel mixing
+        net = ii.LearnedPreconditioner(ctx)
+        net.init_neumann(kappa, noise=0.05, seed=args.seed)
+        ms = net.bench(10)
+        rows = {}
+        for l, t in enumerate(ms):
+            cin, cout = net.WIDTHS[l], net.WIDTHS[l + 1]
+            sites = ctx.V if l < 3 else ctx.Vh
+            # bytes: input channels + output channels (FP32 spinors) + 4 r12 FP32 links per site
+            nbytes = (96 * cin * (ctx.V / sites) + 96 * cout + 192) * sites
+            flops = 9 * (8 * cout) * (8 * cin) * 3 * 2 * sites
+            rows["layer%d_%dto%d" % (l, cin, cout)] = {
+                "ms": t, "alg_gbs": nbytes / (t * 1e-3) / 1e9,
+                "mfma_tflops": flops / (t * 1e-3) / 1e12}
+            log("net layer %d (%d->%d): %.3f ms, %.0f GB/s, %.1f TFLOP/s" %
+                (l, cin, cout, t, rows["layer%d_%dto%d" % (l, cin, cout)]["alg_gbs"],
+                 rows["layer%d_%dto%d" % (l, cin, cout)]["mfma_tflops"]))
+        print(json.dumps({"net_only": rows, "lattice": list(dims)}), flush=True)
+        ctx.close()
+        return
+
     src = ii.DeviceField(ctx, "full", 12)
     sol = ii.DeviceField(ctx, "full", 12)
     src.point_sources((0, 0, 0, 0))
@@ -173,6 +222,8 @@ def main():
     log("FP64 hop %.3f ms = %.0f GB/s (%.1f%% of HBM peak); %s 12-rhs hop %.3f ms; "
         "plain copy %.0f GB/s" % (hop_ms, hop_gbs, 100 * hop_gbs / HBM_PEAK_GBS, args.precision,
                                   hop_lp_ms, copy_gbs))
+
+    traffic, traffic_src = measured_traffic(dims)
 
     # ---- plain FP64 GPU CG, rhs after rhs (reference Qprop_EvenOdd shape)
     fp64 = None
@@ -257,7 +308,9 @@ def main():
         "fp64_cg": fp64,
         "speedup_vs_fp64_cg": (fp64["s"] / (ms_per_step / 1000.0)) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": hop_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hop_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": hop_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes": FP64_HOP_BYTES_PER_SITE * Vh,
                      "kernel": "k_hop FP64 even/odd Wilson hop, V/2 = %d sites/launch, "
                                "1536 B/site algorithmic" % Vh,
                      "avg_ms": hop_ms,
