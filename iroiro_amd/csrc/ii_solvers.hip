@@ -1376,14 +1376,38 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
     int its = 0;
     bool all_done = false;
     int chunk = 1;
+    // one CG iteration: 4 hops (the last with <p, A p>), scalars, the fused
+    // x/r update with |r|^2, scalars, the p update.  Every argument is fixed
+    // for the batch and every kernel reads its rhs's done flag on the device,
+    // so the sequence can be recorded once into a HIP graph and replayed.
+    auto iteration = [&]() -> ii_status {
+      II_CHECK(ddagdP(ctx, P, kappa, ps, ss_, qs, ts, &ps, part, nr, skip_it, ss));
+      hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
+                         hop_dot_parts(g, nr, P), sc);
+      II_CHECK(cg_blas_P(ctx, P, ps, ss_, rs, ys, ytype, sc, nr, part, nb));
+      hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc, 1);
+      return cg_p_P(ctx, P, rs, ps, sc, nr);
+    };
+    hipGraphExec_t gexec = nullptr;
+    struct GraphGuard {
+      hipGraphExec_t* e;
+      ~GraphGuard() { if (*e) (void)hipGraphExecDestroy(*e); }
+    } gguard{&gexec};
+    if (prm->use_graph) {
+      hipGraph_t graph = nullptr;
+      II_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      ii_status cst = iteration();
+      hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
+      II_CHECK(cst);
+      if (ee != hipSuccess) return set_error(ctx, II_ERR_DEVICE, "graph capture failed");
+      hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) return set_error(ctx, II_ERR_DEVICE, "graph instantiation failed");
+    }
     while (its < maxiter && !all_done) {
       for (int i = 0; i < chunk; ++i) {
-        II_CHECK(ddagdP(ctx, P, kappa, ps, ss_, qs, ts, &ps, part, nr, skip_it, ss));
-        hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
-                           hop_dot_parts(g, nr, P), sc);
-        II_CHECK(cg_blas_P(ctx, P, ps, ss_, rs, ys, ytype, sc, nr, part, nb));
-        hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc, 1);
-        II_CHECK(cg_p_P(ctx, P, rs, ps, sc, nr));
+        if (gexec) II_HIP(ctx, hipGraphLaunch(gexec, ctx->stream));
+        else II_CHECK(iteration());
       }
       its += chunk;
       II_CHECK(read_scalars(ctx, nr));
