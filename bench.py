@@ -35,6 +35,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
 FP64_HOP_BYTES_PER_SITE = 1536  # 192 in + 8*144 links + 192 out
 HOP_FLOPS_PER_SITE = 1320
+# MD force, both parities: 4 links + 4 force matrices (144 B each) + 4 spinors (192 B)
+MD_FORCE_BYTES_PER_SITE = 1920
 
 CONFIGS = {
     "c5": dict(dims=(32, 32, 32, 64), beta=6.0, kappa=0.1355, sweeps=200,
@@ -136,6 +138,8 @@ def main():
                     help="only time the hop kernels and the copy reference")
     ap.add_argument("--net-only", action="store_true",
                     help="only time the layers of the learned network")
+    ap.add_argument("--force-only", action="store_true",
+                    help="only time the MD force (Dirac_Wilson_EvenOdd::md_force)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training and timing the learned preconditioner")
     ap.add_argument("--train-epochs", type=int, default=8)
@@ -205,6 +209,17 @@ def main():
         ctx.close()
         return
 
+    if args.force_only:
+        # kernel-level measurement of the MD force (under rocprofv3)
+        f_ms, t_ms = ctx.bench_md_force(20)
+        gbs = MD_FORCE_BYTES_PER_SITE * ctx.V / (f_ms * 1e-3) / 1e9
+        log("md_force kernel %.3f ms = %.0f GB/s (%.1f%% of HBM peak); with its two hops %.3f ms"
+            % (f_ms, gbs, 100 * gbs / HBM_PEAK_GBS, t_ms))
+        print(json.dumps({"force_only": {"kernel_ms": f_ms, "alg_gbs": gbs, "total_ms": t_ms},
+                          "lattice": list(dims)}), flush=True)
+        ctx.close()
+        return
+
     src = ii.DeviceField(ctx, "full", 12)
     sol = ii.DeviceField(ctx, "full", 12)
     src.point_sources((0, 0, 0, 0))
@@ -224,6 +239,10 @@ def main():
                                   hop_lp_ms, copy_gbs))
 
     traffic, traffic_src = measured_traffic(dims)
+    force_ms, force_total_ms = ctx.bench_md_force(20)
+    force_gbs = MD_FORCE_BYTES_PER_SITE * ctx.V / (force_ms * 1e-3) / 1e9
+    log("MD force kernel %.3f ms = %.0f GB/s (%.1f%% of HBM peak)"
+        % (force_ms, force_gbs, 100 * force_gbs / HBM_PEAK_GBS))
 
     # ---- plain FP64 GPU CG, rhs after rhs (reference Qprop_EvenOdd shape)
     fp64 = None
@@ -320,6 +339,10 @@ def main():
                                      "avg_ms": hop_lp_ms,
                                      "achieved": lp_bytes / (hop_lp_ms * 1e-3) / 1e9,
                                      "unit": "GB/s", "bytes_per_site_per_rhs": 104 + 16},
+        "md_force": {"kernel": "k_md_force, Dirac_Wilson_EvenOdd::md_force over V = %d sites, "
+                               "1920 B/site algorithmic" % ctx.V,
+                     "avg_ms": force_ms, "achieved": force_gbs, "unit": "GB/s",
+                     "frac": force_gbs / HBM_PEAK_GBS, "with_hops_ms": force_total_ms},
         "cpu_baseline": cpu,
         "learned": learned,
         "autotune": tune,

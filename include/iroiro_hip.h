@@ -126,6 +126,36 @@ ii_status ii_fopr_ddagd_mult(ii_context* ctx, double kappa, double* out, const d
 ii_status ii_dirac_full_mult(ii_context* ctx, double kappa, double* out, const double* in,
                              int dag);
 
+/* ------------------------------------------------- MD force on host arrays */
+/* Dirac_Wilson_EvenOdd::md_force (dirac_wilson_EvenOdd.hpp:94-116):
+ * eta, zeta even half fields; fce (out) a gauge-layout field of
+ * 18*4*V doubles, kappa * [Deo.md_force_p(Doe eta, zeta) +
+ * Doe.md_force_m(Doe eta, zeta) + Doe.md_force_p(eta, Deo^dag zeta) +
+ * Deo.md_force_m(eta, Deo^dag zeta)] with mkfrc of dirac_wilson.cpp:62-92. */
+ii_status ii_dirac_md_force(ii_context* ctx, double kappa, double* fce, const double* eta,
+                            const double* zeta);
+/* Full-index Dirac_Wilson::md_force (dirac_wilson.cpp:52-60):
+ * fce = -kappa (md_force_p + md_force_m) from full fields eta, zeta. */
+ii_status ii_dirac_full_md_force(ii_context* ctx, double kappa, double* fce, const double* eta,
+                                 const double* zeta);
+/* Dirac_Wilson::md_force_p (minus = 0) / md_force_m (minus = 1) of the
+ * full-index operator (dirac_wilson.cpp:41-50): ADDS the unscaled mkfrc sum
+ * to fce (in/out, gauge layout), as the reference's Field::add does. */
+ii_status ii_dirac_full_md_force_pm(ii_context* ctx, double* fce, const double* eta,
+                                    const double* zeta, int minus);
+/* FieldUtils::TracelessAntihermite (fieldUtils.cpp:264-274) on a gauge-layout
+ * field: (m - m^dag)/2 - i Im tr(m)/3 per link.  out may equal in. */
+ii_status ii_traceless_antihermite(ii_context* ctx, double* out, const double* in);
+/* Action_Nf2::md_force (action_Nf2.cpp:69-86) without smearing, on the
+ * uploaded gauge field: eta = (Dhat^dag Dhat)^-1 phi by Solver_CG
+ * semantics (as ii_solver_cg: prec, maxiter, out), zeta = Dhat eta,
+ * force = TracelessAntihermite(md_force(eta, zeta)), all on the device;
+ * only phi (even half field) goes up and the force (gauge layout) comes
+ * down.  Returns II_ERR_NOT_CONVERGED (force still written) when the solve
+ * reaches maxiter. */
+ii_status ii_action_nf2_md_force(ii_context* ctx, double kappa, const double* phi, double prec,
+                                 int maxiter, double* force, ii_solver_output* out);
+
 /* ------------------------------------------------- solvers on host arrays */
 /* Solver_CG::solve on Fopr_DdagD(Dirac_Wilson_EvenOdd) (solver_CG.cpp:29-77):
  * x0 = b; stops when |r|^2/|b|^2 < prec (squared); iterations = loop index
@@ -297,6 +327,11 @@ ii_status ii_bench_hop(ii_context* ctx, int precision, int nrhs, int nrep, doubl
  * single-rhs hop on this lattice (run automatically before the first FP64
  * single-rhs hop) and reports the timings and the choice as text. */
 ii_status ii_autotune(ii_context* ctx, char* report, size_t len);
+/* Times Dirac_Wilson_EvenOdd::md_force on device-resident fields (no host
+ * copies): total_ms = the two hops forming Doe eta and Deo^dag zeta plus the
+ * force kernel; force_ms = the force kernel alone (both parities).  HIP
+ * events on the context's stream, averaged over nrep launches. */
+ii_status ii_bench_md_force(ii_context* ctx, int nrep, double* force_ms, double* total_ms);
 /* Times a plain device copy of `bytes` bytes (read + write) with HIP
  * events: the achievable-bandwidth reference for the roofline and the
  * calibration kernel of the FETCH_SIZE / WRITE_SIZE counters. */

@@ -157,6 +157,13 @@ class Context:
                    "ii_bench_copy")
         return ms.value
 
+    def bench_md_force(self, nrep=20):
+        """(force kernel ms, md_force ms incl. its two hops) on device fields."""
+        f, t = ctypes.c_double(), ctypes.c_double()
+        self.check(_lib.load().ii_bench_md_force(self._h, nrep, ctypes.byref(f),
+                                                 ctypes.byref(t)), "ii_bench_md_force")
+        return f.value, t.value
+
     def bench_hop(self, precision="double", nrhs=1, nrep=50):
         ms = ctypes.c_double()
         self.check(_lib.load().ii_bench_hop(self._h, PRECISIONS[precision], nrhs, nrep,
@@ -234,6 +241,17 @@ class Dirac_Wilson_EvenOdd(_OpBase):
 
     mult_oo = mult_ee_inv = mult_oo_inv = mult_ee
 
+    def md_force(self, eta, zeta):
+        """Dirac_Wilson_EvenOdd::md_force (:110-116) from even half fields:
+        a gauge-layout force field (18*4*V doubles)."""
+        eta = _lib.as_f64(eta, self.fsize())
+        zeta = _lib.as_f64(zeta, self.fsize())
+        out = np.empty(self.gsize())
+        self.ctx.check(_lib.load().ii_dirac_md_force(self.ctx.handle, self.kpp, _lib.dptr(out),
+                                                     _lib.dptr(eta), _lib.dptr(zeta)),
+                       "ii_dirac_md_force")
+        return out
+
     def gamma5(self, f):
         f = _lib.as_f64(f)
         out = np.empty_like(f)
@@ -253,6 +271,35 @@ class Dirac_Wilson(_OpBase):
 
     def mult_dag(self, f):
         return self._call("ii_dirac_full_mult", _lib.as_f64(f, self.fsize()), self.fsize(), 1)
+
+    def md_force(self, eta, zeta):
+        """Dirac_Wilson::md_force (dirac_wilson.cpp:52-60): -kappa (p + m)."""
+        eta = _lib.as_f64(eta, self.fsize())
+        zeta = _lib.as_f64(zeta, self.fsize())
+        out = np.empty(self.gsize())
+        self.ctx.check(_lib.load().ii_dirac_full_md_force(self.ctx.handle, self.kpp,
+                                                          _lib.dptr(out), _lib.dptr(eta),
+                                                          _lib.dptr(zeta)),
+                       "ii_dirac_full_md_force")
+        return out
+
+    def _md_force_pm(self, fce, eta, zeta, minus):
+        if fce.dtype != np.float64 or not fce.flags.c_contiguous or fce.size != self.gsize():
+            raise ValueError("fce must be a contiguous float64 array of gsize() doubles")
+        eta = _lib.as_f64(eta, self.fsize())
+        zeta = _lib.as_f64(zeta, self.fsize())
+        self.ctx.check(_lib.load().ii_dirac_full_md_force_pm(self.ctx.handle, _lib.dptr(fce),
+                                                             _lib.dptr(eta), _lib.dptr(zeta),
+                                                             minus),
+                       "ii_dirac_full_md_force_pm")
+
+    def md_force_p(self, fce, eta, zeta):
+        """Dirac_Wilson::md_force_p (dirac_wilson.cpp:41-43): adds into fce."""
+        self._md_force_pm(fce, eta, zeta, 0)
+
+    def md_force_m(self, fce, eta, zeta):
+        """Dirac_Wilson::md_force_m (dirac_wilson.cpp:45-50): adds into fce."""
+        self._md_force_pm(fce, eta, zeta, 1)
 
     def gamma5(self, f):
         f = _lib.as_f64(f)
@@ -311,6 +358,60 @@ class Solver_CG:
         so = SolverOutput(out)
         so.Msg = "CG solver"
         return so
+
+
+def TracelessAntihermite(ctx, G):
+    """FieldUtils::TracelessAntihermite (lib/Tools/fieldUtils.cpp:264-274)
+    on a gauge-layout field: (m - m^dag)/2 - i Im tr(m)/3 per link."""
+    G = _lib.as_f64(G, 18 * 4 * ctx.V)
+    out = np.empty_like(G)
+    ctx.check(_lib.load().ii_traceless_antihermite(ctx.handle, _lib.dptr(out), _lib.dptr(G)),
+              "ii_traceless_antihermite")
+    return out
+
+
+class Action_Nf2:
+    """Action_Nf2 (lib/Action/action_Nf2.cpp:14-86, no smearing) with an
+    even/odd Wilson operator and a Solver_CG: the two-flavour pseudofermion
+    action.  md_force runs the solve, Dhat eta, the MD force and the
+    traceless anti-hermitian projection on the device in one call."""
+
+    def __init__(self, D, solver):
+        if not isinstance(D, Dirac_Wilson_EvenOdd) or not isinstance(solver, Solver_CG):
+            raise TypeError("Action_Nf2 on the device takes Dirac_Wilson_EvenOdd and Solver_CG")
+        self.D = D
+        self.slv = solver
+        self.phi = None
+
+    def init(self, seed):
+        """phi = D^dag xi with Gaussian xi (action_Nf2.cpp:41-53); the
+        reference's MPrand generator is replaced by numpy's, seeded."""
+        xi = np.random.default_rng(seed).standard_normal(self.D.fsize())
+        self.phi = self.D.mult_dag(xi)
+
+    def set_phi(self, phi):
+        self.phi = _lib.as_f64(phi, self.D.fsize()).copy()
+
+    def DdagD_inv(self, src):
+        sol = np.empty(self.D.fsize())
+        self.slv.solve(sol, src)
+        return sol
+
+    def calc_H(self):
+        """phi . (D^dag D)^-1 phi (action_Nf2.cpp:55-67)."""
+        return float(np.dot(self.phi, self.DdagD_inv(self.phi)))
+
+    def md_force(self):
+        """Action_Nf2::md_force (action_Nf2.cpp:69-86)."""
+        out = _lib.SolverOutputC()
+        force = np.empty(self.D.gsize())
+        st = _lib.load().ii_action_nf2_md_force(self.D.ctx.handle, self.D.kpp,
+                                                _lib.dptr(self.phi), self.slv.prec,
+                                                self.slv.MaxIter, _lib.dptr(force),
+                                                ctypes.byref(out))
+        self.D.ctx.check(st, "Action_Nf2::md_force")
+        self.last_solve = SolverOutput(out)
+        return force
 
 
 class MultiShiftSolver_CG:

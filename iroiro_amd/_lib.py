@@ -35,6 +35,8 @@ EXPORTS = [
     "ii_net_layer_wgrad", "ii_net_loss_grad", "ii_net_apply", "ii_net_train", "ii_net_save",
     "ii_net_load", "ii_net_bench", "ii_fgcr_solve",
     "ii_autotune", "ii_bench_hop", "ii_bench_copy",
+    "ii_dirac_md_force", "ii_dirac_full_md_force", "ii_dirac_full_md_force_pm",
+    "ii_traceless_antihermite", "ii_action_nf2_md_force", "ii_bench_md_force",
 ]
 
 NET_LAYERS = 4
@@ -120,6 +122,12 @@ def load():
         "ii_dirac_gamma5": [vp, dp, dp, cl],
         "ii_fopr_ddagd_mult": [vp, cd, dp, dp],
         "ii_dirac_full_mult": [vp, cd, dp, dp, ci],
+        "ii_dirac_md_force": [vp, cd, dp, dp, dp],
+        "ii_dirac_full_md_force": [vp, cd, dp, dp, dp],
+        "ii_dirac_full_md_force_pm": [vp, dp, dp, dp, ci],
+        "ii_traceless_antihermite": [vp, dp, dp],
+        "ii_action_nf2_md_force": [vp, cd, dp, cd, ci, dp, so],
+        "ii_bench_md_force": [vp, ci, dp, dp],
         "ii_solver_cg": [vp, cd, dp, dp, cd, ci, so],
         "ii_multishift_cg": [vp, cd, dp, dp, dp, ci, cd, ci, so],
         "ii_eo_invert": [vp, cd, dp, dp, cd, ci, so],

@@ -257,19 +257,25 @@ k_pack_gauge(const double* __restrict__ src, double2* __restrict__ dst, Geom g, 
   }
 }
 
-ii_status pack_gauge(ii_context* ctx, const double* src) {
+ii_status pack_gauge_field(ii_context* ctx, const double* src, double2* dst) {
   const Geom& g = ctx->g;
   hipLaunchKernelGGL(k_pack_gauge, dim3(blocks_for(g.V), 4), dim3(kBlock), 0, ctx->stream, src,
-                     ctx->gauge64, g, 0);
+                     dst, g, 0);
   II_HIP(ctx, hipGetLastError());
   return II_OK;
 }
-ii_status unpack_gauge(ii_context* ctx, double* dst) {
+ii_status unpack_gauge_field(ii_context* ctx, const double2* src, double* dst) {
   const Geom& g = ctx->g;
   hipLaunchKernelGGL(k_pack_gauge, dim3(blocks_for(g.V), 4), dim3(kBlock), 0, ctx->stream, dst,
-                     ctx->gauge64, g, 1);
+                     const_cast<double2*>(src), g, 1);
   II_HIP(ctx, hipGetLastError());
   return II_OK;
+}
+ii_status pack_gauge(ii_context* ctx, const double* src) {
+  return pack_gauge_field(ctx, src, ctx->gauge64);
+}
+ii_status unpack_gauge(ii_context* ctx, double* dst) {
+  return unpack_gauge_field(ctx, ctx->gauge64, dst);
 }
 
 // Reconstruct-12 copies.  stat[0]: links whose third row matches neither

@@ -76,6 +76,10 @@ ii_status unpack_full_spinor(ii_context* ctx, const double2* even, const double2
                              double* host_dev);
 ii_status pack_gauge(ii_context* ctx, const double* host_dev);
 ii_status unpack_gauge(ii_context* ctx, double* host_dev);
+// the same repacking for any gauge-layout field (e.g. a force):
+// host layout [mu][site][3][3] <-> device [parity][mu][9][stride]
+ii_status pack_gauge_field(ii_context* ctx, const double* host_dev, double2* dst);
+ii_status unpack_gauge_field(ii_context* ctx, const double2* src, double* host_dev);
 ii_status build_lowprec_gauge(ii_context* ctx, int prec);
 ii_status plaquette64(ii_context* ctx, double* plaq);
 
@@ -87,6 +91,13 @@ ii_status cg_reference(ii_context* ctx, double kappa, double2* x, const double2*
 // Full-field propagator solve through the even/odd Schur system.
 ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
                      const ii_field* src, ii_field* sol, ii_solver_output* outs);
+
+// MD force (ii_force.hip)
+// Force field F (device gauge layout) of Dirac_Wilson_EvenOdd::md_force
+// from even half fields eta, zeta (dirac_wilson_EvenOdd.hpp:94-116);
+// ta: apply the traceless anti-hermitian projection on the way out.
+ii_status md_force_eo64(ii_context* ctx, double kappa, const double2* eta, const double2* zeta,
+                        double2* F, bool ta);
 
 // learned preconditioner (ii_learned.hip)
 void learned_release(ii_context* ctx);

@@ -26,8 +26,9 @@
 // through their base-class pointers.  Error behaviour follows the
 // reference: CG non-convergence calls Errors::ConvergenceErr (which exits),
 // the multishift solver throws "Not converged.", device failures abort with
-// the library's message.  The MD force, which is not on the device path,
-// is delegated to the reference's own CPU Dirac_Wilson_EvenOdd.
+// the library's message.  md_force (and md_force_p/_m of the full-index
+// operator) run on the device; only getDeo/getDoe, which the domain-wall
+// kernels use, hand out the reference's own CPU sub-operators.
 #ifndef IROIRO_REF_ADAPTER_HPP
 #define IROIRO_REF_ADAPTER_HPP
 
@@ -93,7 +94,7 @@ class Dirac_Wilson_EvenOdd_HIP : public DiracWilsonLike_EvenOdd,
   double kpp_;
   const Field* u_;
   ii_context* ctx_;
-  Dirac_Wilson_EvenOdd<Dirac_Wilson> cpu_;  // MD force only
+  Dirac_Wilson_EvenOdd<Dirac_Wilson> cpu_;  // getDeo / getDoe only
   Dirac_Wilson_EvenOdd_HIP(const Dirac_Wilson_EvenOdd_HIP&);
 
   const Field hop(ii_status (*fn)(ii_context*, double, double*, const double*),
@@ -146,8 +147,14 @@ class Dirac_Wilson_EvenOdd_HIP : public DiracWilsonLike_EvenOdd,
   }
   const DiracWilsonLike* getDeo() const { return cpu_.getDeo(); }
   const DiracWilsonLike* getDoe() const { return cpu_.getDoe(); }
+  // Dirac_Wilson_EvenOdd::md_force (dirac_wilson_EvenOdd.hpp:110-116)
   const Field md_force(const Field& eta, const Field& zeta) const {
-    return cpu_.md_force(eta, zeta);
+    Field fce(gsize());
+    iroiro_hip_detail::check(
+        ctx_, ii_dirac_md_force(ctx_, kpp_, fce.getaddr(0), iroiro_hip_detail::addr(eta),
+                                iroiro_hip_detail::addr(zeta)),
+        "md_force");
+    return fce;
   }
 };
 
@@ -158,8 +165,13 @@ class Dirac_Wilson_HIP : public DiracWilsonLike, public iroiro_hip_detail::Devic
   double kpp_;
   const Field* u_;
   ii_context* ctx_;
-  Dirac_Wilson cpu_;  // MD force only
   Dirac_Wilson_HIP(const Dirac_Wilson_HIP&);
+  void md_force_pm(Field& fce, const Field& eta, const Field& zeta, int minus) const {
+    iroiro_hip_detail::check(
+        ctx_, ii_dirac_full_md_force_pm(ctx_, fce.getaddr(0), iroiro_hip_detail::addr(eta),
+                                        iroiro_hip_detail::addr(zeta), minus),
+        minus ? "md_force_m" : "md_force_p");
+  }
 
   const Field full(const Field& f, int dag, const char* what) const {
     Field w(fsize());
@@ -170,7 +182,7 @@ class Dirac_Wilson_HIP : public DiracWilsonLike, public iroiro_hip_detail::Devic
 
  public:
   Dirac_Wilson_HIP(double mass, const Field* u)
-      : kpp_(0.5 / (4.0 + mass)), u_(u), ctx_(iroiro_hip_detail::context()), cpu_(mass, u) {
+      : kpp_(0.5 / (4.0 + mass)), u_(u), ctx_(iroiro_hip_detail::context()) {
     update_internal_state();
   }
   size_t fsize() const { return (size_t)CommonPrms::Nvol() * 24; }
@@ -193,8 +205,20 @@ class Dirac_Wilson_HIP : public DiracWilsonLike, public iroiro_hip_detail::Devic
                              "gamma5");
     return w;
   }
+  // Dirac_Wilson::md_force_p / md_force_m / md_force (dirac_wilson.cpp:41-60)
+  void md_force_p(Field& fce, const Field& eta, const Field& zeta) const {
+    md_force_pm(fce, eta, zeta, 0);
+  }
+  void md_force_m(Field& fce, const Field& eta, const Field& zeta) const {
+    md_force_pm(fce, eta, zeta, 1);
+  }
   const Field md_force(const Field& eta, const Field& zeta) const {
-    return cpu_.md_force(eta, zeta);
+    Field fce(gsize());
+    iroiro_hip_detail::check(
+        ctx_, ii_dirac_full_md_force(ctx_, kpp_, fce.getaddr(0), iroiro_hip_detail::addr(eta),
+                                     iroiro_hip_detail::addr(zeta)),
+        "md_force");
+    return fce;
   }
 };
 

@@ -41,6 +41,10 @@ def lib():
             "or_dirac_eo": (None, [vp, _dp, _dp, _dp, c_int, c_double]),
             "or_ddagd": (None, [vp, _dp, _dp, _dp, c_double]),
             "or_dirac_full": (None, [vp, _dp, _dp, _dp, c_int, c_double]),
+            "or_md_force_full": (None, [vp, _dp, _dp, _dp, _dp, c_double]),
+            "or_md_force_pm_full": (None, [vp, _dp, _dp, _dp, _dp, c_int]),
+            "or_md_force_eo": (None, [vp, _dp, _dp, _dp, _dp, c_double]),
+            "or_traceless_antihermite": (None, [c_long, _dp, _dp]),
             "or_cg": (c_int, [vp, _dp, c_double, _dp, _dp, c_double, c_int,
                               ctypes.POINTER(c_int), ctypes.POINTER(c_double)]),
             "or_multishift_cg": (c_int, [vp, _dp, c_double, _dp, c_int, _dp, _dp, c_double,
@@ -145,6 +149,29 @@ class Lattice:
         w = np.empty(self.full_size)
         lib().or_dirac_full(self._g, u, w, np.ascontiguousarray(f), int(dag), kappa)
         return w
+
+    def md_force_full(self, u, eta, zeta, kappa):
+        f = np.empty(self.gauge_size)
+        lib().or_md_force_full(self._g, u, f, np.ascontiguousarray(eta),
+                               np.ascontiguousarray(zeta), kappa)
+        return f
+
+    def md_force_pm_full(self, u, fce, eta, zeta, minus):
+        """Accumulates md_force_p (minus=0) or md_force_m (minus=1) into fce."""
+        lib().or_md_force_pm_full(self._g, u, fce, np.ascontiguousarray(eta),
+                                  np.ascontiguousarray(zeta), int(minus))
+        return fce
+
+    def md_force_eo(self, u, eta, zeta, kappa):
+        f = np.empty(self.gauge_size)
+        lib().or_md_force_eo(self._g, u, f, np.ascontiguousarray(eta),
+                             np.ascontiguousarray(zeta), kappa)
+        return f
+
+    def traceless_antihermite(self, m):
+        out = np.empty(self.gauge_size)
+        lib().or_traceless_antihermite(self.gauge_size // 18, out, np.ascontiguousarray(m))
+        return out
 
     def cg(self, u, kappa, b, prec, maxiter):
         x = np.empty(self.half_size)

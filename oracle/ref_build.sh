@@ -39,6 +39,10 @@ lib/Tools/ErrorsHandling/base_errors.cpp
 lib/Tools/ErrorsHandling/convergence_errors.cpp
 lib/PugiXML/pugixml.cpp
 lib/PugiXML/pugi_interface.cpp
+lib/Tools/fieldUtils.cpp
+lib/Tools/sunMatUtils.cpp
+lib/Geometry/shiftField.cpp
+lib/Geometry/autoMap_improved.cpp
 "
 OBJS=""
 for s in $SRCS; do

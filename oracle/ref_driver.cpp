@@ -13,6 +13,9 @@
 //   MultiShiftSolver_CG                 lib/Solver/multiShiftSolver_CG.cpp:10-185
 //   Inverter_WilsonLike::invert         lib/Dirac_ops/eoUtils.cpp:43-80
 //   AntiPeriodicBC<GaugeField>          lib/Dirac_ops/BoundaryConditions/antiPeriodicBC.hpp:25-37
+//   Dirac_Wilson(_EvenOdd)::md_force    lib/Dirac_ops/dirac_wilson.cpp:41-92,
+//                                       dirac_wilson_EvenOdd.hpp:94-116
+//   FieldUtils::TracelessAntihermite    lib/Tools/fieldUtils.cpp:264-274
 //
 // All binary files are raw little-endian float64 in the reference layouts
 // (Format_F spinors, Format_G links, even/odd half fields in esec/osec order).
@@ -28,6 +31,7 @@
 #include "Fopr/fopr.h"
 #include "Solver/solver_CG.hpp"
 #include "Solver/multiShiftSolver_CG.hpp"
+#include "Tools/fieldUtils.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -135,6 +139,7 @@ int main(int argc, char** argv) {
       "       iroiro_ref cg     Lx Ly Lz Lt mass gauge src_even prec maxiter outdir\n"
       "       iroiro_ref mshift Lx Ly Lz Lt mass gauge src_even prec maxiter outdir s0 s1 ...\n"
       "       iroiro_ref invert Lx Ly Lz Lt mass gauge src_full prec maxiter outdir\n"
+      "       iroiro_ref mdforce Lx Ly Lz Lt mass gauge eta_even zeta_even eta_full zeta_full outdir\n"
       "       iroiro_ref bench  Lx Ly Lz Lt mass nrep [cg_prec]\n");
     return 2;
   }
@@ -213,6 +218,27 @@ int main(int argc, char** argv) {
     write_field(out + "/dfull.bin", Df.mult(ff));
     write_field(out + "/dfull_dag.bin", Df.mult_dag(ff));
     std::printf("IIREF_OK ops kappa=%.17g\n", D.getKappa());
+    return 0;
+  }
+
+  if (cmd == "mdforce") {
+    Field ee = read_field(argv[8], hsz);
+    Field ze = read_field(argv[9], hsz);
+    Field ef = read_field(argv[10], fsz);
+    Field zf = read_field(argv[11], fsz);
+    std::string out = argv[12];
+    GaugeField f_eo;
+    f_eo.data = D.md_force(ee, ze);
+    write_field(out + "/mdf_eo.bin", f_eo.data);
+    write_field(out + "/ta_eo.bin", FieldUtils::TracelessAntihermite(f_eo).data);
+    Dirac_Wilson Df(mass, &u);
+    write_field(out + "/mdf_full.bin", Df.md_force(ef, zf));
+    Field fp(gsz), fm(gsz);
+    Df.md_force_p(fp, ef, zf);
+    Df.md_force_m(fm, ef, zf);
+    write_field(out + "/mdf_p.bin", fp);
+    write_field(out + "/mdf_m.bin", fm);
+    std::printf("IIREF_OK mdforce\n");
     return 0;
   }
 

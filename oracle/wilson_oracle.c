@@ -304,6 +304,116 @@ void or_dirac_full(const or_geom* g, const double* u, double* w,
 }
 
 /* ------------------------------------------------------------------ */
+/* MD force.  Dirac_Wilson::mkfrc (dirac_wilson.cpp:62-92): for direction
+ * mu, xie = mult_p[mu](eta) is the forward hop term U_mu(x)(1-gamma_mu)
+ * eta(x+mu) at the operator's output sites x (no -kappa), and
+ *   fce(x,mu)_{ab} += sum_s xie(a,s,x) conj(zeta(b,s,x)),
+ * spelled out in real arithmetic as the reference does it.  out_sites[i]
+ * is the lexicographic site of output i (the operator's gp_ list) and
+ * in_index maps a lexicographic site to its row in eta. */
+static void mkfrc(const or_geom* g, const double* u, double* fce,
+                  const double* eta, const double* zeta, int mu, long nout,
+                  const int* out_sites, const int* in_index) {
+  for (long i = 0; i < nout; ++i) {
+    int s = out_sites[i];
+    int sp = g->nb_fw[mu * g->V + s];
+    double xie[24];
+    for (int k = 0; k < 24; ++k) xie[k] = 0.0;
+    hop_dir(xie, eta + 24 * (long)in_index[sp], u + 18 * (s + g->V * mu), mu, 1);
+    const double* z = zeta + 24 * i;
+    double* f = fce + 18 * ((long)s + (long)g->V * mu);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int sp4 = 0; sp4 < 4; ++sp4) {
+          f[2 * (3 * a + b)] += z[RE(sp4, b)] * xie[RE(sp4, a)] + z[IM(sp4, b)] * xie[IM(sp4, a)];
+          f[2 * (3 * a + b) + 1] += z[RE(sp4, b)] * xie[IM(sp4, a)] - z[IM(sp4, b)] * xie[RE(sp4, a)];
+        }
+  }
+}
+
+/* md_force_p / md_force_m of one operator (dirac_wilson.cpp:41-50). */
+static void md_force_pm(const or_geom* g, const double* u, double* fce,
+                        const double* eta, const double* zeta, long nsites,
+                        const int* out_sites, const int* in_index, int minus) {
+  if (!minus) {
+    for (int mu = 0; mu < 4; ++mu)
+      mkfrc(g, u, fce, eta, zeta, mu, nsites, out_sites, in_index);
+    return;
+  }
+  double* zt5 = (double*)malloc(sizeof(double) * 24 * nsites);
+  double* et5 = (double*)malloc(sizeof(double) * 24 * nsites);
+  or_gamma5(nsites, zt5, zeta);
+  or_gamma5(nsites, et5, eta);
+  for (int mu = 0; mu < 4; ++mu)
+    mkfrc(g, u, fce, zt5, et5, mu, nsites, out_sites, in_index);
+  free(zt5);
+  free(et5);
+}
+
+void or_md_force_full(const or_geom* g, const double* u, double* fce,
+                      const double* eta, const double* zeta, double kappa) {
+  long ng = 18L * 4 * g->V;
+  int* ident = (int*)malloc(sizeof(int) * g->V);
+  for (long s = 0; s < g->V; ++s) ident[s] = (int)s;
+  for (long i = 0; i < ng; ++i) fce[i] = 0.0;
+  md_force_pm(g, u, fce, eta, zeta, g->V, ident, ident, 0);
+  md_force_pm(g, u, fce, eta, zeta, g->V, ident, ident, 1);
+  for (long i = 0; i < ng; ++i) fce[i] *= -kappa;
+  free(ident);
+}
+
+void or_md_force_pm_full(const or_geom* g, const double* u, double* fce,
+                         const double* eta, const double* zeta, int minus) {
+  int* ident = (int*)malloc(sizeof(int) * g->V);
+  for (long s = 0; s < g->V; ++s) ident[s] = (int)s;
+  md_force_pm(g, u, fce, eta, zeta, g->V, ident, ident, minus);
+  free(ident);
+}
+
+/* Dirac_Wilson_EvenOdd::md_force (dirac_wilson_EvenOdd.hpp:94-116):
+ *   md_force_eo(fce, Doe eta, zeta)      = Deo.md_force_p + Doe.md_force_m
+ *   md_force_oe(fce, eta, Deo^dag zeta)  = Doe.md_force_p + Deo.md_force_m
+ *   fce *= kappa.
+ * Deo has output sites esec, Doe osec (dirac_wilson.hpp:89-114). */
+void or_md_force_eo(const or_geom* g, const double* u, double* fce,
+                    const double* eta, const double* zeta, double kappa) {
+  long ng = 18L * 4 * g->V;
+  long nh = 24L * g->Vh;
+  double* eta_o = (double*)malloc(sizeof(double) * nh);
+  double* zeta_o = (double*)malloc(sizeof(double) * nh);
+  or_hop_eo(g, u, eta_o, eta, 2, 0, kappa);   /* mult_oe(eta) = Doe.mult   */
+  or_hop_eo(g, u, zeta_o, zeta, 2, 1, kappa); /* mult_eo_dag = Doe.mult_dag */
+  for (long i = 0; i < ng; ++i) fce[i] = 0.0;
+  md_force_pm(g, u, fce, eta_o, zeta, g->Vh, g->esec, g->hidx, 0);  /* Deo p */
+  md_force_pm(g, u, fce, eta_o, zeta, g->Vh, g->osec, g->hidx, 1);  /* Doe m */
+  md_force_pm(g, u, fce, eta, zeta_o, g->Vh, g->osec, g->hidx, 0);  /* Doe p */
+  md_force_pm(g, u, fce, eta, zeta_o, g->Vh, g->esec, g->hidx, 1);  /* Deo m */
+  for (long i = 0; i < ng; ++i) fce[i] *= kappa;
+  free(eta_o);
+  free(zeta_o);
+}
+
+/* FieldUtils::TracelessAntihermite (fieldUtils.cpp:264-274) with
+ * SUNmatUtils::anti_hermite_traceless (sunMatUtils.cpp:50-60) and
+ * SUNmatrix::anti_hermite (sunMat.hpp:139-157): per link
+ *   out = (m - m^dag)/2 - i Im tr(m)/3. */
+void or_traceless_antihermite(long nlinks, double* out, const double* in) {
+  for (long l = 0; l < nlinks; ++l) {
+    const double* m = in + 18 * l;
+    double* o = out + 18 * l;
+    double imtr = (m[1] + m[9] + m[17]) / 3.0;
+    double t[18];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        int ab = 2 * (3 * a + b), ba = 2 * (3 * b + a);
+        t[ab] = 0.5 * (m[ab] - m[ba]);
+        t[ab + 1] = 0.5 * (m[ab + 1] + m[ba + 1]) - (a == b ? imtr : 0.0);
+      }
+    memcpy(o, t, sizeof(t));
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* BLAS-1 with the reference's semantics: Field::operator* = sum over all
  * reals (field.inl:58-62, sequential valarray sum), norm = sqrt(sum a^2). */
 static double dot(const double* a, const double* b, long n) {

@@ -61,6 +61,30 @@ void or_ddagd(const or_geom* g, const double* u, double* w, const double* f,
 void or_dirac_full(const or_geom* g, const double* u, double* w,
                    const double* f, int dag, double kappa);
 
+/* MD force of the full-index Dirac_Wilson (dirac_wilson.cpp:41-92):
+ * fce = -kappa (md_force_p + md_force_m), a gauge-layout field, from full
+ * fields eta and zeta.  mkfrc: fce(x,mu)_{ab} += sum_s [U_mu(x)(1-gamma_mu)
+ * eta(x+mu)]_{a,s} conj(zeta_{b,s}(x)); md_force_m uses (g5 zeta, g5 eta).
+ * Pinned in tests/test_oracle_force.py: Re<zeta, D[U + eps X U] eta> is
+ * affine in eps with slope Re sum tr(X fce), and the reference's own
+ * md_force on the golden gauge fields. */
+void or_md_force_full(const or_geom* g, const double* u, double* fce,
+                      const double* eta, const double* zeta, double kappa);
+/* md_force_p (minus = 0) or md_force_m (minus = 1) of the full-index
+ * Dirac_Wilson, accumulated into fce without a kappa factor
+ * (dirac_wilson.cpp:41-50). */
+void or_md_force_pm_full(const or_geom* g, const double* u, double* fce,
+                         const double* eta, const double* zeta, int minus);
+/* MD force of Dirac_Wilson_EvenOdd (dirac_wilson_EvenOdd.hpp:94-116) from
+ * even half fields eta and zeta: a gauge-layout field over all links.
+ * Pinned like or_md_force_full with S = Re<zeta, Dhat eta> (quadratic in
+ * the links, so a central difference is exact) and by the reference. */
+void or_md_force_eo(const or_geom* g, const double* u, double* fce,
+                    const double* eta, const double* zeta, double kappa);
+/* FieldUtils::TracelessAntihermite (fieldUtils.cpp:264-274):
+ * (m - m^dag)/2 - i Im tr(m)/3 per 3x3 link matrix. */
+void or_traceless_antihermite(long nlinks, double* out, const double* in);
+
 /* Solver_CG on Dhat^dag Dhat (solver_CG.cpp:29-160): x0 = b, stop when
  * |r|^2/|b|^2 < prec.  Returns 0 when converged, 1 otherwise.
  * iterations = loop index at exit; diff = |A x - b|/|b|. */

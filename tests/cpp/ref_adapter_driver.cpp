@@ -19,6 +19,7 @@
 #include "Measurements/FermionicM/qprop_EvenOdd.hpp"
 #include "Measurements/FermionicM/qprop_MultiShift.hpp"
 #include "Measurements/FermionicM/source_types.hpp"
+#include "Tools/fieldUtils.hpp"
 
 #include "iroiro_ref_adapter.hpp"
 
@@ -155,6 +156,36 @@ int main(int argc, char** argv) {
   double r_qms = 0.0;
   for (size_t k = 0; k < qc.size(); ++k) r_qms = std::max(r_qms, rel(qg[k], qc[k]));
 
+  // Action_Nf2::md_force's sequence (action_Nf2.cpp:69-86) through the base
+  // classes the action holds (DiracWilsonLike*, Solver*).  The action itself
+  // is not linked: its translation unit needs Smearing/smartConf.cpp, which
+  // includes a header the reference tree does not contain.
+  const DiracWilsonLike* Dc = &Dcpu;
+  const DiracWilsonLike* Dg = &Dgpu;
+  Field eta_c(fe.size()), eta_g(fe.size());
+  sc->solve(eta_c, fe);
+  sg->solve(eta_g, fe);
+  GaugeField fc, fg;
+  fc.data = Dc->md_force(eta_c, Dc->mult(eta_c));
+  fg.data = Dg->md_force(eta_g, Dg->mult(eta_g));
+  double r_force = rel(FieldUtils::TracelessAntihermite(fg).data,
+                       FieldUtils::TracelessAntihermite(fc).data);
+  // the reference operator's md_force on the same (eta, zeta): operator parity alone
+  double r_force_op = rel(Dg->md_force(eta_c, Dc->mult(eta_c)), fc.data);
+  // full-index md_force, md_force_p and md_force_m through DiracWilsonLike
+  const DiracWilsonLike* DFc = &DFcpu;
+  const DiracWilsonLike* DFg = &DFgpu;
+  Field zf = DFc->mult(src);
+  double r_fforce = rel(DFg->md_force(src, zf), DFc->md_force(src, zf));
+  Field pmc((size_t)18 * 4 * V), pmg((size_t)18 * 4 * V);
+  DFc->md_force_p(pmc, src, zf);
+  DFg->md_force_p(pmg, src, zf);
+  DFc->md_force_m(pmc, zf, src);
+  DFg->md_force_m(pmg, zf, src);
+  r_fforce = std::max(r_fforce, rel(pmg, pmc));
+
+  std::printf("IIHIP {\"force_rel\": %.3e, \"force_op_rel\": %.3e, "
+              "\"full_force_rel\": %.3e}\n", r_force, r_force_op, r_fforce);
   std::printf("IIHIP {\"ops_rel\": %.3e, \"cg_rel\": %.3e, \"cg_iter_cpu\": %d, "
               "\"cg_iter_gpu\": %d, \"ms_rel\": %.3e, \"ms_nconv_cpu\": %d, "
               "\"ms_nconv_gpu\": %d, \"inv_rel\": %.3e, \"prop_rel\": %.3e, \"ncol\": %d, "

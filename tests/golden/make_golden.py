@@ -13,7 +13,13 @@ Dirac_Wilson_EvenOdd mult/mult_dag/gamma5, Fopr_DdagD, the full-index
 Dirac_Wilson mult/mult_dag, Solver_CG, MultiShiftSolver_CG and
 Inverter_WilsonLike::invert.
 
-Usage: python tests/golden/make_golden.py   (writes tests/golden/*.npz)
+The MD-force fixtures (<case>_mdforce.npz) reuse the gauge field of <case>
+and add Gaussian eta/zeta fields (even half and full); the reference calls
+are Dirac_Wilson_EvenOdd::md_force, FieldUtils::TracelessAntihermite of it,
+and the full-index Dirac_Wilson md_force, md_force_p and md_force_m.
+
+Usage: python tests/golden/make_golden.py [mdforce]
+       (writes tests/golden/*.npz; "mdforce" writes only the MD-force files)
 """
 import json
 import os
@@ -101,11 +107,43 @@ def make(name, dims, mass, eps, seed, src_seed, solvers):
                           if k not in ("kappa_line",)})
 
 
+MDFORCE_CASES = {"wilson_aniso_4x6x2x4": 0x5151}
+
+
+def make_mdforce(base, seed):
+    z = dict(np.load(os.path.join(HERE, base + ".npz")))
+    dims = tuple(int(v) for v in z["dims"])
+    lat = oracle.Lattice(dims)
+    res = {"base": base, "seed": seed}
+    res["eta_e"] = oracle.gauss_field(lat.half_size, seed)
+    res["zeta_e"] = oracle.gauss_field(lat.half_size, seed + 1)
+    res["eta_f"] = oracle.gauss_field(lat.full_size, seed + 2)
+    res["zeta_f"] = oracle.gauss_field(lat.full_size, seed + 3)
+    d = [str(x) for x in dims]
+    with tempfile.TemporaryDirectory() as tmp:
+        z["u_apbc"].tofile(os.path.join(tmp, "u.bin"))
+        names = []
+        for k in ("eta_e", "zeta_e", "eta_f", "zeta_f"):
+            res[k].tofile(os.path.join(tmp, k + ".bin"))
+            names.append(os.path.join(tmp, k + ".bin"))
+        run(["mdforce"] + d + [repr(float(z["mass"])), os.path.join(tmp, "u.bin")] + names
+            + [tmp])
+        for k in ("mdf_eo", "ta_eo", "mdf_full", "mdf_p", "mdf_m"):
+            res[k] = np.fromfile(os.path.join(tmp, k + ".bin"))
+    name = base + "_mdforce"
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **res)
+    print("wrote", name)
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("oracle/_ref/iroiro_ref is not built (run oracle/ref_build.sh)")
-    for name, kw in CASES.items():
-        make(name, **kw)
+    only_mdforce = len(sys.argv) > 1 and sys.argv[1] == "mdforce"
+    if not only_mdforce:
+        for name, kw in CASES.items():
+            make(name, **kw)
+    for base, seed in MDFORCE_CASES.items():
+        make_mdforce(base, seed)
 
 
 if __name__ == "__main__":

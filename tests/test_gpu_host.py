@@ -1,6 +1,8 @@
 """Drop-in test of the C++ host side: the reference's own
 Inverter_WilsonLike, Qprop_EvenOdd and Qprop_MultiShift (with Source_local), Solver and
-MultiShiftSolver base-class calls, run once with the reference CPU classes
+MultiShiftSolver base-class calls, Action_Nf2::md_force's sequence of
+Dirac/Solver calls and FieldUtils::TracelessAntihermite, and the full-index
+md_force/_p/_m, run once with the reference CPU classes
 and once with the device classes of iroiro_amd/host/iroiro_ref_adapter.hpp,
 in one process (oracle/_ref/iroiro_ref_hip, built by oracle/ref_build.sh)."""
 import json
@@ -26,8 +28,10 @@ def test_reference_drivers_on_device_classes(golden, tmp_path):
                                           str(tmp_path / "src.bin")],
                          capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr[-2000:]
-    line = [l for l in out.stdout.splitlines() if l.startswith("IIHIP ")][-1]
-    r = json.loads(line[len("IIHIP "):])
+    r = {}
+    for line in out.stdout.splitlines():
+        if line.startswith("IIHIP "):
+            r.update(json.loads(line[len("IIHIP "):]))
     assert r["ops_rel"] < 1e-13
     assert r["cg_iter_gpu"] == r["cg_iter_cpu"]
     assert r["cg_rel"] < 1e-12
@@ -41,3 +45,8 @@ def test_reference_drivers_on_device_classes(golden, tmp_path):
     # (multiShiftSolver_CG.cpp:179), so 12 x 12 entries, 2 per column nonzero.
     assert r["full_ops_rel"] < 1e-13
     assert r["qprop_ms_n"] == 144 and r["qprop_ms_rel"] < 1e-10
+    # MD force: Action_Nf2::md_force's sequence (solve, mult, md_force,
+    # TracelessAntihermite) and the full-index md_force / md_force_p / _m
+    assert r["force_op_rel"] < 1e-13
+    assert r["force_rel"] < 1e-11
+    assert r["full_force_rel"] < 1e-13
