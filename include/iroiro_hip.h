@@ -126,6 +126,45 @@ ii_status ii_fopr_ddagd_mult(ii_context* ctx, double kappa, double* out, const d
 ii_status ii_dirac_full_mult(ii_context* ctx, double kappa, double* out, const double* in,
                              int dag);
 
+/* ------------------------------------------------ rational approximation */
+/* RationalApprox (lib/Tools/RationalApprox/rationalapprox.cpp:20-80): the
+ * minimax relative-error rational approximation of x^(exp_num/exp_den) on
+ * [low, high] with num_deg = den_deg (the reference asserts equal degrees),
+ * in partial fractions
+ *   x^(a/b)  ~ *a0  + sum_i res[i]  / (x + pole[i])    (Residuals, Poles, Const)
+ *   x^(-a/b) ~ *ia0 + sum_i ires[i] / (x + ipole[i])   (InvResiduals, ...)
+ * poles ascending; error = the maximum relative error.  Host computation
+ * (no device use): a levelled-error Remez exchange in 113-bit floating point
+ * in place of the reference's MPFR AlgRemez (lib/Tools/Remez/alg_remez.cpp);
+ * ctx may be NULL.  II_ERR_NOT_CONVERGED when the exchange does not level
+ * the error (too high a degree for the arithmetic). */
+ii_status ii_rational_approx(ii_context* ctx, int num_deg, int den_deg, int exp_num, int exp_den,
+                             double low, double high, double* res, double* pole, double* a0,
+                             double* ires, double* ipole, double* ia0, double* error);
+
+/* RationalSolver_CG::solve / solve_inv (lib/Solver/rationalSolver_CG.cpp:
+ * 12-68) on Fopr_DdagD(Dirac_Wilson_EvenOdd): the shifted solutions x_i of
+ * (Dhat^dag Dhat + poles[i]) x_i = src, then sol = a0 src + sum res[i] x_i
+ * (pass Poles/Residuals/Const for solve, the Inv* set for solve_inv).
+ * precision II_PREC_DOUBLE: the reference's MultiShiftSolver_CG recursion,
+ * prec = its squared stopping test (multiShiftSolver_CG.cpp:95-185);
+ * II_PREC_SINGLE/HALF/BF16: the mixed-precision multishift, prec = the
+ * target true residual |b - (A + pole_i) x_i| / |b| of every shift.
+ * out: the multishift's summary (largest over the shifts). */
+ii_status ii_rational_solve(ii_context* ctx, double kappa, double* sol, const double* src,
+                            double a0, const double* res, const double* poles, int npoles,
+                            double prec, int maxiter, int precision, ii_solver_output* out);
+/* Action_Nf::md_force (lib/Action/action_Nf.cpp:84-110, no smearing) for npf
+ * pseudofermions phi (even half fields back to back): per pseudofermion
+ * eta_i = (Dhat^dag Dhat + ipoles[i])^-1 phi (solve_noReconstruct), then
+ * force = TracelessAntihermite(sum_pf sum_i ires[i] md_force(eta_i, Dhat eta_i)),
+ * with the MolecularDynApprox InvResiduals / InvPoles.  precision and prec
+ * as in ii_rational_solve.  force: gauge layout, 18*4*V doubles. */
+ii_status ii_action_nf_md_force(ii_context* ctx, double kappa, const double* phi, int npf,
+                                const double* ires, const double* ipoles, int npoles,
+                                double prec, int maxiter, int precision, double* force,
+                                ii_solver_output* out);
+
 /* ------------------------------------------------- MD force on host arrays */
 /* Dirac_Wilson_EvenOdd::md_force (dirac_wilson_EvenOdd.hpp:94-116):
  * eta, zeta even half fields; fce (out) a gauge-layout field of

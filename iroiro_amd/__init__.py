@@ -448,6 +448,208 @@ class MultiShiftSolver_CG:
         return so, out.diff, out.iterations
 
 
+class RationalApprox:
+    """RationalApprox (lib/Tools/RationalApprox/rationalapprox.cpp:11-103,
+    rationalapprox.hpp:56-96): the minimax relative-error approximations
+        x^(a/b)  ~ Const()    + sum Residuals()[i]    / (x + Poles()[i])
+        x^(-a/b) ~ InvConst() + sum InvResiduals()[i] / (x + InvPoles()[i])
+    on [low, high], computed by the library's quad-precision Remez
+    (ii_rational_approx).  `precision` (the reference's MPFR digits) is
+    accepted for the same parameter list; the arithmetic is 113-bit."""
+
+    def __init__(self, num_deg, den_deg, exp_num, exp_den, precision, low, high):
+        if num_deg != den_deg:
+            raise ValueError("numerator and denominator degrees must be equal "
+                             "(rationalapprox.cpp:45)")
+        self.params = dict(num_deg=int(num_deg), den_deg=int(den_deg), exp_num=int(exp_num),
+                           exp_den=int(exp_den), precision=int(precision), low=float(low),
+                           high=float(high))
+        n = int(num_deg)
+        res, pole, ires, ipole = (np.empty(n) for _ in range(4))
+        a0, ia0, err = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        st = _lib.load().ii_rational_approx(None, n, int(den_deg), int(exp_num), int(exp_den),
+                                            float(low), float(high), _lib.dptr(res),
+                                            _lib.dptr(pole), ctypes.byref(a0), _lib.dptr(ires),
+                                            _lib.dptr(ipole), ctypes.byref(ia0),
+                                            ctypes.byref(err))
+        if st != _lib.II_OK:
+            raise IIError(st, "ii_rational_approx: the Remez exchange failed for degree %d on "
+                              "[%g, %g]" % (n, low, high))
+        self.RA_res, self.RA_pole, self.RA_a0 = res, pole, a0.value
+        self.I_RA_res, self.I_RA_pole, self.I_RA_a0 = ires, ipole, ia0.value
+        self.error = err.value
+
+    def exponent(self):
+        return self.params["exp_num"] / self.params["exp_den"]
+
+    def Residuals(self):
+        return self.RA_res.copy()
+
+    def Poles(self):
+        return self.RA_pole.copy()
+
+    def Const(self):
+        return self.RA_a0
+
+    def InvResiduals(self):
+        return self.I_RA_res.copy()
+
+    def InvPoles(self):
+        return self.I_RA_pole.copy()
+
+    def InvConst(self):
+        return self.I_RA_a0
+
+    def rescale(self, min_in, max_in):
+        """rationalapprox.cpp:82-103: the coefficients are taken as computed
+        on [a, 1] and stretched to 1.05 * max (min enters only as 0.95 min,
+        unused by the reference)."""
+        mx = 1.05 * max_in
+        factor = mx ** self.exponent()
+        self.RA_a0 *= factor
+        self.RA_res = self.RA_res * (factor * mx)
+        self.RA_pole = self.RA_pole * mx
+        factor = mx ** (-self.exponent())
+        self.I_RA_a0 *= factor
+        self.I_RA_res = self.I_RA_res * (factor * mx)
+        self.I_RA_pole = self.I_RA_pole * mx
+
+
+class RationalSolver_CG:
+    """RationalSolver_CG (lib/Solver/rationalSolver_CG.cpp:12-111) over the
+    device multishift of a MultiShiftSolver_CG on Fopr_DdagD(Dirac_Wilson_
+    EvenOdd); the reconstruction sum runs on the device too.  precision
+    "double" is the reference's FP64 recursion (stop_cond = its squared
+    test); "single"/"half"/"bf16" use the mixed-precision multishift with
+    stop_cond as the target true residual of every shift."""
+
+    def __init__(self, ms_solver, approx=None, precision="double"):
+        self.MS_Solver_ = ms_solver
+        self.precision = PRECISIONS[precision]
+        self.Residuals = self.Poles = self.InvResiduals = self.InvPoles = None
+        self.ConstTerm = self.InvConstTerm = 0.0
+        if approx is not None:
+            self.set_Approx(approx)
+
+    def set_Approx(self, RA):
+        self.Residuals, self.Poles, self.ConstTerm = RA.Residuals(), RA.Poles(), RA.Const()
+        self.InvResiduals, self.InvPoles = RA.InvResiduals(), RA.InvPoles()
+        self.InvConstTerm = RA.InvConst()
+
+    def check_DdagD(self):
+        return True
+
+    def _D(self):
+        D = self.MS_Solver_.opr.D
+        if not isinstance(D, Dirac_Wilson_EvenOdd):
+            raise TypeError("RationalSolver_CG on the device runs on Dirac_Wilson_EvenOdd")
+        return D
+
+    def _solve(self, sol, src, a0, res, poles, what):
+        if res is None or len(res) == 0:
+            raise RuntimeError("[RationalSolver] Rational Approximation not initialized yet")
+        D = self._D()
+        b = _lib.as_f64(src, D.fsize())
+        x = np.empty(D.fsize())
+        res = _lib.as_f64(res)
+        poles = _lib.as_f64(poles)
+        out = _lib.SolverOutputC()
+        st = _lib.load().ii_rational_solve(D.ctx.handle, D.kpp, _lib.dptr(x), _lib.dptr(b), a0,
+                                           _lib.dptr(res), _lib.dptr(poles), len(res),
+                                           self.MS_Solver_.prec, self.MS_Solver_.MaxIter,
+                                           self.precision, ctypes.byref(out))
+        sol[...] = x
+        if st == _lib.II_ERR_NOT_CONVERGED:
+            raise NotConverged(st, "Not converged.")
+        D.ctx.check(st, what)
+        so = SolverOutput(out)
+        so.Msg = what
+        return so
+
+    def solve(self, sol, src):
+        """(M^dag M)^(a/b) src."""
+        return self._solve(sol, src, self.ConstTerm, self.Residuals, self.Poles,
+                           "RationalSolver_CG::solve")
+
+    def solve_inv(self, sol, src):
+        """(M^dag M)^(-a/b) src."""
+        return self._solve(sol, src, self.InvConstTerm, self.InvResiduals, self.InvPoles,
+                           "RationalSolver_CG::solve_inv")
+
+    def _no_reconstruct(self, shifted_sol, src, poles):
+        if poles is None or len(poles) == 0:
+            raise RuntimeError("[RationalSolver] Rational Approximation not initialized yet")
+        shifted_sol[:] = [np.empty(self._D().fsize()) for _ in range(len(poles))]
+        so, _, _ = self.MS_Solver_.solve(shifted_sol, src, poles)
+        return so
+
+    def solve_noReconstruct(self, shifted_sol, src):
+        """The shifted solutions with the inverse poles (force term)."""
+        return self._no_reconstruct(shifted_sol, src, self.InvPoles)
+
+    def solve_noReconstruct_inv(self, shifted_sol, src):
+        return self._no_reconstruct(shifted_sol, src, self.Poles)
+
+
+class Action_Nf:
+    """Action_Nf (lib/Action/action_Nf.cpp, action_Nf.hpp:61-140), the
+    RHMC term for n_flav flavours with n_pseudof pseudofermions on an
+    even/odd Wilson kernel, no smearing.  Approximations, as the reference
+    builds them: Metropolis x^(Nf/2n), molecular dynamics x^(Nf/2n) (used
+    through its inverse), pseudofermions x^(Nf/4n); degree, precision and
+    bounds per step in the order Metropolis, MD, PF.  md_force runs the
+    shifted solves, the per-pole MD force and the projection on the device
+    in one call."""
+
+    def __init__(self, D, slv, n_flav, n_pseudof, degree, precision, b_low, b_high):
+        self.D, self.slv = D, slv
+        self.n_flav, self.n_pseudof = int(n_flav), int(n_pseudof)
+        mk = lambda k, den: RationalApprox(degree[k], degree[k], self.n_flav, den,
+                                           precision[k], b_low[k], b_high[k])
+        self.MetropolisApprox_ = mk(0, 2 * self.n_pseudof)
+        self.MolecularDynApprox_ = mk(1, 2 * self.n_pseudof)
+        self.PseudoFermionsApprox_ = mk(2, 4 * self.n_pseudof)
+        self.phi_ = [np.zeros(D.fsize()) for _ in range(self.n_pseudof)]
+
+    def init(self, seed):
+        """phi_i = (M^dag M)^(Nf/4n) xi_i (action_Nf.cpp:39-57); numpy's
+        Gaussian generator, seeded, in place of the reference's MPrand."""
+        self.slv.set_Approx(self.PseudoFermionsApprox_)
+        rng = np.random.default_rng(seed)
+        for i in range(self.n_pseudof):
+            self.slv.solve(self.phi_[i], rng.standard_normal(self.D.fsize()))
+
+    def calc_H(self):
+        """sum_i phi_i . (M^dag M)^(-Nf/2n) phi_i (action_Nf.cpp:59-79)."""
+        self.slv.set_Approx(self.MetropolisApprox_)
+        H = 0.0
+        tmp = np.empty(self.D.fsize())
+        for i in range(self.n_pseudof):
+            self.slv.solve_inv(tmp, self.phi_[i])
+            H += float(np.dot(self.phi_[i], tmp))
+        return H
+
+    def md_force(self):
+        """action_Nf.cpp:81-110 on the device (ii_action_nf_md_force)."""
+        self.slv.set_Approx(self.MolecularDynApprox_)
+        ms = self.slv.MS_Solver_
+        phi = np.concatenate(self.phi_)
+        ires = _lib.as_f64(self.MolecularDynApprox_.InvResiduals())
+        ipoles = _lib.as_f64(self.MolecularDynApprox_.InvPoles())
+        force = np.empty(self.D.gsize())
+        out = _lib.SolverOutputC()
+        st = _lib.load().ii_action_nf_md_force(self.D.ctx.handle, self.D.kpp, _lib.dptr(phi),
+                                               self.n_pseudof, _lib.dptr(ires),
+                                               _lib.dptr(ipoles), len(ires), ms.prec,
+                                               ms.MaxIter, self.slv.precision,
+                                               _lib.dptr(force), ctypes.byref(out))
+        if st == _lib.II_ERR_NOT_CONVERGED:
+            raise NotConverged(st, "Not converged.")
+        self.D.ctx.check(st, "Action_Nf::md_force")
+        self.last_solve = SolverOutput(out)
+        return force
+
+
 class MultiShiftSolver_Mixed:
     """Mixed-precision multishift (no reference counterpart): all shifts
     fused in an FP32 shifted recursion, then a batched reliable-update CG

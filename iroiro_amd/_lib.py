@@ -37,6 +37,7 @@ EXPORTS = [
     "ii_autotune", "ii_bench_hop", "ii_bench_copy",
     "ii_dirac_md_force", "ii_dirac_full_md_force", "ii_dirac_full_md_force_pm",
     "ii_traceless_antihermite", "ii_action_nf2_md_force", "ii_bench_md_force",
+    "ii_rational_approx", "ii_rational_solve", "ii_action_nf_md_force",
 ]
 
 NET_LAYERS = 4
@@ -128,6 +129,9 @@ def load():
         "ii_traceless_antihermite": [vp, dp, dp],
         "ii_action_nf2_md_force": [vp, cd, dp, cd, ci, dp, so],
         "ii_bench_md_force": [vp, ci, dp, dp],
+        "ii_rational_approx": [vp, ci, ci, ci, ci, cd, cd, dp, dp, dp, dp, dp, dp, dp],
+        "ii_rational_solve": [vp, cd, dp, dp, cd, dp, dp, ci, cd, ci, ci, so],
+        "ii_action_nf_md_force": [vp, cd, dp, ci, dp, dp, ci, cd, ci, ci, dp, so],
         "ii_solver_cg": [vp, cd, dp, dp, cd, ci, so],
         "ii_multishift_cg": [vp, cd, dp, dp, dp, ci, cd, ci, so],
         "ii_eo_invert": [vp, cd, dp, dp, cd, ci, so],

@@ -208,7 +208,7 @@ static ii_status ws_force(ii_context* ctx, double2** p) {
 }
 
 ii_status md_force_eo64(ii_context* ctx, double kappa, const double2* eta, const double2* zeta,
-                        double2* F, bool ta) {
+                        double2* F, bool ta, double weight, bool accumulate) {
   double2 *eta_o, *zeta_o;
   II_CHECK(ws_half(ctx, kWsEtaO, 1, &eta_o));
   II_CHECK(ws_half(ctx, kWsZetaO, 1, &zeta_o));
@@ -226,7 +226,7 @@ ii_status md_force_eo64(ii_context* ctx, double kappa, const double2* eta, const
   fp.g5psi = (1u << 1) | (1u << 2);
   fp.g5chi = (1u << 1) | (1u << 2);
   fp.npairs = 2;
-  return launch_force(ctx, fp, F, kappa, 0, 2, false, ta);
+  return launch_force(ctx, fp, F, kappa * weight, 0, 2, accumulate, ta);
 }
 
 // Full-index pairs on fields split into even/odd halves.

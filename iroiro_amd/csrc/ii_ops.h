@@ -4,6 +4,7 @@
 #include "ii_internal.h"
 
 #include <type_traits>
+#include <vector>
 
 namespace ii {
 
@@ -94,10 +95,22 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
 
 // MD force (ii_force.hip)
 // Force field F (device gauge layout) of Dirac_Wilson_EvenOdd::md_force
-// from even half fields eta, zeta (dirac_wilson_EvenOdd.hpp:94-116);
-// ta: apply the traceless anti-hermitian projection on the way out.
+// from even half fields eta, zeta (dirac_wilson_EvenOdd.hpp:94-116), times
+// weight; accumulate: add to F; ta: apply the (linear, idempotent)
+// traceless anti-hermitian projection on the way out.
 ii_status md_force_eo64(ii_context* ctx, double kappa, const double2* eta, const double2* zeta,
-                        double2* F, bool ta);
+                        double2* F, bool ta, double weight = 1.0, bool accumulate = false);
+// multishift on the even Schur system with the reference's recursion
+// (ii_solvers.hip); full = false: x and b even half fields
+ii_status multishift_reference(ii_context* ctx, double kappa, double2* x, const double2* b,
+                               const double* shifts, int nshift, double prec, int maxiter,
+                               ii_solver_output* out, std::vector<double>* per_shift,
+                               bool full);
+// mixed-precision multishift: fused FP32 stage + batched refinement to the
+// true residual tol (ii_solvers.hip); out: nshift entries
+ii_status multishift_mixed_dev(ii_context* ctx, double kappa, double2* x, const double2* b,
+                               const double* shifts, int nshift, double tol, int precision,
+                               int maxiter, ii_solver_output* out);
 
 // learned preconditioner (ii_learned.hip)
 void learned_release(ii_context* ctx);

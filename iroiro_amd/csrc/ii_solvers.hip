@@ -1582,6 +1582,36 @@ k_copy_stream(const double4* __restrict__ a, double4* __restrict__ b, size_t n) 
     b[i] = a[i];
 }
 
+// Mixed-precision multishift on device fields (x: nshift even half fields,
+// b: one): stage 1 runs the reference's shifted recursion with FP32 vectors
+// and operator, all shifts fused per launch, to |r|/|b| < max(tol, 1e-6);
+// stage 2 refines every shift at once with a batched reliable-update CG.
+ii_status multishift_mixed_dev(ii_context* ctx, double kappa, double2* x, const double2* b,
+                               const double* shifts, int nshift, double tol, int precision,
+                               int maxiter, ii_solver_output* out) {
+  if (nshift < 1 || nshift > kMaxShift || !(tol > 0) || precision < II_PREC_DOUBLE ||
+      precision > II_PREC_BF16)
+    return set_error(ctx, II_ERR_BAD_ARG, "multishift_mixed: bad arguments");
+  auto t0 = std::chrono::steady_clock::now();
+  SF x32;
+  II_CHECK(fieldP(ctx, 42, II_PREC_SINGLE, nshift, &x32));
+  const double t1 = std::max(tol, 1e-6);
+  int its1 = 0;
+  II_CHECK(multishift_sloppy(ctx, kappa, (float2*)x32.v, b, shifts, nshift, t1 * t1, maxiter,
+                             &its1));
+  II_CHECK(convert_32_to_64(ctx, (const float2*)x32.v, x, nshift));
+  ii_status st = shifted_refine(ctx, kappa, shifts, nshift, b, x, tol, precision, maxiter, out);
+  if (st != II_OK && st != II_ERR_NOT_CONVERGED) return st;
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (int i = 0; i < nshift; ++i) {
+    out[i].inner_iterations = its1;
+    out[i].iterations += its1;
+    out[i].timing_ms = ms;
+  }
+  return st;
+}
+
 }  // namespace ii
 
 using namespace ii;
@@ -1640,36 +1670,17 @@ ii_status ii_multishift_mixed(ii_context* ctx, double kappa, double* x, const do
   II_CHECK(require_gauge(ctx));
   II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
-  const size_t he = half_elems(g);
   ii_field *fb = nullptr, *fx = nullptr;
   II_CHECK(ii_field_create(ctx, II_EVEN, 1, &fb));
   struct Del { ii_field* f; ~Del() { if (f) ii_field_destroy(f); } } d1{fb};
   II_CHECK(ii_field_create(ctx, II_EVEN, nshift, &fx));
   Del d2{fx};
   II_CHECK(ii_field_upload(fb, 0, b));
-  auto t0 = std::chrono::steady_clock::now();
-  // stage 1: fused FP32 multishift to |r|/|b| < max(tol, 1e-6)
-  SF x32;
-  II_CHECK(fieldP(ctx, 42, II_PREC_SINGLE, nshift, &x32));
-  const double t1 = std::max(tol, 1e-6);
-  int its1 = 0;
-  II_CHECK(multishift_sloppy(ctx, kappa, (float2*)x32.v, fb->data, shifts, nshift, t1 * t1,
-                             maxiter, &its1));
-  II_CHECK(convert_32_to_64(ctx, (const float2*)x32.v, fx->data, nshift));
-  // stage 2: batched refinement of every shift
-  ii_status st = shifted_refine(ctx, kappa, shifts, nshift, fb->data, fx->data, tol, precision,
-                                maxiter, out);
+  ii_status st = multishift_mixed_dev(ctx, kappa, fx->data, fb->data, shifts, nshift, tol,
+                                      precision, maxiter, out);
   if (st != II_OK && st != II_ERR_NOT_CONVERGED) return st;
   std::string msg = ctx->err;
-  const double ms =
-      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  for (int i = 0; i < nshift; ++i) {
-    out[i].inner_iterations = its1;
-    out[i].iterations += its1;
-    out[i].timing_ms = ms;
-    II_CHECK(ii_field_download(fx, i, x + (size_t)24 * g.Vh * i));
-  }
-  (void)he;
+  for (int i = 0; i < nshift; ++i) II_CHECK(ii_field_download(fx, i, x + (size_t)24 * g.Vh * i));
   ctx->err = msg;
   return st;
 }
