@@ -124,6 +124,43 @@ def cpu_baseline(cfg, n_iter_per_rhs, nrhs=12):
     }
 
 
+def rhmc_timings(ii, ctx, cfg):
+    """RationalSolver_CG::solve_inv with a degree-10 approximation of
+    x^(-1/2) on [1e-4, 5] (the spectrum of Dhat^dag Dhat at this kappa lies
+    inside), FP64 reference multishift against the mixed-precision one, and
+    Action_Nf::md_force (Nf = 1, one pseudofermion) on the device."""
+    import numpy as np
+    kappa = cfg["kappa"]
+    D = ii.Dirac_Wilson_EvenOdd(1.0 / (2 * kappa) - 4.0, ctx.download_gauge(), ctx)
+    t0 = time.time()
+    ra = ii.RationalApprox(10, 10, 1, 2, 50, 1e-4, 5.0)
+    remez_s = time.time() - t0
+    b = np.random.default_rng(7).standard_normal(D.fsize())
+    rows = {"remez_s": remez_s, "remez_error": ra.error, "poles": len(ra.InvPoles())}
+    for prec, stop in [("double", 1e-20), ("half", 1e-10)]:
+        slv = ii.RationalSolver_CG(ii.MultiShiftSolver_CG(ii.Fopr_DdagD(D), stop, 20000), ra,
+                                   precision=prec)
+        x = np.empty_like(b)
+        slv.solve_inv(x, b)            # warm-up
+        t0 = time.time()
+        out = slv.solve_inv(x, b)
+        rows["solve_inv_" + prec] = {"s": time.time() - t0, "iterations": out.Iterations,
+                                     "true_residual": out.true_residual}
+        log("rational solve_inv (%s): %.3f s, %d iterations" % (prec, rows["solve_inv_" + prec]["s"],
+                                                               out.Iterations))
+    act = ii.Action_Nf(D, ii.RationalSolver_CG(ii.MultiShiftSolver_CG(ii.Fopr_DdagD(D), 1e-10,
+                                                                       20000), precision="half"),
+                       n_flav=1, n_pseudof=1, degree=[10, 10, 10], precision=[50] * 3,
+                       b_low=[1e-4] * 3, b_high=[5.0] * 3)
+    act.phi_[0][...] = b
+    act.md_force()
+    t0 = time.time()
+    act.md_force()
+    rows["action_nf_md_force_s"] = time.time() - t0
+    log("Action_Nf md_force (10 poles, half multishift): %.3f s" % rows["action_nf_md_force_s"])
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +177,8 @@ def main():
                     help="only time the layers of the learned network")
     ap.add_argument("--force-only", action="store_true",
                     help="only time the MD force (Dirac_Wilson_EvenOdd::md_force)")
+    ap.add_argument("--rhmc-only", action="store_true",
+                    help="only time RationalSolver_CG and the RHMC force (Action_Nf)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training and timing the learned preconditioner")
     ap.add_argument("--train-epochs", type=int, default=8)
@@ -217,6 +256,12 @@ def main():
             % (f_ms, gbs, 100 * gbs / HBM_PEAK_GBS, t_ms))
         print(json.dumps({"force_only": {"kernel_ms": f_ms, "alg_gbs": gbs, "total_ms": t_ms},
                           "lattice": list(dims)}), flush=True)
+        ctx.close()
+        return
+
+    if args.rhmc_only:
+        print(json.dumps({"rhmc_only": rhmc_timings(ii, ctx, cfg), "lattice": list(dims)}),
+              flush=True)
         ctx.close()
         return
 

@@ -19,6 +19,13 @@
 //   Dirac_Wilson_HIP : DiracWilsonLike  (the full-index Dirac_Wilson,
 //                                        dirac_wilson.hpp:117-130, of
 //                                        Qprop_MultiShift)
+//   RationalSolver_CG_HIP : RationalSolver
+//                                       (lib/Solver/rationalSolver.hpp:16-27;
+//                                        replaces RationalSolver_CG,
+//                                        rationalSolver_CG.cpp:12-111)
+//   RationalApprox_HIP                  the coefficients of RationalApprox
+//                                       (rationalapprox.hpp:56-96) from the
+//                                       library's Remez (no MPFR needed)
 //
 // Inverter_WilsonLike (eoUtils.cpp:43-80), Qprop_EvenOdd
 // (qprop_EvenOdd.cpp:11-43), Qprop_MultiShift (qprop_MultiShift.cpp:10-31),
@@ -32,6 +39,7 @@
 #ifndef IROIRO_REF_ADAPTER_HPP
 #define IROIRO_REF_ADAPTER_HPP
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <sstream>
@@ -47,6 +55,7 @@
 #include "Fopr/fopr.h"
 #include "Solver/solver.hpp"
 #include "Solver/multiShiftSolver.hpp"
+#include "Solver/rationalSolver.hpp"
 
 #include "iroiro_hip.h"
 
@@ -287,6 +296,9 @@ class MultiShiftSolver_CG_HIP : public MultiShiftSolver {
  public:
   MultiShiftSolver_CG_HIP(const Fopr_DdagD_HIP* fopr, double stop_cond, int niter)
       : opr_(fopr), prec_(stop_cond), maxiter_(niter) {}
+  const Fopr_DdagD_HIP* fopr() const { return opr_; }
+  double stop_cond() const { return prec_; }
+  int max_iter() const { return maxiter_; }
 
   SolverOutput solve(vector_Field& xq, const Field& b, const vector_double& sigma, double& diff,
                      int& Nconv) const {
@@ -313,6 +325,115 @@ class MultiShiftSolver_CG_HIP : public MultiShiftSolver {
     out.timing = o.timing_ms;
     return out;
   }
+};
+
+// The partial fractions of RationalApprox (rationalapprox.cpp:20-103),
+// x^(a/b) ~ Const + sum Residuals/(x + Poles) and the inverse set, from the
+// library's quad-precision Remez (ii_rational_approx) instead of the MPFR
+// AlgRemez; same parameters (RationalApprox_params) and accessors.
+class RationalApprox_HIP {
+  int num_, den_;
+  std::vector<double> res_, pole_, ires_, ipole_;
+  double a0_, ia0_, err_;
+
+ public:
+  RationalApprox_HIP(int num_deg, int den_deg, int exp_num, int exp_den, double low, double high)
+      : num_(exp_num), den_(exp_den), res_(num_deg), pole_(num_deg), ires_(num_deg),
+        ipole_(num_deg), a0_(0), ia0_(0), err_(0) {
+    ii_status st = ii_rational_approx(0, num_deg, den_deg, exp_num, exp_den, low, high, &res_[0],
+                                      &pole_[0], &a0_, &ires_[0], &ipole_[0], &ia0_, &err_);
+    if (st != II_OK) iroiro_hip_detail::die(0, st, "ii_rational_approx");
+  }
+  std::vector<double> Residuals() const { return res_; }
+  std::vector<double> Poles() const { return pole_; }
+  double Const() const { return a0_; }
+  std::vector<double> InvResiduals() const { return ires_; }
+  std::vector<double> InvPoles() const { return ipole_; }
+  double InvConst() const { return ia0_; }
+  double exponent() const { return (double)num_ / (double)den_; }
+  double error() const { return err_; }
+  // RationalApprox::rescale (rationalapprox.cpp:82-103)
+  void rescale(double /*min*/, double maxIn) {
+    const double mx = 1.05 * maxIn;
+    double f = std::pow(mx, exponent());
+    a0_ *= f;
+    for (size_t i = 0; i < res_.size(); ++i) { res_[i] *= f * mx; pole_[i] *= mx; }
+    f = std::pow(mx, -exponent());
+    ia0_ *= f;
+    for (size_t i = 0; i < ires_.size(); ++i) { ires_[i] *= f * mx; ipole_[i] *= mx; }
+  }
+};
+
+// RationalSolver_CG (rationalSolver_CG.cpp:12-111) on the device: the
+// multishift and the reconstruction sum both run there, so solve/solve_inv
+// move one field each way instead of one per pole.  set_Approx takes the
+// reference's RationalApprox (its accessors) or a RationalApprox_HIP.
+class RationalSolver_CG_HIP : public RationalSolver {
+  const MultiShiftSolver_CG_HIP* MS_Solver_;
+  vector_double Residuals, Poles, InvResiduals, InvPoles;
+  double ConstTerm, InvConstTerm;
+
+  SolverOutput rational(Field& sol, const Field& src, double a0, const vector_double& res,
+                        const vector_double& poles, const char* what) const {
+    if (res.size() == 0) {
+      std::fprintf(stderr, "[RationalSolver] Rational Approximation not initialized yet\n");
+      std::abort();
+    }
+    const iroiro_hip_detail::DeviceOp* D = MS_Solver_->fopr()->dirac();
+    ii_context* ctx = D->context();
+    if (D->full_index()) iroiro_hip_detail::die(ctx, II_ERR_BAD_ARG, "RationalSolver_CG on the full index");
+    sol.resize(src.size());
+    ii_solver_output o;
+    ii_status st = ii_rational_solve(ctx, D->getKappa(), sol.getaddr(0),
+                                     iroiro_hip_detail::addr(src), a0, &res[0], &poles[0],
+                                     (int)poles.size(), MS_Solver_->stop_cond(),
+                                     MS_Solver_->max_iter(), II_PREC_DOUBLE, &o);
+    if (st == II_ERR_NOT_CONVERGED) throw "Not converged.";
+    iroiro_hip_detail::check(ctx, st, what);
+    SolverOutput out;
+    out.Msg = "Rational CG solver";
+    out.Iterations = o.iterations;
+    out.diff = o.true_residual;
+    out.timing = o.timing_ms;
+    return out;
+  }
+
+ public:
+  explicit RationalSolver_CG_HIP(const MultiShiftSolver_CG_HIP* Solv)
+      : MS_Solver_(Solv), ConstTerm(0), InvConstTerm(0) {}
+  RationalSolver_CG_HIP(const MultiShiftSolver_CG_HIP* Solv, const RationalApprox_HIP& RA)
+      : MS_Solver_(Solv), ConstTerm(0), InvConstTerm(0) {
+    set_Approx(RA);
+  }
+  void set_Approx(RationalApprox& RA) {
+    Residuals = RA.Residuals(); Poles = RA.Poles(); ConstTerm = RA.Const();
+    InvResiduals = RA.InvResiduals(); InvPoles = RA.InvPoles(); InvConstTerm = RA.InvConst();
+  }
+  void set_Approx(const RationalApprox_HIP& RA) {
+    Residuals = RA.Residuals(); Poles = RA.Poles(); ConstTerm = RA.Const();
+    InvResiduals = RA.InvResiduals(); InvPoles = RA.InvPoles(); InvConstTerm = RA.InvConst();
+  }
+  SolverOutput solve(Field& sol, const Field& src) const {
+    return rational(sol, src, ConstTerm, Residuals, Poles, "RationalSolver_CG::solve");
+  }
+  SolverOutput solve_inv(Field& sol, const Field& src) const {
+    return rational(sol, src, InvConstTerm, InvResiduals, InvPoles, "RationalSolver_CG::solve_inv");
+  }
+  SolverOutput solve_noReconstruct(vector_Field& shifted_sol, const Field& src) const {
+    shifted_sol.resize(InvPoles.size());
+    for (size_t i = 0; i < shifted_sol.size(); ++i) shifted_sol[i].resize(src.size());
+    double diff = 0;
+    int nconv = 0;
+    return MS_Solver_->solve(shifted_sol, src, InvPoles, diff, nconv);
+  }
+  SolverOutput solve_noReconstruct_inv(vector_Field& shifted_sol, const Field& src) const {
+    shifted_sol.resize(Poles.size());
+    for (size_t i = 0; i < shifted_sol.size(); ++i) shifted_sol[i].resize(src.size());
+    double diff = 0;
+    int nconv = 0;
+    return MS_Solver_->solve(shifted_sol, src, Poles, diff, nconv);
+  }
+  bool check_DdagD() const { return true; }
 };
 
 #endif

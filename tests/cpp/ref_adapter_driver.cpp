@@ -184,8 +184,36 @@ int main(int argc, char** argv) {
   DFg->md_force_m(pmg, zf, src);
   r_fforce = std::max(r_fforce, rel(pmg, pmc));
 
+  // RationalSolver through its base class.  The reference's RationalSolver_CG
+  // needs a RationalApprox, whose Remez needs MPFR (absent here), so its
+  // reconstruction (rationalSolver_CG.cpp:27-37) is written out on the
+  // reference's own multishift with the same coefficients.
+  RationalApprox_HIP RA(10, 10, 1, 2, 1e-3, 4.0);
+  RationalSolver_CG_HIP Rgpu(&Mgpu, RA);
+  const RationalSolver* rs = &Rgpu;
+  Field rsol;
+  rs->solve(rsol, fe);
+  vector_Field rxs(RA.Poles().size(), Field(fe.size()));
+  double rd = 0;
+  int rn = 0;
+  Mcpu.solve(rxs, fe, RA.Poles(), rd, rn);
+  Field rref(fe);
+  rref *= RA.Const();
+  for (size_t i = 0; i < rxs.size(); ++i) {
+    Field t(rxs[i]);
+    t *= RA.Residuals()[i];
+    rref += t;
+  }
+  double r_rat = rel(rsol, rref);
+  vector_Field rg;
+  rs->solve_noReconstruct(rg, fe);
+  vector_Field rc(RA.InvPoles().size(), Field(fe.size()));
+  Mcpu.solve(rc, fe, RA.InvPoles(), rd, rn);
+  for (size_t i = 0; i < rc.size(); ++i) r_rat = std::max(r_rat, rel(rg[i], rc[i]));
+
   std::printf("IIHIP {\"force_rel\": %.3e, \"force_op_rel\": %.3e, "
-              "\"full_force_rel\": %.3e}\n", r_force, r_force_op, r_fforce);
+              "\"full_force_rel\": %.3e, \"rational_rel\": %.3e}\n", r_force, r_force_op,
+              r_fforce, r_rat);
   std::printf("IIHIP {\"ops_rel\": %.3e, \"cg_rel\": %.3e, \"cg_iter_cpu\": %d, "
               "\"cg_iter_gpu\": %d, \"ms_rel\": %.3e, \"ms_nconv_cpu\": %d, "
               "\"ms_nconv_gpu\": %d, \"inv_rel\": %.3e, \"prop_rel\": %.3e, \"ncol\": %d, "

@@ -50,3 +50,6 @@ def test_reference_drivers_on_device_classes(golden, tmp_path):
     assert r["force_op_rel"] < 1e-13
     assert r["force_rel"] < 1e-11
     assert r["full_force_rel"] < 1e-13
+    # RationalSolver_CG_HIP through the RationalSolver base class, against the
+    # reference's multishift and reconstruction
+    assert r["rational_rel"] < 1e-10
