@@ -88,6 +88,40 @@ ii_status ii_gauge_plaquette(ii_context* ctx, double* plaq);
 /* Downloads the device gauge field back into the host layout. */
 ii_status ii_gauge_download(ii_context* ctx, double* u);
 
+/* Gauge configuration files (GaugeGlobal::initialize, lib/Main/gaugeGlobal.cpp:
+ * 133-180; CCIO::ReadFromDisk / SaveOnDisk, lib/IO/fields_io.hpp:47-138,
+ * 250-360).  File order: sites lexicographic (x fastest), 4 directions per
+ * site, 3x3 matrix row by row as (re, im).
+ *   II_IO_BINARY  read: big-endian doubles (BinaryReader); write: host-order
+ *                 doubles, as BinaryWriter does (binary_writer.cpp:19-22)
+ *   II_IO_NERSC   read: 4D_SU3_GAUGE (3x2) or 4D_SU3_GAUGE_3x3, IEEE32(BIG) or
+ *                 IEEE64BIG (NERSC_reader.cpp); write: 3x3 IEEE64BIG with the
+ *                 header of NERSC_writer.cpp:55-76
+ *   II_IO_ILDG    LIME "ildg-binary-data" record, big-endian doubles (or
+ *                 floats when an "ildg-format" record says precision 32);
+ *                 write: "lime-test-text" + data record (ILDG_writer.cpp) */
+typedef enum { II_IO_BINARY = 0, II_IO_NERSC = 1, II_IO_ILDG = 2 } ii_io_format;
+typedef struct {
+  double plaquette;          /* of the field on the device (Staples::plaquette) */
+  double link_trace;         /* Staples::link_trace: <Re tr U>/3 */
+  double header_plaquette;   /* NERSC header PLAQUETTE (read) / written */
+  double header_link_trace;  /* NERSC header LINK_TRACE */
+  unsigned int checksum;     /* NERSC: sum of the data's 32-bit words as stored */
+  unsigned int header_checksum;
+  int bits;                  /* 64 or 32 */
+  int is3x2;                 /* NERSC 4D_SU3_GAUGE: third rows reconstructed */
+} ii_io_info;
+/* Reads a file into the device gauge field (which becomes valid, as after
+ * ii_gauge_upload).  check != 0 applies NERSCReader::check: the plaquette
+ * must match the header's to 1e-15 (64-bit) / 1e-8 (32-bit), else
+ * II_ERR_STATE.  The checksum is reported, not enforced (the reference
+ * prints it only, NERSC_reader.cpp:126-166). */
+ii_status ii_gauge_read(ii_context* ctx, const char* path, int format, int check,
+                        ii_io_info* info);
+/* Writes the device gauge field to a file (SaveOnDisk with the Binary, NERSC
+ * or ILDG writer). */
+ii_status ii_gauge_write(ii_context* ctx, const char* path, int format, ii_io_info* info);
+
 /* Cold start U = 1 on the device (GaugeConf_unit, lib/Main/gaugeConf.cpp:211-228). */
 ii_status ii_gauge_unit(ii_context* ctx);
 /* nsweeps SU(3) Cabibbo-Marinari / Kennedy-Pendleton heat-bath sweeps of

@@ -19,6 +19,7 @@ arithmetic runs on the GPU; importing works without a GPU, but creating a
 Context needs the HIP library and a device.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -156,6 +157,23 @@ class Context:
         self.check(_lib.load().ii_bench_copy(self._h, int(nbytes), int(nrep), ctypes.byref(ms)),
                    "ii_bench_copy")
         return ms.value
+
+    def read_gauge(self, path, fmt="NERSC", check=False):
+        """GaugeGlobal::initialize with Type Binary / NERSC / ILDG
+        (lib/Main/gaugeGlobal.cpp:133-180): the file goes straight into the
+        device gauge field; returns the ii_io_info fields as a dict."""
+        info = _lib.IOInfoC()
+        self.check(_lib.load().ii_gauge_read(self._h, os.fsencode(path), _lib.II_IO[fmt],
+                                             1 if check else 0, ctypes.byref(info)),
+                   "ii_gauge_read")
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def write_gauge(self, path, fmt="NERSC"):
+        """CCIO::SaveOnDisk<Format_G> with the Binary / NERSC / ILDG writer."""
+        info = _lib.IOInfoC()
+        self.check(_lib.load().ii_gauge_write(self._h, os.fsencode(path), _lib.II_IO[fmt],
+                                              ctypes.byref(info)), "ii_gauge_write")
+        return {k: getattr(info, k) for k, _ in info._fields_}
 
     def bench_md_force(self, nrep=20):
         """(force kernel ms, md_force ms incl. its two hops) on device fields."""

@@ -38,6 +38,7 @@ EXPORTS = [
     "ii_dirac_md_force", "ii_dirac_full_md_force", "ii_dirac_full_md_force_pm",
     "ii_traceless_antihermite", "ii_action_nf2_md_force", "ii_bench_md_force",
     "ii_rational_approx", "ii_rational_solve", "ii_action_nf_md_force",
+    "ii_gauge_read", "ii_gauge_write",
 ]
 
 NET_LAYERS = 4
@@ -86,6 +87,15 @@ class TrainReportC(ctypes.Structure):
                 ("steps", ctypes.c_int)]
 
 
+class IOInfoC(ctypes.Structure):
+    _fields_ = [("plaquette", ctypes.c_double), ("link_trace", ctypes.c_double),
+                ("header_plaquette", ctypes.c_double), ("header_link_trace", ctypes.c_double),
+                ("checksum", ctypes.c_uint), ("header_checksum", ctypes.c_uint),
+                ("bits", ctypes.c_int), ("is3x2", ctypes.c_int)]
+
+
+II_IO = {"Binary": 0, "NERSC": 1, "ILDG": 2}
+
 _lib = None
 
 
@@ -132,6 +142,8 @@ def load():
         "ii_rational_approx": [vp, ci, ci, ci, ci, cd, cd, dp, dp, dp, dp, dp, dp, dp],
         "ii_rational_solve": [vp, cd, dp, dp, cd, dp, dp, ci, cd, ci, ci, so],
         "ii_action_nf_md_force": [vp, cd, dp, ci, dp, dp, ci, cd, ci, ci, dp, so],
+        "ii_gauge_read": [vp, ctypes.c_char_p, ci, ci, ctypes.POINTER(IOInfoC)],
+        "ii_gauge_write": [vp, ctypes.c_char_p, ci, ctypes.POINTER(IOInfoC)],
         "ii_solver_cg": [vp, cd, dp, dp, cd, ci, so],
         "ii_multishift_cg": [vp, cd, dp, dp, dp, ci, cd, ci, so],
         "ii_eo_invert": [vp, cd, dp, dp, cd, ci, so],

@@ -16,6 +16,11 @@
 //   Dirac_Wilson(_EvenOdd)::md_force    lib/Dirac_ops/dirac_wilson.cpp:41-92,
 //                                       dirac_wilson_EvenOdd.hpp:94-116
 //   FieldUtils::TracelessAntihermite    lib/Tools/fieldUtils.cpp:264-274
+//   CCIO::BinaryReader / BinaryWriter   lib/IO/binary_reader.cpp, binary_writer.cpp
+//   CCIO::NERSCWriter                   lib/IO/NERSC_writer.cpp
+//   (driven row by row as CCIO::ReadFromDisk / SaveOnDisk do on one
+//   process, lib/IO/fields_io.hpp:47-138, 250-360; that header itself pulls
+//   in the ILDG reader, which needs the LIME library)
 //
 // All binary files are raw little-endian float64 in the reference layouts
 // (Format_F spinors, Format_G links, even/odd half fields in esec/osec order).
@@ -32,6 +37,9 @@
 #include "Solver/solver_CG.hpp"
 #include "Solver/multiShiftSolver_CG.hpp"
 #include "Tools/fieldUtils.hpp"
+#include "IO/binary_reader.hpp"
+#include "IO/binary_writer.hpp"
+#include "IO/NERSC_writer.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -140,6 +148,8 @@ int main(int argc, char** argv) {
       "       iroiro_ref mshift Lx Ly Lz Lt mass gauge src_even prec maxiter outdir s0 s1 ...\n"
       "       iroiro_ref invert Lx Ly Lz Lt mass gauge src_full prec maxiter outdir\n"
       "       iroiro_ref mdforce Lx Ly Lz Lt mass gauge eta_even zeta_even eta_full zeta_full outdir\n"
+      "       iroiro_ref iowrite Lx Ly Lz Lt Binary|NERSC gauge file\n"
+      "       iroiro_ref ioread  Lx Ly Lz Lt Binary file gauge_out\n"
       "       iroiro_ref bench  Lx Ly Lz Lt mass nrep [cg_prec]\n");
     return 2;
   }
@@ -157,6 +167,63 @@ int main(int argc, char** argv) {
     U.data = read_field(argv[6], gsz);
     AntiPeriodicBC<GaugeField>(TDIR).apply_bc(U);
     write_field(argv[7], U.data);
+    return 0;
+  }
+
+  if (cmd == "iowrite" || cmd == "ioread") {
+    // the row loop of SaveOnDisk / ReadFromDisk on one process: rows of Lx
+    // sites in (t, z, y) order, each row one reader/writer call of
+    // Nin*Lx*Nex doubles arranged by format(x, in, ex)
+    const std::string id = argv[6];
+    const int Lx = CommonPrms::Lx(), Ly = CommonPrms::Ly(), Lz = CommonPrms::Lz();
+    const int Lt = CommonPrms::Lt(), Nin = 18, Nex = 4;
+    std::vector<double> row((size_t)Nin * Lx * Nex);
+    if (cmd == "iowrite") {
+      Field u = read_field(argv[7], gsz);
+      CCIO::GeneralWriter* w = 0;
+      if (id == "Binary") w = new CCIO::BinaryWriter(V, Nin, Nex);
+      else if (id == "NERSC") w = new CCIO::NERSCWriter(V, Nin, Nex, u);
+      else die("unknown writer");
+      FILE* fp = std::fopen(argv[8], "w");
+      if (!fp) die("cannot create output");
+      w->set_output(fp);
+      w->header();
+      for (int t = 0; t < Lt; ++t)
+        for (int z = 0; z < Lz; ++z)
+          for (int y = 0; y < Ly; ++y) {
+            for (int x = 0; x < Lx; ++x)
+              for (int ex = 0; ex < Nex; ++ex)
+                for (int in = 0; in < Nin; ++in) {
+                  const int site = x + Lx * (y + Ly * (z + Lz * t));
+                  row[w->format(x, in, ex)] = u[in + Nin * (site + V * ex)];
+                }
+            w->write(&row[0], (unsigned)row.size());
+          }
+      std::fclose(fp);
+      std::printf("IIREF_OK iowrite %s\n", id.c_str());
+      return 0;
+    }
+    if (id != "Binary") die("unknown reader");
+    CCIO::BinaryReader r(V, Nin, Nex);
+    FILE* fp = std::fopen(argv[7], "r");
+    if (!fp) die("cannot open input");
+    r.set_sources(fp);
+    r.header();
+    Field u(gsz);
+    for (int t = 0; t < Lt; ++t)
+      for (int z = 0; z < Lz; ++z)
+        for (int y = 0; y < Ly; ++y) {
+          r.read(&row[0], (unsigned)row.size());
+          for (int x = 0; x < Lx; ++x)
+            for (int ex = 0; ex < Nex; ++ex)
+              for (int in = 0; in < Nin; ++in) {
+                const int site = x + Lx * (y + Ly * (z + Lz * t));
+                u.set(in + Nin * (site + V * ex), row[r.format(x, in, ex)]);
+              }
+        }
+    std::fclose(fp);
+    write_field(argv[8], u);
+    std::printf("IIREF_OK ioread %s\n", id.c_str());
     return 0;
   }
 

@@ -18,8 +18,13 @@ and add Gaussian eta/zeta fields (even half and full); the reference calls
 are Dirac_Wilson_EvenOdd::md_force, FieldUtils::TracelessAntihermite of it,
 and the full-index Dirac_Wilson md_force, md_force_p and md_force_m.
 
-Usage: python tests/golden/make_golden.py [mdforce]
-       (writes tests/golden/*.npz; "mdforce" writes only the MD-force files)
+The gauge-IO fixture (<case>_io.npz) holds the files the reference's own
+BinaryWriter and NERSCWriter write for the gauge field of <case>, and the
+field its BinaryReader decodes from a big-endian file made here (the
+writers and reader are driven row by row as ReadFromDisk / SaveOnDisk do).
+
+Usage: python tests/golden/make_golden.py [mdforce|io]
+       (writes tests/golden/*.npz; "mdforce" / "io" write only those files)
 """
 import json
 import os
@@ -135,15 +140,48 @@ def make_mdforce(base, seed):
     print("wrote", name)
 
 
+IO_CASES = ["wilson_aniso_4x6x2x4"]
+
+
+def make_io(base):
+    z = dict(np.load(os.path.join(HERE, base + ".npz")))
+    dims = tuple(int(v) for v in z["dims"])
+    V = int(np.prod(dims))
+    d = [str(x) for x in dims]
+    u = z["u_apbc"]
+    res = {"base": base}
+    with tempfile.TemporaryDirectory() as tmp:
+        ub = os.path.join(tmp, "u.bin")
+        u.tofile(ub)
+        for wid in ("Binary", "NERSC"):
+            out = os.path.join(tmp, "out." + wid)
+            run(["iowrite"] + d + [wid, ub, out])
+            res["written_" + wid] = np.fromfile(out, dtype=np.uint8)
+        # a big-endian file in the reader's (in, ex, site) order, made here
+        be = u.reshape(4, V, 18).transpose(1, 0, 2).astype(">f8")
+        fin = os.path.join(tmp, "in.be")
+        be.tofile(fin)
+        res["binary_input_be"] = np.fromfile(fin, dtype=np.uint8)
+        dec = os.path.join(tmp, "dec.bin")
+        run(["ioread"] + d + ["Binary", fin, dec])
+        res["binary_read"] = np.fromfile(dec)
+    np.savez_compressed(os.path.join(HERE, base + "_io.npz"), **res)
+    print("wrote", base + "_io")
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("oracle/_ref/iroiro_ref is not built (run oracle/ref_build.sh)")
-    only_mdforce = len(sys.argv) > 1 and sys.argv[1] == "mdforce"
-    if not only_mdforce:
+    only = sys.argv[1] if len(sys.argv) > 1 else None
+    if only is None:
         for name, kw in CASES.items():
             make(name, **kw)
-    for base, seed in MDFORCE_CASES.items():
-        make_mdforce(base, seed)
+    if only in (None, "mdforce"):
+        for base, seed in MDFORCE_CASES.items():
+            make_mdforce(base, seed)
+    if only in (None, "io"):
+        for base in IO_CASES:
+            make_io(base)
 
 
 if __name__ == "__main__":
