@@ -306,6 +306,20 @@ typedef struct {
 ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_params* p,
                               const ii_field* src, ii_field* sol, ii_solver_output* out);
 
+/* ------------------------------------------------------ meson correlators */
+/* MesonCorrelator::calculate / calculate_mom / calculate_PSmom
+ * (lib/Measurements/FermionicM/meson_correlator.hpp:138-296) for all 18
+ * MesonTypes (enum order Pion .. PS_AV4, :20-24) at once, on two device
+ * propagators q1, q2 (II_FULL fields of 12 columns, column c + 3 s; q1 may
+ * equal q2).
+ *   mode 0: calculate along `direction` (0..3): out[18][L_direction]
+ *   mode 1: calculate_mom with momentum mom[3] (time slices):
+ *           out[18][2 Lt], (re, im) interleaved
+ *   mode 2: calculate_PSmom with mom[3]: out[18][Lt]
+ * Sums over sites run in a fixed order (deterministic). */
+ii_status ii_meson_correlators(ii_context* ctx, const ii_field* q1, const ii_field* q2, int mode,
+                               int direction, const double* mom, double* out);
+
 /* ------------------------------------------- learned preconditioner
  * No counterpart in the reference (its solvers are unpreconditioned CG):
  * these calls add a trained, gauge-covariant preconditioner M for the

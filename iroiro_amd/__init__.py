@@ -668,6 +668,58 @@ class Action_Nf:
         return force
 
 
+MESON_TYPES = ["Pion", "Scalar", "Vector1", "Vector2", "Vector3", "Vector4", "AVector1",
+               "AVector2", "AVector3", "AVector4", "Tensor12", "Tensor13", "Tensor14",
+               "Tensor23", "Tensor24", "Tensor34", "AV4_PS", "PS_AV4"]
+
+
+def _prop_field(ctx, q):
+    """A 12-column propagator as a device field: a DeviceField is used as
+    is; a list of 12 host full fields (prop_t) is uploaded."""
+    if isinstance(q, DeviceField):
+        if q.subset != "full" or q.nrhs != 12:
+            raise ValueError("a propagator is a full DeviceField of 12 columns")
+        return q
+    f = DeviceField(ctx, "full", 12)
+    for k in range(12):
+        f.upload(k, q[k])
+    return f
+
+
+def meson_correlators(ctx, q1, q2, mode=0, direction=3, mom=(0.0, 0.0, 0.0)):
+    """All 18 MesonTypes at once (ii_meson_correlators): mode 0 calculate,
+    1 calculate_mom, 2 calculate_PSmom; returns an array [18][...]."""
+    f1 = _prop_field(ctx, q1)
+    f2 = f1 if q2 is q1 else _prop_field(ctx, q2)
+    L = ctx.dims[direction]
+    n = 2 * L if mode == 1 else L
+    out = np.empty(18 * n)
+    m = _lib.as_f64(mom, 3)
+    ctx.check(_lib.load().ii_meson_correlators(ctx.handle, f1.handle, f2.handle, int(mode),
+                                               int(direction), _lib.dptr(m), _lib.dptr(out)),
+              "ii_meson_correlators")
+    return out.reshape(18, n)
+
+
+class MesonCorrelator:
+    """MesonCorrelator (lib/Measurements/FermionicM/meson_correlator.hpp:
+    26-136) on the device: q1, q2 are 12-column propagators (DeviceField or
+    a list of 12 host full fields)."""
+
+    def __init__(self, ctx, mtype="Pion"):
+        self.ctx = ctx
+        self.type = MESON_TYPES.index(mtype) if isinstance(mtype, str) else int(mtype)
+
+    def calculate(self, q1, q2, direction=3):
+        return meson_correlators(self.ctx, q1, q2, 0, direction)[self.type]
+
+    def calculate_mom(self, q1, q2, a, b, c):
+        return meson_correlators(self.ctx, q1, q2, 1, 3, (a, b, c))[self.type]
+
+    def calculate_PSmom(self, q1, q2, a, b, c):
+        return meson_correlators(self.ctx, q1, q2, 2, 3, (a, b, c))[self.type]
+
+
 class MultiShiftSolver_Mixed:
     """Mixed-precision multishift (no reference counterpart): all shifts
     fused in an FP32 shifted recursion, then a batched reliable-update CG

@@ -38,7 +38,7 @@ EXPORTS = [
     "ii_dirac_md_force", "ii_dirac_full_md_force", "ii_dirac_full_md_force_pm",
     "ii_traceless_antihermite", "ii_action_nf2_md_force", "ii_bench_md_force",
     "ii_rational_approx", "ii_rational_solve", "ii_action_nf_md_force",
-    "ii_gauge_read", "ii_gauge_write",
+    "ii_gauge_read", "ii_gauge_write", "ii_meson_correlators",
 ]
 
 NET_LAYERS = 4
@@ -144,6 +144,7 @@ def load():
         "ii_action_nf_md_force": [vp, cd, dp, ci, dp, dp, ci, cd, ci, ci, dp, so],
         "ii_gauge_read": [vp, ctypes.c_char_p, ci, ci, ctypes.POINTER(IOInfoC)],
         "ii_gauge_write": [vp, ctypes.c_char_p, ci, ctypes.POINTER(IOInfoC)],
+        "ii_meson_correlators": [vp, vp, vp, ci, ci, dp, dp],
         "ii_solver_cg": [vp, cd, dp, dp, cd, ci, so],
         "ii_multishift_cg": [vp, cd, dp, dp, dp, ci, cd, ci, so],
         "ii_eo_invert": [vp, cd, dp, dp, cd, ci, so],

@@ -47,6 +47,7 @@ lib/IO/binary_reader.cpp
 lib/IO/binary_writer.cpp
 lib/IO/NERSC_writer.cpp
 lib/Measurements/GaugeM/staples.cpp
+lib/Tools/gammaMatrices.cpp
 "
 OBJS=""
 for s in $SRCS; do

@@ -18,6 +18,8 @@
 //   FieldUtils::TracelessAntihermite    lib/Tools/fieldUtils.cpp:264-274
 //   CCIO::BinaryReader / BinaryWriter   lib/IO/binary_reader.cpp, binary_writer.cpp
 //   CCIO::NERSCWriter                   lib/IO/NERSC_writer.cpp
+//   MesonCorrelator::calculate(_mom, _PSmom)
+//                                       lib/Measurements/FermionicM/meson_correlator.hpp
 //   (driven row by row as CCIO::ReadFromDisk / SaveOnDisk do on one
 //   process, lib/IO/fields_io.hpp:47-138, 250-360; that header itself pulls
 //   in the ILDG reader, which needs the LIME library)
@@ -40,6 +42,7 @@
 #include "IO/binary_reader.hpp"
 #include "IO/binary_writer.hpp"
 #include "IO/NERSC_writer.hpp"
+#include "Measurements/FermionicM/meson_correlator.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -72,6 +75,13 @@ void write_field(const std::string& path, const Field& f) {
   if (!fp) die(("cannot create " + path).c_str());
   Field& g = const_cast<Field&>(f);
   std::fwrite(g.getaddr(0), sizeof(double), f.size(), fp);
+  std::fclose(fp);
+}
+
+void write_vec(const std::string& path, const std::vector<double>& v) {
+  FILE* fp = std::fopen(path.c_str(), "wb");
+  if (!fp) die(("cannot create " + path).c_str());
+  std::fwrite(&v[0], sizeof(double), v.size(), fp);
   std::fclose(fp);
 }
 
@@ -150,6 +160,7 @@ int main(int argc, char** argv) {
       "       iroiro_ref mdforce Lx Ly Lz Lt mass gauge eta_even zeta_even eta_full zeta_full outdir\n"
       "       iroiro_ref iowrite Lx Ly Lz Lt Binary|NERSC gauge file\n"
       "       iroiro_ref ioread  Lx Ly Lz Lt Binary file gauge_out\n"
+      "       iroiro_ref meson   Lx Ly Lz Lt q1 q2 px py pz outdir\n"
       "       iroiro_ref bench  Lx Ly Lz Lt mass nrep [cg_prec]\n");
     return 2;
   }
@@ -224,6 +235,41 @@ int main(int argc, char** argv) {
     std::fclose(fp);
     write_field(argv[8], u);
     std::printf("IIREF_OK ioread %s\n", id.c_str());
+    return 0;
+  }
+
+  if (cmd == "meson") {
+    // q1, q2: 12 full fields each (columns c + 3 s), back to back
+    prop_t q1(12), q2(12);
+    Field a = read_field(argv[6], 12 * fsz), b = read_field(argv[7], 12 * fsz);
+    for (int k = 0; k < 12; ++k) {
+      q1[k] = Field(fsz);
+      q2[k] = Field(fsz);
+      for (size_t i = 0; i < fsz; ++i) {
+        q1[k].set(i, a[k * fsz + i]);
+        q2[k].set(i, b[k * fsz + i]);
+      }
+    }
+    const double px = std::atof(argv[8]), py = std::atof(argv[9]), pz = std::atof(argv[10]);
+    std::string out = argv[11];
+    for (int type = 0; type <= PS_AV4; ++type) {
+      char name[64];
+      for (int dir = 0; dir < 4; ++dir) {
+        MesonCorrelator mc((MesonType)type);
+        std::vector<double> c = mc.calculate<Format::Format_F>(q1, q2, dir);
+        std::snprintf(name, sizeof name, "/meson_%d_dir%d.bin", type, dir);
+        write_vec(out + name, c);
+      }
+      MesonCorrelator mm((MesonType)type);
+      std::vector<double> cm = mm.calculate_mom<Format::Format_F>(q1, q2, px, py, pz);
+      std::snprintf(name, sizeof name, "/meson_%d_mom.bin", type);
+      write_vec(out + name, cm);
+      MesonCorrelator mp((MesonType)type);
+      std::vector<double> cp = mp.calculate_PSmom<Format::Format_F>(q1, q2, px, py, pz);
+      std::snprintf(name, sizeof name, "/meson_%d_psmom.bin", type);
+      write_vec(out + name, cp);
+    }
+    std::printf("IIREF_OK meson\n");
     return 0;
   }
 

@@ -23,8 +23,12 @@ BinaryWriter and NERSCWriter write for the gauge field of <case>, and the
 field its BinaryReader decodes from a big-endian file made here (the
 writers and reader are driven row by row as ReadFromDisk / SaveOnDisk do).
 
-Usage: python tests/golden/make_golden.py [mdforce|io]
-       (writes tests/golden/*.npz; "mdforce" / "io" write only those files)
+The meson fixture (meson_2x4x2x6.npz) holds two Gaussian 12-column
+propagators and the reference's MesonCorrelator::calculate (all four
+directions), calculate_mom and calculate_PSmom for all 18 MesonTypes.
+
+Usage: python tests/golden/make_golden.py [mdforce|io|meson]
+       (writes tests/golden/*.npz; a name writes only those files)
 """
 import json
 import os
@@ -169,6 +173,32 @@ def make_io(base):
     print("wrote", base + "_io")
 
 
+MESON_DIMS = (2, 4, 2, 6)
+MESON_MOM = (0.3, 0.7, 1.1)
+
+
+def make_meson():
+    dims = MESON_DIMS
+    V = int(np.prod(dims))
+    q1 = oracle.gauss_field(12 * 24 * V, 0x7101)
+    q2 = oracle.gauss_field(12 * 24 * V, 0x7102)
+    res = {"dims": np.array(dims), "mom": np.array(MESON_MOM), "q1": q1, "q2": q2}
+    d = [str(x) for x in dims]
+    with tempfile.TemporaryDirectory() as tmp:
+        q1.tofile(os.path.join(tmp, "q1.bin"))
+        q2.tofile(os.path.join(tmp, "q2.bin"))
+        run(["meson"] + d + [os.path.join(tmp, "q1.bin"), os.path.join(tmp, "q2.bin")]
+            + [repr(m) for m in MESON_MOM] + [tmp])
+        res["calc"] = np.stack([np.concatenate([np.fromfile(os.path.join(
+            tmp, "meson_%d_dir%d.bin" % (t, dr))) for dr in range(4)]) for t in range(18)])
+        res["mom_out"] = np.stack([np.fromfile(os.path.join(tmp, "meson_%d_mom.bin" % t))
+                                   for t in range(18)])
+        res["psmom_out"] = np.stack([np.fromfile(os.path.join(tmp, "meson_%d_psmom.bin" % t))
+                                     for t in range(18)])
+    np.savez_compressed(os.path.join(HERE, "meson_2x4x2x6.npz"), **res)
+    print("wrote meson_2x4x2x6")
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("oracle/_ref/iroiro_ref is not built (run oracle/ref_build.sh)")
@@ -182,6 +212,8 @@ def main():
     if only in (None, "io"):
         for base in IO_CASES:
             make_io(base)
+    if only in (None, "meson"):
+        make_meson()
 
 
 if __name__ == "__main__":
