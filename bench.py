@@ -37,6 +37,9 @@ FP64_HOP_BYTES_PER_SITE = 1536  # 192 in + 8*144 links + 192 out
 HOP_FLOPS_PER_SITE = 1320
 # MD force, both parities: 4 links + 4 force matrices (144 B each) + 4 spinors (192 B)
 MD_FORCE_BYTES_PER_SITE = 1920
+# meson contraction of one 12-column FP64 propagator: 12 x 24 doubles read,
+# 18 channel values written and read back by the slice sums
+MESON_BYTES_PER_SITE = 12 * 24 * 8 + 2 * 18 * 8
 
 CONFIGS = {
     "c5": dict(dims=(32, 32, 32, 64), beta=6.0, kappa=0.1355, sweeps=200,
@@ -321,6 +324,19 @@ def main():
                                    [o.Iterations for o in outs], [o.restarts for o in outs],
                                    max_res))
 
+    # ---- meson correlators of the solved propagator (stays on the device)
+    import numpy as np
+    ii.meson_correlators(ctx, sol, sol, 0, 3)
+    t0 = time.time()
+    nrep_m = 5
+    for _ in range(nrep_m):
+        corr = ii.meson_correlators(ctx, sol, sol, 0, 3)
+    meson_ms = (time.time() - t0) * 1000.0 / nrep_m
+    meson = {"ms": meson_ms, "pion": [float(v) for v in corr[0][:8]],
+             "alg_gbs": MESON_BYTES_PER_SITE * ctx.V / (meson_ms * 1e-3) / 1e9}
+    log("meson correlators (18 channels, time slices): %.3f ms, pion C(0..3) = %s"
+        % (meson_ms, np.array2string(corr[0][:4], precision=4)))
+
     # ---- learned preconditioner: trained on this gauge field, flexible GCR
     learned = None
     if not args.no_learned:
@@ -388,6 +404,7 @@ def main():
                                "1920 B/site algorithmic" % ctx.V,
                      "avg_ms": force_ms, "achieved": force_gbs, "unit": "GB/s",
                      "frac": force_gbs / HBM_PEAK_GBS, "with_hops_ms": force_total_ms},
+        "meson": meson,
         "cpu_baseline": cpu,
         "learned": learned,
         "autotune": tune,
