@@ -18,15 +18,18 @@
 
 namespace ii {
 
+// BS threads per block (kBlock for the DOT variants, whose partial sums are
+// indexed by kBlock-site ranges; the autotuner picks it for the others).
 template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Gauge,
-          typename StOut, typename SpX>
-__global__ void __launch_bounds__(kBlock)
+          typename StOut, typename SpX, int BS = kBlock>
+__global__ void __launch_bounds__(BS)
 k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
       typename SpIn::R a, bool tbc_flip, double* __restrict__ partials,
       const int* __restrict__ skip, int skip_stride, HopOrder order) {
+  static_assert(!DOT || BS == kBlock, "DOT partials are per kBlock sites");
   using C = typename SpIn::C;
   const int nblk = gridDim.x;
-  const int base = hop_block_base(g, order, blockIdx.x, nblk, kBlock);
+  const int base = hop_block_base(g, order, blockIdx.x, nblk, BS);
   const int hs = base + threadIdx.x;
   const int rhs = blockIdx.y;
   if (skip && skip[rhs * skip_stride]) return;  // uniform per block
@@ -225,10 +228,24 @@ ii_status launch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, const 
   const Geom& g = ctx->g;
   bool flip = ctx->tbc_sign < 0;
   if (nrhs == 1) {
+    const HopOrder& o = ctx->hop_order;
+    if constexpr (!DOT && std::is_same<Gauge, GaugeF64>::value) {
+      if (o.bs == 128) {
+        dim3 grid(blocks_for(g.Vh, 128), 1);
+        hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX, 128>), grid,
+                           dim3(128), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a, flip,
+                           partials, skip, skip_stride, o);
+        II_HIP(ctx, hipGetLastError());
+        return II_OK;
+      }
+    }
+    HopOrder ok = o;
+    ok.bs = kBlock;
+    if (ok.mode == 1 && o.bs != kBlock) ok = HopOrder();  // strip orders are built for kBlock
     dim3 grid(blocks_for(g.Vh), 1);
     hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX>), grid,
                        dim3(kBlock), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a, flip,
-                       partials, skip, skip_stride, ctx->hop_order);
+                       partials, skip, skip_stride, ok);
   } else {
     constexpr int TS = MrhsShape<Gauge>::TS, MAXT = MrhsShape<Gauge>::MAXT;
     const int rb = std::min(nrhs, MAXT / TS);
@@ -286,6 +303,13 @@ ii_status tune_hop(ii_context* ctx) {
   const Geom& g = ctx->g;
   const int P = g.Lxh * g.L[1];
   std::vector<HopOrder> cands(1);
+  // contiguous order with 128 threads per block (non-DOT FP64 hops; 512
+  // would cap the FP64 stencil's 256 registers and spill)
+  if (g.Vh >= 8 * 128) {
+    HopOrder o;
+    o.bs = 128;
+    cands.push_back(o);
+  }
   const bool tiled = P % kBlock == 0 && g.L[3] % 8 == 0 && g.Vh >= 8 * kBlock;
   if (tiled)
     for (int W : {1, 2, 4, 8, 16})
@@ -340,8 +364,10 @@ ii_status tune_hop(ii_context* ctx) {
       st = set_error(ctx, II_ERR_DEVICE, "autotuner: event timing failed");
     ms /= 10.f;
     char buf[64];
-    std::snprintf(buf, sizeof buf, " %s%d=%.4f", cands[i].mode ? "strip" : "contig",
-                  cands[i].mode ? cands[i].W : 0, ms);
+    if (cands[i].mode)
+      std::snprintf(buf, sizeof buf, " strip%d=%.4f", cands[i].W, ms);
+    else
+      std::snprintf(buf, sizeof buf, " contig/bs%d=%.4f", cands[i].bs, ms);
     rep += buf;
     if (ms < best) {
       best = ms;

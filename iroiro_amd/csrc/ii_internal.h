@@ -45,10 +45,12 @@ struct Geom {
 // slices [x*Lt/8, (x+1)*Lt/8) and walks them in strips of W slices, z-planes
 // outer and t inner, so that the t- and z-neighbour planes of a block were
 // read a few planes earlier into the same L2.  bpp: blocks per (z,t) plane.
+// bs: block size of the contiguous order.
 struct HopOrder {
   int mode = 0;
   int W = 4;
   int bpp = 1;
+  int bs = 256;  // threads per block of the non-DOT FP64 hops (128 or 256)
 };
 
 // Solver scalars kept on the device so that iterations need no host sync.
