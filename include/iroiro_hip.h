@@ -159,6 +159,17 @@ ii_status ii_fopr_ddagd_mult(ii_context* ctx, double kappa, double* out, const d
  * mult_full :115-118) on full lexicographic fields. */
 ii_status ii_dirac_full_mult(ii_context* ctx, double kappa, double* out, const double* in,
                              int dag);
+/* The hop mult_eo / mult_oe (dag = 0) or mult_oe_dag / mult_eo_dag
+ * (dag = 1) in a storage precision (II_PREC_*): the nrhs FP64 half fields
+ * (back to back, Format_F esec/osec order, parity 1 - out_par) are stored
+ * in that precision (16-bit: per-site norm; links FP32 or 16-bit
+ * reconstruct-12), hopped to parity out_par in one launch (the fused
+ * multi-rhs kernel for nrhs > 1) and converted back to FP64.  The
+ * reference is FP64 only; this applies the inner operator of the
+ * mixed-precision solvers so its error against the FP64 operator
+ * (dirac_wilson.cpp:28-36) can be measured. */
+ii_status ii_dirac_mult_eo_prec(ii_context* ctx, double kappa, int precision, int out_par,
+                                int dag, int nrhs, double* out, const double* in);
 
 /* ------------------------------------------------ rational approximation */
 /* RationalApprox (lib/Tools/RationalApprox/rationalapprox.cpp:20-80): the

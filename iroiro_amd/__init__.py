@@ -254,6 +254,22 @@ class Dirac_Wilson_EvenOdd(_OpBase):
         """Deo.mult_dag: odd -> even (:74-77)."""
         return self._call("ii_dirac_mult_oe_dag", _lib.as_f64(f, self.fsize()), self.fsize())
 
+    def mult_eo_prec(self, f, precision, out_par=0, dag=False):
+        """The hop to parity out_par (mult_eo / mult_oe, or mult_oe_dag /
+        mult_eo_dag with dag) in a storage precision ("single", "half",
+        "bf16", "double"), as the mixed-precision solvers apply it.  f holds
+        nrhs half fields back to back; nrhs > 1 runs the fused multi-rhs
+        kernel.  Returns FP64."""
+        n = self.fsize()
+        f = np.ascontiguousarray(f, dtype=np.float64).ravel()
+        if f.size % n:
+            raise ValueError("mult_eo_prec: size %d is not a multiple of %d" % (f.size, n))
+        out = np.empty_like(f)
+        self.ctx.check(_lib.load().ii_dirac_mult_eo_prec(
+            self.ctx.handle, self.kpp, PRECISIONS[precision], int(out_par), int(bool(dag)),
+            f.size // n, _lib.dptr(out), _lib.dptr(f)), "ii_dirac_mult_eo_prec")
+        return out
+
     def mult_ee(self, f):
         return np.array(f, dtype=np.float64)
 

@@ -76,20 +76,20 @@ ii_status require_gauge(ii_context* ctx) {
 
 // Host layout staging: copies n doubles from a host array into the staging
 // workspace (slot 0).
-static ii_status stage_in(ii_context* ctx, const double* host, size_t n, double** dev) {
+ii_status stage_in(ii_context* ctx, const double* host, size_t n, double** dev) {
   void* p = nullptr;
   II_CHECK(workspace(ctx, 0, n * sizeof(double), &p));
   II_HIP(ctx, hipMemcpyAsync(p, host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   *dev = (double*)p;
   return II_OK;
 }
-static ii_status stage_out(ii_context* ctx, double* host, const double* dev, size_t n) {
+ii_status stage_out(ii_context* ctx, double* host, const double* dev, size_t n) {
   II_HIP(ctx, hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   II_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return II_OK;
 }
 // A second staging area for outputs (slot 1).
-static ii_status stage_buf(ii_context* ctx, size_t n, double** dev) {
+ii_status stage_buf(ii_context* ctx, size_t n, double** dev) {
   void* p = nullptr;
   II_CHECK(workspace(ctx, 1, n * sizeof(double), &p));
   *dev = (double*)p;

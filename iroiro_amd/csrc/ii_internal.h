@@ -98,6 +98,7 @@ struct ii_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // autotuned launch shapes
   ii::HopOrder hop_order;
+  ii::HopOrder mrhs_order;  // tile order of the multi-rhs hop (64-site tiles)
   bool hop_tuned = false;
   std::string tune_report;
   // learned preconditioner state (owned by ii_learned.hip)

@@ -70,6 +70,11 @@ ii_status axpby64(ii_context* ctx, double a, const double2* x, double b, double2
 ii_status gamma5_64(ii_context* ctx, const double2* in, double2* out, int nrhs = 1);
 
 // ------------------------------------------------------------- packing
+// Host-array staging (ii_api.hip): n doubles in through workspace slot 0,
+// an output buffer in slot 1, and the synchronous copy back.
+ii_status stage_in(ii_context* ctx, const double* host, size_t n, double** dev);
+ii_status stage_buf(ii_context* ctx, size_t n, double** dev);
+ii_status stage_out(ii_context* ctx, double* host, const double* dev, size_t n);
 ii_status pack_half_spinor(ii_context* ctx, const double* host_dev, double2* dst);
 ii_status unpack_half_spinor(ii_context* ctx, const double2* src, double* host_dev);
 ii_status pack_full_spinor(ii_context* ctx, const double* host_dev, double2* even, double2* odd);

@@ -1861,6 +1861,40 @@ ii_status ii_bench_copy(ii_context* ctx, size_t bytes, int nrep, double* avg_ms)
   return II_OK;
 }
 
+ii_status ii_dirac_mult_eo_prec(ii_context* ctx, double kappa, int precision, int out_par,
+                                int dag, int nrhs, double* out, const double* in) {
+  if (!out || !in || nrhs < 1 || out_par < 0 || out_par > 1) return II_ERR_BAD_ARG;
+  const int P = precision;
+  if (P < 0 || P > 3) return set_error(ctx, II_ERR_BAD_ARG, "bad precision");
+  II_CHECK(require_gauge(ctx));
+  II_CHECK(enter(ctx));
+  if (P != II_PREC_DOUBLE && !ctx->gauge_lp[P]) II_CHECK(build_lowprec_gauge(ctx, P));
+  const Geom& g = ctx->g;
+  const size_t n = (size_t)24 * g.Vh, he = half_elems(g);
+  double *din = nullptr, *dout = nullptr;
+  II_CHECK(stage_in(ctx, in, n * nrhs, &din));
+  II_CHECK(stage_buf(ctx, n * nrhs, &dout));
+  SF a64, b64, a, b;
+  II_CHECK(fieldP(ctx, 56, II_PREC_DOUBLE, nrhs, &a64));
+  II_CHECK(fieldP(ctx, 57, II_PREC_DOUBLE, nrhs, &b64));
+  II_CHECK(fieldP(ctx, 58, P, nrhs, &a));
+  II_CHECK(fieldP(ctx, 59, P, nrhs, &b));
+  for (int k = 0; k < nrhs; ++k)
+    II_CHECK(pack_half_spinor(ctx, din + n * k, (double2*)a64.v + he * k));
+  II_CHECK(convertP(ctx, II_PREC_DOUBLE, P, a64, a, nrhs));
+  HopSpec h;
+  h.out_par = out_par;
+  h.a = -kappa;
+  h.g5in = dag != 0;
+  h.g5out = dag != 0;
+  h.nrhs = nrhs;
+  II_CHECK(hopP(ctx, P, h, a, b, nullptr, nullptr));
+  II_CHECK(convertP(ctx, P, II_PREC_DOUBLE, b, b64, nrhs));
+  for (int k = 0; k < nrhs; ++k)
+    II_CHECK(unpack_half_spinor(ctx, (const double2*)b64.v + he * k, dout + n * k));
+  return stage_out(ctx, out, dout, n * nrhs);
+}
+
 ii_status ii_bench_hop(ii_context* ctx, int precision, int nrhs, int nrep, double* avg_ms) {
   if (!avg_ms || nrep < 1 || nrhs < 1) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));

@@ -461,6 +461,104 @@ __device__ __forceinline__ void hop_dir_with(typename SpIn::C acc[4][3], const S
   else reconstruct<MU, FWD>(acc, h1, h2);
 }
 
+// ---------------------------------------------------------------- FP16 dot2
+// The FP16 multi-rhs hop does its colour products with v_dot2_f32_f16 (two
+// FP16 products summed into an FP32 accumulator): one complex multiply-add
+// is two instructions instead of four FP32 FMAs.  The operands stay in
+// FP16: the projected vectors a + c b are formed with packed FP16 adds from
+// the stored (norm-scaled) components, |a + c b| <= 2, and the links are
+// staged in LDS as FP16.
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ h2v as_h2v(__half2 v) { return __builtin_bit_cast(h2v, v); }
+
+// One entry M_ab = mx + i my of the colour matrix a direction applies (U for
+// a forward hop, U^dag for a backward one), staged so that for w = (wx, wy)
+// Re(M_ab w) = dot2(re, w) and Im(M_ab w) = dot2(im, w):
+// re = (mx, -my), im = (my, mx).
+struct MDot2 {
+  h2v re, im;
+};
+__device__ __forceinline__ MDot2 mdot2(float mx, float my) {
+  MDot2 m;
+  m.re = h2v{(_Float16)mx, (_Float16)(-my)};
+  m.im = h2v{(_Float16)my, (_Float16)mx};
+  return m;
+}
+
+// dot2(a, b) + 0 as the three-operand v_dot2_f32_f16 with an inline zero:
+// left to itself the compiler starts every chain with a v_mov of 0 and the
+// accumulating two-operand form (96 extra moves per site and rhs).
+__device__ __forceinline__ float dot2_first(h2v a, h2v b) {
+  float r;
+  asm("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// a + c b for the projector codes of combine_u, in packed FP16 (each form
+// is one v_pk_add_f16 with operand-select and negate modifiers)
+template <int CODE>
+__device__ __forceinline__ h2v combine_h(h2v a, h2v b) {
+  if (CODE == 0) return a + b;
+  else if (CODE == 1) return a - b;
+  else if (CODE == 2) return a + h2v{-b.y, b.x};
+  else if (CODE == 3) return a + h2v{b.y, -b.x};
+  else return a;
+}
+
+// One direction of the FP16 stencil, split into its loads and its
+// arithmetic so that the caller can issue the next direction's loads before
+// this one's products (software pipelining).  getM(k) returns the staged
+// entry k = 3a + b of M; projection / reconstruction tables as in
+// hop_dir_with.
+struct RawDir {
+  __half2 v[12];  // a1, b1, a2, b2 components (3 colours each); b unused for t
+  float n;        // site norm
+};
+template <int MU, bool FWD, bool G5IN, typename SpIn>
+__device__ __forceinline__ void load_dir_raw(const SpIn& in, int nb, RawDir& r) {
+  using P = Proj<MU, FWD>;
+  constexpr int a1 = G5IN ? (P::A1 ^ 2) : P::A1, b1 = G5IN ? (P::B1 ^ 2) : P::B1;
+  constexpr int a2 = G5IN ? (P::A2 ^ 2) : P::A2, b2 = G5IN ? (P::B2 ^ 2) : P::B2;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    r.v[c] = in.raw(nb, 3 * a1 + c);
+    r.v[6 + c] = in.raw(nb, 3 * a2 + c);
+    if (P::C1 != 4) r.v[3 + c] = in.raw(nb, 3 * b1 + c);
+    if (P::C2 != 4) r.v[9 + c] = in.raw(nb, 3 * b2 + c);
+  }
+  r.n = in.scale(nb);
+}
+template <int MU, bool FWD, typename GetM>
+__device__ __forceinline__ void hop_dir_dot2_raw(float2 acc[4][3], const RawDir& r, GetM&& getM) {
+  using P = Proj<MU, FWD>;
+  h2v w1[3], w2[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    w1[c] = combine_h<P::C1>(as_h2v(r.v[c]), P::C1 == 4 ? h2v{} : as_h2v(r.v[3 + c]));
+    w2[c] = combine_h<P::C2>(as_h2v(r.v[6 + c]), P::C2 == 4 ? h2v{} : as_h2v(r.v[9 + c]));
+  }
+  const float sc = P::C1 == 4 ? r.n + r.n : r.n;
+  float2 h1[3], h2[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const MDot2 m0 = getM(3 * a);
+    float r1 = dot2_first(m0.re, w1[0]), i1 = dot2_first(m0.im, w1[0]);
+    float r2 = dot2_first(m0.re, w2[0]), i2 = dot2_first(m0.im, w2[0]);
+#pragma unroll
+    for (int b = 1; b < 3; ++b) {
+      const MDot2 m = getM(3 * a + b);
+      r1 = __builtin_amdgcn_fdot2(m.re, w1[b], r1, false);
+      i1 = __builtin_amdgcn_fdot2(m.im, w1[b], i1, false);
+      r2 = __builtin_amdgcn_fdot2(m.re, w2[b], r2, false);
+      i2 = __builtin_amdgcn_fdot2(m.im, w2[b], i2, false);
+    }
+    h1[a] = make_float2(r1, i1);
+    h2[a] = make_float2(r2, i2);
+  }
+  reconstruct_scaled<MU, FWD>(acc, h1, h2, sc);
+}
+
 template <int MU, bool FWD, bool G5IN, typename SpIn, typename Gauge>
 __device__ __forceinline__ void hop_dir(typename SpIn::C acc[4][3], const SpIn& in,
                                         const Gauge& gauge, int nb, int lpar, int lsite,
