@@ -46,10 +46,17 @@ constexpr int kScalInts = sizeof(RUScal) / sizeof(int);
 
 __device__ __forceinline__ bool finite(double v) { return isfinite(v); }
 
-// pap -> alpha = rr/pap
-__global__ void k_fin_pap(const double* __restrict__ partials, int nb, RUScal* __restrict__ sc) {
+// pap -> alpha = rr/pap; with ahist, alpha is also kept at
+// ahist[rhs * astride + aidx] (0 for a finished rhs) for the deferred
+// solution update
+__global__ void k_fin_pap(const double* __restrict__ partials, int nb, RUScal* __restrict__ sc,
+                          double* __restrict__ ahist, int aidx, int astride) {
   RUScal& S = sc[blockIdx.x];
-  if (S.done) return;
+  double* ah = ahist ? ahist + (size_t)blockIdx.x * astride + aidx : nullptr;
+  if (S.done) {
+    if (ah && threadIdx.x == 0) *ah = 0.0;
+    return;
+  }
   const double* p = partials + (size_t)blockIdx.x * nb;
   double s = 0.0;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
@@ -58,6 +65,7 @@ __global__ void k_fin_pap(const double* __restrict__ partials, int nb, RUScal* _
     S.pap = s;
     S.alpha = S.rr / s;
     if (!finite(S.alpha)) { S.nan = 1; S.done = 1; S.skip_ru = 1; }
+    if (ah) *ah = S.done ? 0.0 : S.alpha;
   }
 }
 
@@ -189,6 +197,67 @@ k_cg_p(Geom g, SpR r, SpP pr, StP pw, const RUScal* __restrict__ sc) {
 #pragma unroll
   for (int k = 0; k < 24; ++k) pv[k] = fma(pv[k], b, rv[k]);
   st_site(pw.rhs(rhs), hs, pv);
+}
+
+// r -= alpha s ; partial |r|^2: the CG step with the solution update
+// deferred (k_cg_flush adds the search directions to y later)
+template <typename SpS, typename SpR, typename StR>
+__global__ void __launch_bounds__(kBlock)
+k_cg_r(Geom g, SpS s, SpR rr_, StR rw, RUScal* __restrict__ sc, double* __restrict__ partials) {
+  const int rhs = blockIdx.y;
+  const RUScal& S = sc[rhs];
+  if (S.done) return;
+  const double a = S.alpha;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (hs < g.Vh) {
+    double sv[24], rv[24];
+    ld_site(s.rhs(rhs), hs, sv);
+    ld_site(rr_.rhs(rhs), hs, rv);
+#pragma unroll
+    for (int k = 0; k < 24; ++k) {
+      rv[k] = fma(-a, sv[k], rv[k]);
+      acc = fma(rv[k], rv[k], acc);
+    }
+    st_site(rw.rhs(rhs), hs, rv);
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = acc;
+}
+
+// Deferred solution update y += sum_j alpha_j p_j over up to kDefer stored
+// search directions: alpha_j = ahist[rhs * astride + slot[j]] (0 for the
+// iterations after an rhs finished, whose directions are not read).  One
+// pass over y for kDefer iterations instead of a y read and write in each.
+constexpr int kDefer = 8;
+template <typename SpP>
+struct DeferredDirs {
+  SpP p[kDefer];
+  int slot[kDefer];
+  int n;
+};
+template <typename SpP, typename SpY, typename StY>
+__global__ void __launch_bounds__(kBlock)
+k_cg_flush(Geom g, DeferredDirs<SpP> d, const double* __restrict__ ahist, int astride, SpY yr,
+           StY yw) {
+  const int rhs = blockIdx.y;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (hs >= g.Vh) return;
+  double yv[24];
+  ld_site(yr.rhs(rhs), hs, yv);
+#pragma unroll
+  for (int j = 0; j < kDefer; ++j) {
+    if (j < d.n) {
+      const double a = ahist[(size_t)rhs * astride + d.slot[j]];
+      if (a != 0.0) {
+        double pv[24];
+        ld_site(d.p[j].rhs(rhs), hs, pv);
+#pragma unroll
+        for (int k = 0; k < 24; ++k) yv[k] = fma(a, pv[k], yv[k]);
+      }
+    }
+  }
+  st_site(yw.rhs(rhs), hs, yv);
 }
 
 // dst = src (precision conversion), masked by skip flags
@@ -412,13 +481,45 @@ ii_status cg_step_blas(ii_context* ctx, const SF& p, const SF& s, const SF& r, c
   return II_OK;
 }
 
+// p_out = r + beta p_in (p_out may be p_in)
 template <int P>
-ii_status cg_p_update(ii_context* ctx, const SF& r, const SF& p, RUScal* sc, int nrhs) {
+ii_status cg_p_update(ii_context* ctx, const SF& r, const SF& p, const SF& pout, RUScal* sc,
+                      int nrhs) {
   const Geom& g = ctx->g;
   using A = Acc<P>;
   hipLaunchKernelGGL((k_cg_p<typename A::Sp, typename A::Sp, typename A::St>),
                      dim3(blocks_for(g.Vh), nrhs), dim3(kBlock), 0, ctx->stream, g,
-                     A::sp(r, g.stride), A::sp(p, g.stride), A::st(p, g.stride), sc);
+                     A::sp(r, g.stride), A::sp(p, g.stride), A::st(pout, g.stride), sc);
+  II_HIP(ctx, hipGetLastError());
+  return II_OK;
+}
+template <int P>
+ii_status cg_r_update(ii_context* ctx, const SF& s, const SF& r, RUScal* sc, int nrhs,
+                      double* partials, int nb) {
+  const Geom& g = ctx->g;
+  using A = Acc<P>;
+  hipLaunchKernelGGL((k_cg_r<typename A::Sp, typename A::Sp, typename A::St>), dim3(nb, nrhs),
+                     dim3(kBlock), 0, ctx->stream, g, A::sp(s, g.stride), A::sp(r, g.stride),
+                     A::st(r, g.stride), sc, partials);
+  II_HIP(ctx, hipGetLastError());
+  return II_OK;
+}
+// y (FP32) += the deferred directions ring[slots[j]], j < n <= kDefer
+template <int P>
+ii_status cg_flush(ii_context* ctx, const SF* ring, const int* slots, int n, const double* ahist,
+                   int astride, const SF& y, int nrhs) {
+  const Geom& g = ctx->g;
+  using A = Acc<P>;
+  DeferredDirs<typename A::Sp> d{};
+  d.n = n;
+  for (int j = 0; j < n; ++j) {
+    d.p[j] = A::sp(ring[slots[j]], g.stride);
+    d.slot[j] = slots[j];
+  }
+  for (int j = n; j < kDefer; ++j) d.p[j] = d.p[0];
+  hipLaunchKernelGGL((k_cg_flush<typename A::Sp, SpF32, StF32>), dim3(blocks_for(g.Vh), nrhs),
+                     dim3(kBlock), 0, ctx->stream, g, d, ahist, astride,
+                     Acc<II_PREC_SINGLE>::sp(y, g.stride), Acc<II_PREC_SINGLE>::st(y, g.stride));
   II_HIP(ctx, hipGetLastError());
   return II_OK;
 }
@@ -473,12 +574,31 @@ ii_status cg_blas_P(ii_context* ctx, int P, const SF& p, const SF& s, const SF& 
     default: return cg_step_blas<3>(ctx, p, s, r, y, ytype, sc, nrhs, partials, nb);
   }
 }
-ii_status cg_p_P(ii_context* ctx, int P, const SF& r, const SF& p, RUScal* sc, int nrhs) {
+ii_status cg_p_P(ii_context* ctx, int P, const SF& r, const SF& p, const SF& pout, RUScal* sc,
+                 int nrhs) {
   switch (P) {
-    case 0: return cg_p_update<0>(ctx, r, p, sc, nrhs);
-    case 1: return cg_p_update<1>(ctx, r, p, sc, nrhs);
-    case 2: return cg_p_update<2>(ctx, r, p, sc, nrhs);
-    default: return cg_p_update<3>(ctx, r, p, sc, nrhs);
+    case 0: return cg_p_update<0>(ctx, r, p, pout, sc, nrhs);
+    case 1: return cg_p_update<1>(ctx, r, p, pout, sc, nrhs);
+    case 2: return cg_p_update<2>(ctx, r, p, pout, sc, nrhs);
+    default: return cg_p_update<3>(ctx, r, p, pout, sc, nrhs);
+  }
+}
+ii_status cg_r_P(ii_context* ctx, int P, const SF& s, const SF& r, RUScal* sc, int nrhs,
+                 double* partials, int nb) {
+  switch (P) {
+    case 1: return cg_r_update<1>(ctx, s, r, sc, nrhs, partials, nb);
+    case 2: return cg_r_update<2>(ctx, s, r, sc, nrhs, partials, nb);
+    case 3: return cg_r_update<3>(ctx, s, r, sc, nrhs, partials, nb);
+    default: return set_error(ctx, II_ERR_BAD_ARG, "deferred update needs a sloppy precision");
+  }
+}
+ii_status cg_flush_P(ii_context* ctx, int P, const SF* ring, const int* slots, int n,
+                     const double* ahist, int astride, const SF& y, int nrhs) {
+  switch (P) {
+    case 1: return cg_flush<1>(ctx, ring, slots, n, ahist, astride, y, nrhs);
+    case 2: return cg_flush<2>(ctx, ring, slots, n, ahist, astride, y, nrhs);
+    case 3: return cg_flush<3>(ctx, ring, slots, n, ahist, astride, y, nrhs);
+    default: return set_error(ctx, II_ERR_BAD_ARG, "deferred update needs a sloppy precision");
   }
 }
 
@@ -582,10 +702,11 @@ ii_status cg_reference(ii_context* ctx, double kappa, double2* x, const double2*
     int n = std::min(chunk, maxiter - queued);
     for (int i = 0; i < n; ++i) {
       II_CHECK(ddagdP(ctx, 0, kappa, fp, fs, fq, ft, &fp, part, 1, skip, ss));
-      hipLaunchKernelGGL(k_fin_pap, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, sc);
+      hipLaunchKernelGGL(k_fin_pap, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, sc, nullptr,
+                         0, 0);
       II_CHECK(cg_blas_P(ctx, 0, fp, fs, fr, fx, 0, sc, 1, part, nb));
       hipLaunchKernelGGL(k_fin_rr, dim3(1), dim3(kBlock), 0, ctx->stream, part, nb, sc, 0);
-      II_CHECK(cg_p_P(ctx, 0, fr, fp, sc, 1));
+      II_CHECK(cg_p_P(ctx, 0, fr, fp, fp, sc, 1));
     }
     queued += n;
     II_CHECK(read_scalars(ctx, 1));
@@ -1204,10 +1325,11 @@ ii_status shifted_refine(ii_context* ctx, double kappa, const double* shifts, in
     for (int i = 0; i < chunk; ++i) {
       II_CHECK(ddagdP(ctx, P, kappa, ps, ss_, qs, ts, nullptr, nullptr, nrhs, skip_it, ss));
       II_CHECK(sp_shift_P(ctx, P, ps, ss_, sig, sc, nrhs, part));
-      hipLaunchKernelGGL(k_fin_pap, dim3(nrhs), dim3(kBlock), 0, ctx->stream, part, nb, sc);
+      hipLaunchKernelGGL(k_fin_pap, dim3(nrhs), dim3(kBlock), 0, ctx->stream, part, nb, sc,
+                         nullptr, 0, 0);
       II_CHECK(cg_blas_P(ctx, P, ps, ss_, rs, ys, ytype, sc, nrhs, part, nb));
       hipLaunchKernelGGL(k_fin_rr, dim3(nrhs), dim3(kBlock), 0, ctx->stream, part, nb, sc, 1);
-      II_CHECK(cg_p_P(ctx, P, rs, ps, sc, nrhs));
+      II_CHECK(cg_p_P(ctx, P, rs, ps, ps, sc, nrhs));
     }
     its += chunk;
     II_CHECK(read_scalars(ctx, nrhs));
@@ -1325,6 +1447,20 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
     ts = t64;
     ys = y;
   }
+  // Deferred solution update (sloppy precisions): the search directions of
+  // up to kDefer iterations stay in a ring of kDefer + 1 fields, their
+  // alphas in ahist[rhs][slot], and k_cg_flush adds them to ys in one pass.
+  const bool defer = lo;
+  const int nring = defer ? kDefer + 1 : 1;
+  std::vector<SF> ring(nring);
+  ring[0] = ps;
+  for (int i = 1; i < nring; ++i) II_CHECK(fieldP(ctx, 63 + i, P, batch, &ring[i]));
+  double* ahist = nullptr;
+  if (defer) {
+    void* v = nullptr;
+    II_CHECK(workspace(ctx, 62, sizeof(double) * nring * batch, &v));
+    ahist = (double*)v;
+  }
   HopSpec h;
   for (int r0 = 0; r0 < nrhs_all; r0 += batch) {
     const int nr = std::min(batch, nrhs_all - r0);
@@ -1376,38 +1512,63 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
     int its = 0;
     bool all_done = false;
     int chunk = 1;
-    // one CG iteration: 4 hops (the last with <p, A p>), scalars, the fused
-    // x/r update with |r|^2, scalars, the p update.  Every argument is fixed
-    // for the batch and every kernel reads its rhs's done flag on the device,
-    // so the sequence can be recorded once into a HIP graph and replayed.
-    auto iteration = [&]() -> ii_status {
-      II_CHECK(ddagdP(ctx, P, kappa, ps, ss_, qs, ts, &ps, part, nr, skip_it, ss));
+    int pos = 0;             // ring slot of the current search direction
+    std::vector<int> pend;   // ring slots whose alpha p is not yet in ys
+    // one CG iteration on the direction in ring[cur]: 4 hops (the last with
+    // <p, A p>), scalars, the r update with |r|^2 (with the y update unless
+    // deferred), scalars, the p update into the next ring slot.  Every
+    // argument is fixed for a ring slot and every kernel reads its rhs's done
+    // flag on the device, so the sequence for each slot can be recorded once
+    // into a HIP graph and replayed.
+    auto iteration = [&](int cur) -> ii_status {
+      const SF& p = ring[cur];
+      II_CHECK(ddagdP(ctx, P, kappa, p, ss_, qs, ts, &p, part, nr, skip_it, ss));
       hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
-                         hop_dot_parts(g, nr, P), sc);
-      II_CHECK(cg_blas_P(ctx, P, ps, ss_, rs, ys, ytype, sc, nr, part, nb));
+                         hop_dot_parts(g, nr, P), sc, ahist, cur, nring);
+      if (defer) II_CHECK(cg_r_P(ctx, P, ss_, rs, sc, nr, part, nb));
+      else II_CHECK(cg_blas_P(ctx, P, p, ss_, rs, ys, ytype, sc, nr, part, nb));
       hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc, 1);
-      return cg_p_P(ctx, P, rs, ps, sc, nr);
+      return cg_p_P(ctx, P, rs, p, ring[(cur + 1) % nring], sc, nr);
     };
-    hipGraphExec_t gexec = nullptr;
+    auto flush = [&]() -> ii_status {
+      if (pend.empty()) return II_OK;
+      II_CHECK(cg_flush_P(ctx, P, ring.data(), pend.data(), (int)pend.size(), ahist, nring, ys,
+                          nr));
+      pend.clear();
+      return II_OK;
+    };
+    std::vector<hipGraphExec_t> gexec(nring, nullptr);
     struct GraphGuard {
-      hipGraphExec_t* e;
-      ~GraphGuard() { if (*e) (void)hipGraphExecDestroy(*e); }
+      std::vector<hipGraphExec_t>* e;
+      ~GraphGuard() {
+        for (hipGraphExec_t x : *e)
+          if (x) (void)hipGraphExecDestroy(x);
+      }
     } gguard{&gexec};
-    if (prm->use_graph) {
-      hipGraph_t graph = nullptr;
-      II_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      ii_status cst = iteration();
-      hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
-      II_CHECK(cst);
-      if (ee != hipSuccess) return set_error(ctx, II_ERR_DEVICE, "graph capture failed");
-      hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ie != hipSuccess) return set_error(ctx, II_ERR_DEVICE, "graph instantiation failed");
-    }
+    auto launch = [&](int cur) -> ii_status {
+      if (!prm->use_graph) return iteration(cur);
+      if (!gexec[cur]) {
+        hipGraph_t graph = nullptr;
+        II_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        ii_status cst = iteration(cur);
+        hipError_t ee = hipStreamEndCapture(ctx->stream, &graph);
+        II_CHECK(cst);
+        if (ee != hipSuccess) return set_error(ctx, II_ERR_DEVICE, "graph capture failed");
+        hipError_t ie = hipGraphInstantiate(&gexec[cur], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return set_error(ctx, II_ERR_DEVICE, "graph instantiation failed");
+      }
+      II_HIP(ctx, hipGraphLaunch(gexec[cur], ctx->stream));
+      return II_OK;
+    };
     while (its < maxiter && !all_done) {
       for (int i = 0; i < chunk; ++i) {
-        if (gexec) II_HIP(ctx, hipGraphLaunch(gexec, ctx->stream));
-        else II_CHECK(iteration());
+        II_CHECK(launch(pos));
+        if (defer) {
+          pend.push_back(pos);
+          pos = (pos + 1) % nring;
+          if ((int)pend.size() == kDefer) II_CHECK(flush());
+        }
       }
       its += chunk;
       II_CHECK(read_scalars(ctx, nr));
@@ -1428,6 +1589,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       if (all_done) break;
       if (any_upd) {
         // reliable update, FP64, for the flagged rhs only
+        II_CHECK(flush());
         if (lo) {
           const Geom& gg = g;
           hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(gg.Vh), nr), dim3(kBlock), 0,
@@ -1486,14 +1648,15 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
         hipLaunchKernelGGL(k_fin_ru, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc);
         II_HIP(ctx, hipGetLastError());
         // restart the search direction where the recursion broke (keep_p == 0)
-        if (lo) II_CHECK(convertP(ctx, P, P, rs, ps, nr, &sc[0].keep_p, ss));
-        else II_CHECK(convertP(ctx, 0, 0, r64, ps, nr, &sc[0].keep_p, ss));
+        if (lo) II_CHECK(convertP(ctx, P, P, rs, ring[pos], nr, &sc[0].keep_p, ss));
+        else II_CHECK(convertP(ctx, 0, 0, r64, ring[pos], nr, &sc[0].keep_p, ss));
         chunk = 1;
       } else {
         chunk = std::min(chunk * 2, 8);
       }
     }
     // fold the sloppy accumulator in (rhs not yet folded at convergence)
+    II_CHECK(flush());
     if (lo) {
       hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(g.Vh), nr), dim3(kBlock), 0,
                          ctx->stream, g, Acc<1>::sp(ys, g.stride), Acc<1>::st(ys, g.stride),
