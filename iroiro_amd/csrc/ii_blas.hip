@@ -465,7 +465,7 @@ ii_status dhat_dag64(ii_context* ctx, double kappa, const double2* f, double2* o
   h2.xpay = true;
   h2.g5out = true;
   h2.nrhs = nrhs;
-  h2.dot = dv != nullptr;
+  h2.epi = dv != nullptr ? kEpiDot : kEpiStore;
   h2.partials = partials;
   return hop64(ctx, h2, t, out, f, dv);
 }

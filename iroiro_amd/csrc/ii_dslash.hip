@@ -2,8 +2,9 @@
 //
 // k_hop computes, for every half site of parity out_par and every rhs,
 //   out = a * [g5] H [g5] in  (+ x)          (XPAY adds the own-parity field x)
-// and optionally the per-block partial sums of Re<d, out> (DOT) for the CG
-// scalar <p, A p>.  With a = -kappa this is Dirac_Wilson::mult for the EO
+// and per-block partial sums for the CG scalars as the epilogue mode asks
+// (HopEpilogue: Re<d, out>, |out|^2, or the residual update r -= alpha out
+// with |r|^2).  With a = -kappa this is Dirac_Wilson::mult for the EO
 // constructors (dirac_wilson.cpp:28-32, 107-113); with g5 folded on both
 // sides it is mult_dag (:34-36); with XPAY and a = -kappa^2 on the second
 // hop it is Dirac_Wilson_EvenOdd::mult/mult_dag
@@ -18,15 +19,67 @@
 
 namespace ii {
 
-// BS threads per block (kBlock for the DOT variants, whose partial sums are
+// The epilogue (HopEpilogue) of one site's output o of rhs-offset accessors
+// dr (the dv field) and outr; returns the site's partial sum.  16- and
+// 32-bit fields: the 24-term site sums in FP32, the sums over sites in FP64.
+template <int EPI, typename R, typename C, typename SpX, typename StOut>
+__device__ __forceinline__ double hop_epilogue(C o[12], int hs, const SpX& dr, const StOut& outr,
+                                               double alpha) {
+  double dsum = 0.0;
+  constexpr bool kF64 = std::is_same<R, double>::value;
+  if constexpr (EPI == kEpiDot) {
+    if constexpr (kF64) {
+      const double ds = (double)dr.scale(hs);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        auto dvv = dr.ld(hs, k);
+        dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
+      }
+    } else {
+      float sd = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        auto dvv = dr.ld(hs, k);
+        sd = fmaf((float)dvv.x, o[k].x, sd);
+        sd = fmaf((float)dvv.y, o[k].y, sd);
+      }
+      dsum = (double)sd * (double)dr.scale(hs);
+    }
+  } else if constexpr (EPI == kEpiNorm) {
+    R sd = R(0);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      sd = fma(o[k].x, o[k].x, sd);
+      sd = fma(o[k].y, o[k].y, sd);
+    }
+    dsum = (double)sd;
+  } else if constexpr (EPI == kEpiRUpd) {
+    const R ds = (R)dr.scale(hs), al = (R)alpha;
+    R sd = R(0);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      auto dvv = dr.ld(hs, k);
+      o[k].x = fma(-al, o[k].x, (R)dvv.x * ds);
+      o[k].y = fma(-al, o[k].y, (R)dvv.y * ds);
+      sd = fma(o[k].x, o[k].x, sd);
+      sd = fma(o[k].y, o[k].y, sd);
+    }
+    dsum = (double)sd;
+  }
+  outr.st(hs, o);
+  return dsum;
+}
+
+// BS threads per block (kBlock for the variants with partial sums, which are
 // indexed by kBlock-site ranges; the autotuner picks it for the others).
-template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Gauge,
+template <bool G5IN, bool G5OUT, bool XPAY, int EPI, typename SpIn, typename Gauge,
           typename StOut, typename SpX, int BS = kBlock>
 __global__ void __launch_bounds__(BS)
 k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
       typename SpIn::R a, bool tbc_flip, double* __restrict__ partials,
-      const int* __restrict__ skip, int skip_stride, HopOrder order) {
-  static_assert(!DOT || BS == kBlock, "DOT partials are per kBlock sites");
+      const int* __restrict__ skip, int skip_stride, HopOrder order,
+      const double* __restrict__ coef, int coef_stride) {
+  static_assert(EPI == kEpiStore || BS == kBlock, "partial sums are per kBlock sites");
   using C = typename SpIn::C;
   const int nblk = gridDim.x;
   const int base = hop_block_base(g, order, blockIdx.x, nblk, BS);
@@ -55,31 +108,10 @@ k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
         o[k].y += (typename SpIn::R)(xv.y * xs);
       }
     }
-    if (DOT) {
-      const SpX dr = dv.rhs(rhs);
-      if constexpr (std::is_same<typename SpIn::R, double>::value) {
-        const double ds = (double)dr.scale(hs);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-          auto dvv = dr.ld(hs, k);
-          dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
-        }
-      } else {
-        // FP32 arithmetic: the 24-term site product in FP32, the sum over
-        // sites in FP64
-        float sd = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-          auto dvv = dr.ld(hs, k);
-          sd = fmaf((float)dvv.x, o[k].x, sd);
-          sd = fmaf((float)dvv.y, o[k].y, sd);
-        }
-        dsum += (double)sd * (double)dr.scale(hs);
-      }
-    }
-    out.rhs(rhs).st(hs, o);
+    dsum = hop_epilogue<EPI, typename SpIn::R>(o, hs, dv.rhs(rhs), out.rhs(rhs),
+                                               EPI == kEpiRUpd ? coef[rhs * coef_stride] : 0.0);
   }
-  if (DOT) {
+  if (EPI != kEpiStore) {
     double s = block_sum(dsum);
     // indexed by the site range, so sums do not depend on the traversal
     if (threadIdx.x == 0) partials[(size_t)rhs * nblk + base / kBlock] = s;
@@ -92,12 +124,13 @@ k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
 // site for all rhs; then thread (tx, ty) applies the stencil to site tx of
 // rhs ty (one wavefront per rhs for TS = 64).  Partial dot products are
 // per (rhs, tile): partials[rhs * gridDim.x + blockIdx.x].
-template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Gauge,
+template <bool G5IN, bool G5OUT, bool XPAY, int EPI, typename SpIn, typename Gauge,
           typename StOut, typename SpX, int TS, int MAXT>
 __global__ void __launch_bounds__(MAXT)
 k_hop_mrhs(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
            typename SpIn::R a, bool tbc_flip, double* __restrict__ partials,
-           const int* __restrict__ skip, int skip_stride, int nrhs, HopOrder order) {
+           const int* __restrict__ skip, int skip_stride, int nrhs, HopOrder order,
+           const double* __restrict__ coef, int coef_stride) {
   using C = typename SpIn::C;
   using UC = typename Gauge::C;
   // FP16 fields and links: colour products with v_dot2_f32_f16 on links
@@ -219,31 +252,10 @@ k_hop_mrhs(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
         o[k].y += (typename SpIn::R)(xv.y * xs);
       }
     }
-    if (DOT) {
-      const SpX dr = dv.rhs(rhs);
-      if constexpr (std::is_same<typename SpIn::R, double>::value) {
-        const double ds = (double)dr.scale(hs);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-          auto dvv = dr.ld(hs, k);
-          dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
-        }
-      } else {
-        // FP32 arithmetic: the 24-term site product in FP32, the sum over
-        // sites in FP64
-        float sd = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-          auto dvv = dr.ld(hs, k);
-          sd = fmaf((float)dvv.x, o[k].x, sd);
-          sd = fmaf((float)dvv.y, o[k].y, sd);
-        }
-        dsum += (double)sd * (double)dr.scale(hs);
-      }
-    }
-    out.rhs(rhs).st(hs, o);
+    dsum = hop_epilogue<EPI, typename SpIn::R>(o, hs, dv.rhs(rhs), out.rhs(rhs),
+                                               EPI == kEpiRUpd ? coef[rhs * coef_stride] : 0.0);
   }
-  if (DOT) {
+  if (EPI != kEpiStore) {
     for (int off = TS / 2; off > 0; off >>= 1) dsum += __shfl_down(dsum, off, TS);
     if (tx == 0 && live) partials[(size_t)rhs * ntile + blockIdx.x] = dsum;
   }
@@ -269,21 +281,20 @@ namespace {
 template <typename Gauge> struct MrhsShape { static constexpr int TS = 64, MAXT = 768; };
 template <> struct MrhsShape<GaugeF64> { static constexpr int TS = 64, MAXT = 768; };
 
-template <bool G5IN, bool G5OUT, bool XPAY, bool DOT, typename SpIn, typename Gauge,
+template <bool G5IN, bool G5OUT, bool XPAY, int EPI, typename SpIn, typename Gauge,
           typename StOut, typename SpX>
 ii_status launch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, const StOut& out,
-                     const SpX& x, const SpX& dv, int out_par, typename SpIn::R a, int nrhs,
-                     double* partials, const int* skip, int skip_stride) {
+                     const SpX& x, const SpX& dv, typename SpIn::R a, const HopSpec& h) {
   const Geom& g = ctx->g;
   bool flip = ctx->tbc_sign < 0;
-  if (nrhs == 1) {
+  if (h.nrhs == 1) {
     const HopOrder& o = ctx->hop_order;
-    if constexpr (!DOT && std::is_same<Gauge, GaugeF64>::value) {
+    if constexpr (EPI == kEpiStore && std::is_same<Gauge, GaugeF64>::value) {
       if (o.bs == 128) {
         dim3 grid(blocks_for(g.Vh, 128), 1);
-        hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX, 128>), grid,
-                           dim3(128), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a, flip,
-                           partials, skip, skip_stride, o);
+        hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, EPI, SpIn, Gauge, StOut, SpX, 128>), grid,
+                           dim3(128), 0, ctx->stream, g, in, gauge, out, x, dv, h.out_par, a,
+                           flip, h.partials, h.skip, h.skip_stride, o, h.coef, h.coef_stride);
         II_HIP(ctx, hipGetLastError());
         return II_OK;
       }
@@ -292,16 +303,17 @@ ii_status launch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, const 
     ok.bs = kBlock;
     if (ok.mode == 1 && o.bs != kBlock) ok = HopOrder();  // strip orders are built for kBlock
     dim3 grid(blocks_for(g.Vh), 1);
-    hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX>), grid,
-                       dim3(kBlock), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a, flip,
-                       partials, skip, skip_stride, ok);
+    hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, EPI, SpIn, Gauge, StOut, SpX>), grid,
+                       dim3(kBlock), 0, ctx->stream, g, in, gauge, out, x, dv, h.out_par, a, flip,
+                       h.partials, h.skip, h.skip_stride, ok, h.coef, h.coef_stride);
   } else {
     constexpr int TS = MrhsShape<Gauge>::TS, MAXT = MrhsShape<Gauge>::MAXT;
-    const int rb = std::min(nrhs, MAXT / TS);
-    dim3 grid((g.Vh + TS - 1) / TS, (nrhs + rb - 1) / rb);
-    hipLaunchKernelGGL((k_hop_mrhs<G5IN, G5OUT, XPAY, DOT, SpIn, Gauge, StOut, SpX, TS, MAXT>),
-                       grid, dim3(TS, rb), 0, ctx->stream, g, in, gauge, out, x, dv, out_par, a,
-                       flip, partials, skip, skip_stride, nrhs, ctx->mrhs_order);
+    const int rb = std::min(h.nrhs, MAXT / TS);
+    dim3 grid((g.Vh + TS - 1) / TS, (h.nrhs + rb - 1) / rb);
+    hipLaunchKernelGGL((k_hop_mrhs<G5IN, G5OUT, XPAY, EPI, SpIn, Gauge, StOut, SpX, TS, MAXT>),
+                       grid, dim3(TS, rb), 0, ctx->stream, g, in, gauge, out, x, dv, h.out_par,
+                       a, flip, h.partials, h.skip, h.skip_stride, h.nrhs, ctx->mrhs_order,
+                       h.coef, h.coef_stride);
   }
   II_HIP(ctx, hipGetLastError());
   return II_OK;
@@ -312,22 +324,27 @@ ii_status dispatch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, cons
                        const SpX& x, const SpX& dv, const HopSpec& h) {
   using R = typename SpIn::R;
   const R a = (R)h.a;
-  const int sel = (h.g5in ? 8 : 0) | (h.g5out ? 4 : 0) | (h.xpay ? 2 : 0) | (h.dot ? 1 : 0);
-#define II_CASE(N, A, B, CC, D)                                                              \
-  case N:                                                                                    \
-    return launch_hop<A, B, CC, D>(ctx, in, gauge, out, x, dv, h.out_par, a, h.nrhs, h.partials, \
-                                   h.skip, h.skip_stride);
+  if (h.epi < kEpiStore || h.epi > kEpiRUpd)
+    return set_error(ctx, II_ERR_BAD_ARG, "unknown hop epilogue");
+  if (h.epi == kEpiRUpd && !h.coef)
+    return set_error(ctx, II_ERR_BAD_ARG, "residual-update hop without its scalars");
+  const int sel = (h.g5in ? 16 : 0) | (h.g5out ? 8 : 0) | (h.xpay ? 4 : 0) | h.epi;
+#define II_CASE(A, B, CC, E)                                                                 \
+  case ((A) ? 16 : 0) | ((B) ? 8 : 0) | ((CC) ? 4 : 0) | (E):                                \
+    return launch_hop<A, B, CC, E>(ctx, in, gauge, out, x, dv, a, h);
   switch (sel) {
-    II_CASE(0, false, false, false, false)
-    II_CASE(2, false, false, true, false)
-    II_CASE(3, false, false, true, true)
-    II_CASE(12, true, true, false, false)
-    II_CASE(14, true, true, true, false)
-    II_CASE(15, true, true, true, true)
-    II_CASE(8, true, false, false, false)
-    II_CASE(4, false, true, false, false)
-    II_CASE(6, false, true, true, false)
-    II_CASE(7, false, true, true, true)
+    II_CASE(false, false, false, kEpiStore)
+    II_CASE(false, false, true, kEpiStore)
+    II_CASE(false, false, true, kEpiDot)
+    II_CASE(false, false, true, kEpiNorm)
+    II_CASE(true, true, false, kEpiStore)
+    II_CASE(true, true, true, kEpiStore)
+    II_CASE(true, true, true, kEpiDot)
+    II_CASE(true, false, false, kEpiStore)
+    II_CASE(false, true, false, kEpiStore)
+    II_CASE(false, true, true, kEpiStore)
+    II_CASE(false, true, true, kEpiDot)
+    II_CASE(false, true, true, kEpiRUpd)
     default:
       return set_error(ctx, II_ERR_BAD_ARG, "unsupported hop variant");
   }
@@ -352,7 +369,7 @@ ii_status tune_hop(ii_context* ctx) {
   const Geom& g = ctx->g;
   const int P = g.Lxh * g.L[1];
   std::vector<HopOrder> cands(1);
-  // contiguous order with 128 threads per block (non-DOT FP64 hops; 512
+  // contiguous order with 128 threads per block (FP64 hops without partial sums; 512
   // would cap the FP64 stencil's 256 registers and spill)
   if (g.Vh >= 8 * 128) {
     HopOrder o;

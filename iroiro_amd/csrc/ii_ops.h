@@ -10,13 +10,24 @@ namespace ii {
 
 // One hop launch: out(out_par) = a * [g5] H [g5] in (+ x), optional
 // partial sums of Re<dv, out> into partials[rhs * nblocks + block].
+// What a hop does with its output o at a site (after the scale a and XPAY):
+//   kEpiStore  store o
+//   kEpiDot    store o; partial sums of Re<dv, o>     (CG <p, A p>)
+//   kEpiNorm   store o; partial sums of |o|^2         (<p, Dhat^dag Dhat p> = |Dhat p|^2)
+//   kEpiRUpd   r = dv - alpha o, alpha = coef[rhs * coef_stride] (a device
+//              scalar); store r in place of o; partial sums of |r|^2
+//              (the CG residual update fused into the last hop of A p)
+enum HopEpilogue { kEpiStore = 0, kEpiDot = 1, kEpiNorm = 2, kEpiRUpd = 3 };
+
 struct HopSpec {
   int out_par = 0;
   double a = 1.0;
   bool g5in = false;
   bool g5out = false;
   bool xpay = false;
-  bool dot = false;
+  int epi = kEpiStore;
+  const double* coef = nullptr;  // kEpiRUpd: alpha of rhs r at coef[r * coef_stride]
+  int coef_stride = 0;
   int nrhs = 1;
   double* partials = nullptr;
   // optional per-rhs skip flags: rhs r is skipped when skip[r * skip_stride] != 0

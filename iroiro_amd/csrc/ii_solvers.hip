@@ -199,32 +199,6 @@ k_cg_p(Geom g, SpR r, SpP pr, StP pw, const RUScal* __restrict__ sc) {
   st_site(pw.rhs(rhs), hs, pv);
 }
 
-// r -= alpha s ; partial |r|^2: the CG step with the solution update
-// deferred (k_cg_flush adds the search directions to y later)
-template <typename SpS, typename SpR, typename StR>
-__global__ void __launch_bounds__(kBlock)
-k_cg_r(Geom g, SpS s, SpR rr_, StR rw, RUScal* __restrict__ sc, double* __restrict__ partials) {
-  const int rhs = blockIdx.y;
-  const RUScal& S = sc[rhs];
-  if (S.done) return;
-  const double a = S.alpha;
-  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc = 0.0;
-  if (hs < g.Vh) {
-    double sv[24], rv[24];
-    ld_site(s.rhs(rhs), hs, sv);
-    ld_site(rr_.rhs(rhs), hs, rv);
-#pragma unroll
-    for (int k = 0; k < 24; ++k) {
-      rv[k] = fma(-a, sv[k], rv[k]);
-      acc = fma(rv[k], rv[k], acc);
-    }
-    st_site(rw.rhs(rhs), hs, rv);
-  }
-  acc = block_sum(acc);
-  if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = acc;
-}
-
 // Deferred solution update y += sum_j alpha_j p_j over up to kDefer stored
 // search directions: alpha_j = ahist[rhs * astride + slot[j]] (0 for the
 // iterations after an rhs finished, whose directions are not read).  One
@@ -451,9 +425,53 @@ ii_status ddagdP(ii_context* ctx, int P, double kappa, const SF& f, const SF& ou
   h.xpay = true;
   h.g5in = false;
   h.g5out = true;
-  h.dot = dv != nullptr;
+  h.epi = dv != nullptr ? kEpiDot : kEpiStore;
   h.partials = partials;
   return hopP(ctx, P, h, t, out, &q, dv);  // out = Dhat^dag q
+}
+
+// One CG step on A = Dhat^dag Dhat with the scalars and the residual update
+// fused into the hops (sloppy multi-rhs solve):
+//   q = Dhat p, with partial sums of |q|^2 = <p, A p>   (second hop, kEpiNorm)
+//   alpha = rr / |q|^2                                   (k_fin_pap; ahist)
+//   r -= alpha Dhat^dag q, with partial sums of |r|^2   (fourth hop, kEpiRUpd)
+// A p itself is never stored; the k_fin_rr that follows reads the |r|^2
+// partials (hop_dot_parts per rhs).
+static_assert(sizeof(RUScal) % sizeof(double) == 0, "RUScal is read as a double array");
+ii_status ddagd_cg_fused(ii_context* ctx, int P, double kappa, const SF& p, const SF& r,
+                         const SF& q, const SF& t, RUScal* sc, double* partials, int nrhs,
+                         const int* skip, int skip_stride, double* ahist, int aidx,
+                         int astride) {
+  HopSpec h;
+  h.nrhs = nrhs;
+  h.skip = skip;
+  h.skip_stride = skip_stride;
+  h.out_par = 1;
+  h.a = 1.0;
+  II_CHECK(hopP(ctx, P, h, p, t, nullptr, nullptr));
+  h.out_par = 0;
+  h.a = -kappa * kappa;
+  h.xpay = true;
+  h.epi = kEpiNorm;
+  h.partials = partials;
+  II_CHECK(hopP(ctx, P, h, t, q, &p, nullptr));  // q = Dhat p, |q|^2
+  hipLaunchKernelGGL(k_fin_pap, dim3(nrhs), dim3(kBlock), 0, ctx->stream, partials,
+                     hop_dot_parts(ctx->g, nrhs, P), sc, ahist, aidx, astride);
+  h.out_par = 1;
+  h.a = 1.0;
+  h.xpay = false;
+  h.g5in = true;
+  h.epi = kEpiStore;
+  II_CHECK(hopP(ctx, P, h, q, t, nullptr, nullptr));
+  h.out_par = 0;
+  h.a = -kappa * kappa;
+  h.xpay = true;
+  h.g5in = false;
+  h.g5out = true;
+  h.epi = kEpiRUpd;
+  h.coef = &sc[0].alpha;
+  h.coef_stride = (int)(sizeof(RUScal) / sizeof(double));
+  return hopP(ctx, P, h, t, r, &q, &r);  // r -= alpha Dhat^dag q, |r|^2
 }
 
 template <int P>
@@ -490,17 +508,6 @@ ii_status cg_p_update(ii_context* ctx, const SF& r, const SF& p, const SF& pout,
   hipLaunchKernelGGL((k_cg_p<typename A::Sp, typename A::Sp, typename A::St>),
                      dim3(blocks_for(g.Vh), nrhs), dim3(kBlock), 0, ctx->stream, g,
                      A::sp(r, g.stride), A::sp(p, g.stride), A::st(pout, g.stride), sc);
-  II_HIP(ctx, hipGetLastError());
-  return II_OK;
-}
-template <int P>
-ii_status cg_r_update(ii_context* ctx, const SF& s, const SF& r, RUScal* sc, int nrhs,
-                      double* partials, int nb) {
-  const Geom& g = ctx->g;
-  using A = Acc<P>;
-  hipLaunchKernelGGL((k_cg_r<typename A::Sp, typename A::Sp, typename A::St>), dim3(nb, nrhs),
-                     dim3(kBlock), 0, ctx->stream, g, A::sp(s, g.stride), A::sp(r, g.stride),
-                     A::st(r, g.stride), sc, partials);
   II_HIP(ctx, hipGetLastError());
   return II_OK;
 }
@@ -581,15 +588,6 @@ ii_status cg_p_P(ii_context* ctx, int P, const SF& r, const SF& p, const SF& pou
     case 1: return cg_p_update<1>(ctx, r, p, pout, sc, nrhs);
     case 2: return cg_p_update<2>(ctx, r, p, pout, sc, nrhs);
     default: return cg_p_update<3>(ctx, r, p, pout, sc, nrhs);
-  }
-}
-ii_status cg_r_P(ii_context* ctx, int P, const SF& s, const SF& r, RUScal* sc, int nrhs,
-                 double* partials, int nb) {
-  switch (P) {
-    case 1: return cg_r_update<1>(ctx, s, r, sc, nrhs, partials, nb);
-    case 2: return cg_r_update<2>(ctx, s, r, sc, nrhs, partials, nb);
-    case 3: return cg_r_update<3>(ctx, s, r, sc, nrhs, partials, nb);
-    default: return set_error(ctx, II_ERR_BAD_ARG, "deferred update needs a sloppy precision");
   }
 }
 ii_status cg_flush_P(ii_context* ctx, int P, const SF* ring, const int* slots, int n,
@@ -1434,8 +1432,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
   const int ytype = lo ? II_PREC_SINGLE : II_PREC_DOUBLE;
   if (lo) {
     II_CHECK(fieldP(ctx, 50, P, batch, &ps));
-    II_CHECK(fieldP(ctx, 51, P, batch, &rs));
-    II_CHECK(fieldP(ctx, 52, P, batch, &ss_));
+    II_CHECK(fieldP(ctx, 51, P, batch, &rs));  // (A p is not stored: fused CG step)
     II_CHECK(fieldP(ctx, 53, P, batch, &qs));
     II_CHECK(fieldP(ctx, 54, P, batch, &ts));
     II_CHECK(fieldP(ctx, 55, II_PREC_SINGLE, batch, &ys));
@@ -1522,12 +1519,18 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
     // into a HIP graph and replayed.
     auto iteration = [&](int cur) -> ii_status {
       const SF& p = ring[cur];
-      II_CHECK(ddagdP(ctx, P, kappa, p, ss_, qs, ts, &p, part, nr, skip_it, ss));
-      hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
-                         hop_dot_parts(g, nr, P), sc, ahist, cur, nring);
-      if (defer) II_CHECK(cg_r_P(ctx, P, ss_, rs, sc, nr, part, nb));
-      else II_CHECK(cg_blas_P(ctx, P, p, ss_, rs, ys, ytype, sc, nr, part, nb));
-      hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc, 1);
+      if (defer) {
+        II_CHECK(ddagd_cg_fused(ctx, P, kappa, p, rs, qs, ts, sc, part, nr, skip_it, ss, ahist,
+                                cur, nring));
+        hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
+                           hop_dot_parts(g, nr, P), sc, 1);
+      } else {
+        II_CHECK(ddagdP(ctx, P, kappa, p, ss_, qs, ts, &p, part, nr, skip_it, ss));
+        hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
+                           hop_dot_parts(g, nr, P), sc, nullptr, 0, 0);
+        II_CHECK(cg_blas_P(ctx, P, p, ss_, rs, ys, ytype, sc, nr, part, nb));
+        hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc, 1);
+      }
       return cg_p_P(ctx, P, rs, p, ring[(cur + 1) % nring], sc, nr);
     };
     auto flush = [&]() -> ii_status {

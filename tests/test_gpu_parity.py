@@ -150,7 +150,8 @@ def _prop_setup(dims, seed=5, eps=0.6):
 
 
 @pytest.mark.parametrize("precision,batched", [("double", False), ("double", True),
-                                               ("single", True), ("half", True), ("bf16", True)])
+                                               ("single", True), ("half", True), ("bf16", True),
+                                               ("half", False)])
 def test_propagator_true_residual(precision, batched):
     dims = (8, 8, 8, 8)
     lat, u = _prop_setup(dims)
