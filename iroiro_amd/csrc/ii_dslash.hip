@@ -337,6 +337,7 @@ ii_status dispatch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, cons
     II_CASE(false, false, true, kEpiStore)
     II_CASE(false, false, true, kEpiDot)
     II_CASE(false, false, true, kEpiNorm)
+    II_CASE(false, false, true, kEpiRUpd)
     II_CASE(true, true, false, kEpiStore)
     II_CASE(true, true, true, kEpiStore)
     II_CASE(true, true, true, kEpiDot)

@@ -203,6 +203,9 @@ k_cg_p(Geom g, SpR r, SpP pr, StP pw, const RUScal* __restrict__ sc) {
 // search directions: alpha_j = ahist[rhs * astride + slot[j]] (0 for the
 // iterations after an rhs finished, whose directions are not read).  One
 // pass over y for kDefer iterations instead of a y read and write in each.
+// With y64, the rhs not masked by skip also fold the sum into the FP64
+// solution, y64 += y + sum_j alpha_j p_j, and restart y at 0 (the reliable
+// update's accumulation in the same pass).
 constexpr int kDefer = 8;
 template <typename SpP>
 struct DeferredDirs {
@@ -213,10 +216,11 @@ struct DeferredDirs {
 template <typename SpP, typename SpY, typename StY>
 __global__ void __launch_bounds__(kBlock)
 k_cg_flush(Geom g, DeferredDirs<SpP> d, const double* __restrict__ ahist, int astride, SpY yr,
-           StY yw) {
+           StY yw, double2* __restrict__ y64, const int* __restrict__ skip, int skip_stride) {
   const int rhs = blockIdx.y;
   const int hs = blockIdx.x * blockDim.x + threadIdx.x;
   if (hs >= g.Vh) return;
+  const bool fold = y64 && !(skip && skip[rhs * skip_stride]);
   double yv[24];
   ld_site(yr.rhs(rhs), hs, yv);
 #pragma unroll
@@ -230,6 +234,18 @@ k_cg_flush(Geom g, DeferredDirs<SpP> d, const double* __restrict__ ahist, int as
         for (int k = 0; k < 24; ++k) yv[k] = fma(a, pv[k], yv[k]);
       }
     }
+  }
+  if (fold) {
+    double2* y = y64 + (size_t)rhs * 12 * g.stride;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      double2 o = y[(size_t)k * g.stride + hs];
+      o.x += yv[2 * k];
+      o.y += yv[2 * k + 1];
+      y[(size_t)k * g.stride + hs] = o;
+    }
+#pragma unroll
+    for (int k = 0; k < 24; ++k) yv[k] = 0.0;
   }
   st_site(yw.rhs(rhs), hs, yv);
 }
@@ -270,29 +286,6 @@ k_accum(Geom g, SpY ys, StY yz, double2* __restrict__ y64, const int* __restrict
 #pragma unroll
   for (int k = 0; k < 24; ++k) z[k] = 0.0;
   st_site(yz.rhs(rhs), hs, z);
-}
-
-// rho = bhat - q ; partial |rho|^2, masked (FP64)
-__global__ void __launch_bounds__(kBlock)
-k_resid(Geom g, const double2* __restrict__ bh, const double2* __restrict__ q,
-        double2* __restrict__ rho, const int* __restrict__ skip, int skip_stride,
-        double* __restrict__ partials) {
-  const int rhs = blockIdx.y;
-  if (skip && skip[rhs * skip_stride]) return;
-  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc = 0.0;
-  if (hs < g.Vh) {
-    size_t off = (size_t)rhs * 12 * g.stride + hs;
-#pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      double2 b = bh[off + (size_t)k * g.stride], v = q[off + (size_t)k * g.stride];
-      double2 d = make_double2(b.x - v.x, b.y - v.y);
-      rho[off + (size_t)k * g.stride] = d;
-      acc = fma(d.x, d.x, fma(d.y, d.y, acc));
-    }
-  }
-  acc = block_sum(acc);
-  if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = acc;
 }
 
 // masked |a|^2 partials (FP64)
@@ -511,10 +504,12 @@ ii_status cg_p_update(ii_context* ctx, const SF& r, const SF& p, const SF& pout,
   II_HIP(ctx, hipGetLastError());
   return II_OK;
 }
-// y (FP32) += the deferred directions ring[slots[j]], j < n <= kDefer
+// y (FP32) += the deferred directions ring[slots[j]], j < n <= kDefer; with
+// y64, also fold into y64 (rhs not masked by skip)
 template <int P>
 ii_status cg_flush(ii_context* ctx, const SF* ring, const int* slots, int n, const double* ahist,
-                   int astride, const SF& y, int nrhs) {
+                   int astride, const SF& y, int nrhs, double2* y64, const int* skip,
+                   int skip_stride) {
   const Geom& g = ctx->g;
   using A = Acc<P>;
   DeferredDirs<typename A::Sp> d{};
@@ -526,7 +521,8 @@ ii_status cg_flush(ii_context* ctx, const SF* ring, const int* slots, int n, con
   for (int j = n; j < kDefer; ++j) d.p[j] = d.p[0];
   hipLaunchKernelGGL((k_cg_flush<typename A::Sp, SpF32, StF32>), dim3(blocks_for(g.Vh), nrhs),
                      dim3(kBlock), 0, ctx->stream, g, d, ahist, astride,
-                     Acc<II_PREC_SINGLE>::sp(y, g.stride), Acc<II_PREC_SINGLE>::st(y, g.stride));
+                     Acc<II_PREC_SINGLE>::sp(y, g.stride), Acc<II_PREC_SINGLE>::st(y, g.stride),
+                     y64, skip, skip_stride);
   II_HIP(ctx, hipGetLastError());
   return II_OK;
 }
@@ -591,11 +587,12 @@ ii_status cg_p_P(ii_context* ctx, int P, const SF& r, const SF& p, const SF& pou
   }
 }
 ii_status cg_flush_P(ii_context* ctx, int P, const SF* ring, const int* slots, int n,
-                     const double* ahist, int astride, const SF& y, int nrhs) {
+                     const double* ahist, int astride, const SF& y, int nrhs,
+                     double2* y64 = nullptr, const int* skip = nullptr, int skip_stride = 0) {
   switch (P) {
-    case 1: return cg_flush<1>(ctx, ring, slots, n, ahist, astride, y, nrhs);
-    case 2: return cg_flush<2>(ctx, ring, slots, n, ahist, astride, y, nrhs);
-    case 3: return cg_flush<3>(ctx, ring, slots, n, ahist, astride, y, nrhs);
+    case 1: return cg_flush<1>(ctx, ring, slots, n, ahist, astride, y, nrhs, y64, skip, skip_stride);
+    case 2: return cg_flush<2>(ctx, ring, slots, n, ahist, astride, y, nrhs, y64, skip, skip_stride);
+    case 3: return cg_flush<3>(ctx, ring, slots, n, ahist, astride, y, nrhs, y64, skip, skip_stride);
     default: return set_error(ctx, II_ERR_BAD_ARG, "deferred update needs a sloppy precision");
   }
 }
@@ -1416,6 +1413,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
   // aux: |b_e|^2 [0, batch), |b_o|^2 [batch, 2 batch), |c|^2 [2 batch, 3 batch)
   II_CHECK(workspace(ctx, 40, sizeof(double) * 3 * batch + 64, &auxp));
   double* aux = (double*)auxp;
+  double* one = aux + 3 * batch;  // the device scalar 1.0 (alpha of rho = bhat - Dhat y)
   // FP64 fields
   SF be, bo, bh, y, r64, q64, t64, rho;
   II_CHECK(fieldP(ctx, 41, 0, batch, &be));
@@ -1494,6 +1492,8 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       for (int k = 0; k < nr; ++k) hb[k] += hb[batch + k];
       II_HIP(ctx, hipMemcpyAsync(aux, hb.data(), nr * sizeof(double), hipMemcpyHostToDevice,
                                  ctx->stream));
+      const double unit = 1.0;
+      II_HIP(ctx, hipMemcpyAsync(one, &unit, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
       II_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     hipLaunchKernelGGL(k_scal_init, dim3(nr), dim3(64), 0, ctx->stream, sc, aux + 2 * batch, aux,
@@ -1537,6 +1537,14 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       if (pend.empty()) return II_OK;
       II_CHECK(cg_flush_P(ctx, P, ring.data(), pend.data(), (int)pend.size(), ahist, nring, ys,
                           nr));
+      pend.clear();
+      return II_OK;
+    };
+    // the pending directions plus ys into the FP64 solution y (rhs not
+    // masked by skip), ys restarted at 0
+    auto flush_fold = [&](const int* skip) -> ii_status {
+      II_CHECK(cg_flush_P(ctx, P, ring.data(), pend.data(), (int)pend.size(), ahist, nring, ys,
+                          nr, (double2*)y.v, skip, ss));
       pend.clear();
       return II_OK;
     };
@@ -1592,14 +1600,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       if (all_done) break;
       if (any_upd) {
         // reliable update, FP64, for the flagged rhs only
-        II_CHECK(flush());
-        if (lo) {
-          const Geom& gg = g;
-          hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(gg.Vh), nr), dim3(kBlock), 0,
-                             ctx->stream, gg, Acc<1>::sp(ys, gg.stride), Acc<1>::st(ys, gg.stride),
-                             (double2*)y.v, skip_ru, ss);
-          II_HIP(ctx, hipGetLastError());
-        }
+        if (lo) II_CHECK(flush_fold(skip_ru));
         // q = Dhat y ; rho = bhat - q ; |rho|^2
         h = HopSpec();
         h.nrhs = nr;
@@ -1608,14 +1609,18 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
         h.out_par = 1;
         h.a = 1.0;
         II_CHECK(hop64(ctx, h, (double2*)y.v, (double2*)t64.v, nullptr, nullptr));
+        // rho = bhat - (y - kappa^2 H_eo t) and |rho|^2 in the hop's epilogue
         h.out_par = 0;
         h.a = -kappa * kappa;
         h.xpay = true;
-        II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)q64.v, (double2*)y.v, nullptr));
-        hipLaunchKernelGGL(k_resid, dim3(nb, nr), dim3(kBlock), 0, ctx->stream, g,
-                           (const double2*)bh.v, (const double2*)q64.v, (double2*)rho.v, skip_ru,
-                           ss, part);
-        hipLaunchKernelGGL(k_fin_rho, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc);
+        h.epi = kEpiRUpd;
+        h.coef = one;
+        h.coef_stride = 0;
+        h.partials = part;
+        II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)rho.v, (double2*)y.v,
+                       (const double2*)bh.v));
+        hipLaunchKernelGGL(k_fin_rho, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
+                           hop_dot_parts(g, nr, II_PREC_DOUBLE), sc);
         // new CG residual r = Dhat^dag rho and |r|^2.  The true residual rho
         // is FP64; r only restarts the sloppy recursion, so with a sloppy
         // inner precision it is formed in that precision from rho.
@@ -1659,13 +1664,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       }
     }
     // fold the sloppy accumulator in (rhs not yet folded at convergence)
-    II_CHECK(flush());
-    if (lo) {
-      hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(g.Vh), nr), dim3(kBlock), 0,
-                         ctx->stream, g, Acc<1>::sp(ys, g.stride), Acc<1>::st(ys, g.stride),
-                         (double2*)y.v, nullptr, 0);
-      II_HIP(ctx, hipGetLastError());
-    }
+    if (lo) II_CHECK(flush_fold(nullptr));
     // x_o = b_o - mult_oe(y) = b_o + kappa H_oe y   (eoUtils.cpp:67)
     h = HopSpec();
     h.out_par = 1;
