@@ -1705,37 +1705,45 @@ ii_status full_residual(ii_context* ctx, double kappa, const ii_field* src, cons
                         double* res) {
   const Geom& g = ctx->g;
   const size_t he = half_elems(g);
-  double2 *w, *aux2;
+  const int n = src->nrhs;
+  double2* w;
+  double* d;  // |b - D x|^2 per (rhs, parity) [0, 2n), |b|^2 [2n, 4n), 1.0 at [4n]
   {
     void* p;
     II_CHECK(workspace(ctx, 56, he * sizeof(double2), &p));
     w = (double2*)p;
-    II_CHECK(workspace(ctx, 57, 8 * sizeof(double), &p));
-    aux2 = (double2*)p;
+    II_CHECK(workspace(ctx, 57, (4 * (size_t)n + 1) * sizeof(double), &p));
+    d = (double*)p;
   }
-  double* d = (double*)aux2;
-  for (int k = 0; k < src->nrhs; ++k) {
+  const double unit = 1.0;
+  II_HIP(ctx, hipMemcpyAsync(d + 4 * n, &unit, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int nblk = hop_dot_parts(g, 1, II_PREC_DOUBLE);
+  II_CHECK(ensure_partials(ctx, (size_t)nblk + 16));
+  for (int k = 0; k < n; ++k) {
     const double2* se = src->data + (size_t)k * 2 * he;
     const double2* xe = sol->data + (size_t)k * 2 * he;
-    double acc_r = 0.0, acc_b = 0.0;
     for (int par = 0; par < 2; ++par) {
+      // w = b_par - (x_par - kappa H x_other) = (b - D x)_par, |w|^2 in the
+      // hop's epilogue
       HopSpec h;
       h.out_par = par;
       h.a = -kappa;
       h.xpay = true;
-      // w = x_par - kappa H x_other = (D x)_par
-      II_CHECK(hop64(ctx, h, xe + (1 - par) * he, w, xe + par * he, nullptr));
-      II_CHECK(axpby64(ctx, 1.0, se + par * he, -1.0, w));
-      II_CHECK(dot64(ctx, w, w, d));
-      II_CHECK(dot64(ctx, se + par * he, se + par * he, d + 1));
-      double hh[2];
-      II_HIP(ctx, hipMemcpyAsync(hh, d, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-      II_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      acc_r += hh[0];
-      acc_b += hh[1];
+      h.epi = kEpiRUpd;
+      h.coef = d + 4 * n;
+      h.partials = ctx->partials;
+      II_CHECK(hop64(ctx, h, xe + (1 - par) * he, w, xe + par * he, se + par * he));
+      II_CHECK(finalize_sums(ctx, ctx->partials, nblk, d + 2 * k + par, 1));
     }
-    res[k] = std::sqrt(acc_r / acc_b);
+    II_CHECK(dot64(ctx, se, se, d + 2 * n + 2 * k, 2));
   }
+  std::vector<double> hh(4 * (size_t)n);
+  II_HIP(ctx, hipMemcpyAsync(hh.data(), d, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < n; ++k)
+    res[k] = std::sqrt((hh[2 * k] + hh[2 * k + 1]) / (hh[2 * n + 2 * k] + hh[2 * n + 2 * k + 1]));
   return II_OK;
 }
 
