@@ -206,8 +206,8 @@ k_hop_mrhs(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
       // 0.59 against 0.556 ms for 12 rhs at 32^3x64
       RawDir r0, r1;
 #define II_LD(MU, FWD, D, R) load_dir_raw<MU, FWD, G5IN>(inr, nbi[D][tx], R);
-#define II_CP(MU, FWD, D, R)                                                       \
-  hop_dir_dot2_raw<MU, FWD>(acc, R, [&](int k) { return lnk[(D * 9 + k) * TS + tx]; }); \
+#define II_CP(MU, FWD, D, R)                                                             \
+  hop_dir_dot2_raw<MU, FWD, G5IN>(acc, R, [&](int k) { return lnk[(D * 9 + k) * TS + tx]; }); \
   pin_acc(acc);
       II_LD(0, true, 0, r0) II_CP(0, true, 0, r0)
       II_LD(0, false, 1, r1) II_CP(0, false, 1, r1)

@@ -161,6 +161,15 @@ template <> __device__ __forceinline__ __hip_bfloat162 f2_to_lo<__hip_bfloat162>
   return r;
 }
 
+// Layout of one rhs: the two spins of a pair side by side, [colour c][spin
+// pair sp][site][2] of T: component comp = 3 s + c sits at 4-byte word
+// ((2 c + s/2) stride + site) 2 + s%2, so one 8-byte load returns spins
+// (2 sp, 2 sp + 1) of a colour.  Every spin projector of the stencil needs
+// either both spins of a pair or (the time direction) one whole pair.
+template <typename T>
+struct LoPair {
+  T lo, hi;  // spins 2 sp and 2 sp + 1
+};
 template <typename T>
 struct SpLo {
   using R = float;
@@ -171,7 +180,12 @@ struct SpLo {
   const float* __restrict__ nrm;
   int stride;
   __device__ __forceinline__ T raw(int site, int comp) const {
-    return ld_at(p, (uint32_t)(comp * stride), (uint32_t)site);
+    const int sp = (comp / 3) >> 1, h = (comp / 3) & 1, c = comp % 3;
+    return ld_at(p, (uint32_t)((2 * c + sp) * stride * 2 + h), (uint32_t)site * 2u);
+  }
+  __device__ __forceinline__ LoPair<T> raw2(int site, int c, int sp) const {
+    return ld_at(reinterpret_cast<const LoPair<T>*>(p), (uint32_t)((2 * c + sp) * stride),
+                 (uint32_t)site);
   }
   __device__ __forceinline__ C ld(int site, int comp) const { return lo_to_f2<T>(raw(site, comp)); }
   __device__ __forceinline__ R scale(int site) const { return ld_at(nrm, 0u, (uint32_t)site); }
@@ -198,9 +212,16 @@ struct StLo {
     const float inv = m > 0.0f ? 1.0f / m : 0.0f;
     st_at<float, kNT>(nrm, 0u, (uint32_t)site, m);
 #pragma unroll
-    for (int k = 0; k < 12; ++k)
-      st_at<T, kNT>(p, (uint32_t)(k * stride), (uint32_t)site,
-                    f2_to_lo<T>(make_float2(v[k].x * inv, v[k].y * inv)));
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int sp = 0; sp < 2; ++sp) {
+        const float2 a = v[3 * (2 * sp) + c], b = v[3 * (2 * sp + 1) + c];
+        LoPair<T> pr;
+        pr.lo = f2_to_lo<T>(make_float2(a.x * inv, a.y * inv));
+        pr.hi = f2_to_lo<T>(make_float2(b.x * inv, b.y * inv));
+        st_at<LoPair<T>, kNT>(reinterpret_cast<LoPair<T>*>(p), (uint32_t)((2 * c + sp) * stride),
+                              (uint32_t)site, pr);
+      }
   }
   __device__ __forceinline__ StLo rhs(int r) const {
     return StLo{p + (size_t)r * 12 * stride, nrm + (size_t)r * stride, stride};
@@ -360,12 +381,23 @@ __device__ __forceinline__ float load_project_u(const SpIn& in, int nb, float2 w
   constexpr int a2 = G5IN ? (P::A2 ^ 2) : P::A2, b2 = G5IN ? (P::B2 ^ 2) : P::B2;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const float2 va = in.ld(nb, 3 * a1 + c);
-    const float2 vb = P::C1 == 4 ? make_float2(0.f, 0.f) : in.ld(nb, 3 * b1 + c);
-    combine_u<P::C1>(va.x, va.y, vb.x, vb.y, w1[c].x, w1[c].y);
-    const float2 vc = in.ld(nb, 3 * a2 + c);
-    const float2 vd = P::C2 == 4 ? make_float2(0.f, 0.f) : in.ld(nb, 3 * b2 + c);
-    combine_u<P::C2>(vc.x, vc.y, vd.x, vd.y, w2[c].x, w2[c].y);
+    // the spins of colour c by pairs (one 8-byte load each); the time
+    // directions need the one pair a1 / 2
+    float2 v[4];
+    if (P::C1 == 4) {
+      const auto pr = in.raw2(nb, c, a1 >> 1);
+      v[a1] = lo_to_f2(pr.lo);
+      v[a1 + 1] = lo_to_f2(pr.hi);
+      v[b1] = v[a1];  // unused by code 4
+    } else {
+      const auto p0 = in.raw2(nb, c, 0), p1 = in.raw2(nb, c, 1);
+      v[0] = lo_to_f2(p0.lo);
+      v[1] = lo_to_f2(p0.hi);
+      v[2] = lo_to_f2(p1.lo);
+      v[3] = lo_to_f2(p1.hi);
+    }
+    combine_u<P::C1>(v[a1].x, v[a1].y, v[b1].x, v[b1].y, w1[c].x, w1[c].y);
+    combine_u<P::C2>(v[a2].x, v[a2].y, v[b2].x, v[b2].y, w2[c].x, w2[c].y);
   }
   const float n = in.scale(nb);
   return P::C1 == 4 ? n + n : n;
@@ -512,31 +544,40 @@ __device__ __forceinline__ h2v combine_h(h2v a, h2v b) {
 // entry k = 3a + b of M; projection / reconstruction tables as in
 // hop_dir_with.
 struct RawDir {
-  __half2 v[12];  // a1, b1, a2, b2 components (3 colours each); b unused for t
-  float n;        // site norm
+  __half2 v[4][3];  // [spin][colour]; the time directions load one spin pair
+  float n;          // site norm
 };
 template <int MU, bool FWD, bool G5IN, typename SpIn>
 __device__ __forceinline__ void load_dir_raw(const SpIn& in, int nb, RawDir& r) {
   using P = Proj<MU, FWD>;
-  constexpr int a1 = G5IN ? (P::A1 ^ 2) : P::A1, b1 = G5IN ? (P::B1 ^ 2) : P::B1;
-  constexpr int a2 = G5IN ? (P::A2 ^ 2) : P::A2, b2 = G5IN ? (P::B2 ^ 2) : P::B2;
+  constexpr int a1 = G5IN ? (P::A1 ^ 2) : P::A1;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    r.v[c] = in.raw(nb, 3 * a1 + c);
-    r.v[6 + c] = in.raw(nb, 3 * a2 + c);
-    if (P::C1 != 4) r.v[3 + c] = in.raw(nb, 3 * b1 + c);
-    if (P::C2 != 4) r.v[9 + c] = in.raw(nb, 3 * b2 + c);
+    if (P::C1 == 4) {
+      // 1 -/+ gamma_t: spins a1, a1 + 1 = the pair a1 / 2
+      const auto pr = in.raw2(nb, c, a1 >> 1);
+      r.v[a1][c] = pr.lo;
+      r.v[a1 + 1][c] = pr.hi;
+    } else {
+      const auto p0 = in.raw2(nb, c, 0), p1 = in.raw2(nb, c, 1);
+      r.v[0][c] = p0.lo;
+      r.v[1][c] = p0.hi;
+      r.v[2][c] = p1.lo;
+      r.v[3][c] = p1.hi;
+    }
   }
   r.n = in.scale(nb);
 }
-template <int MU, bool FWD, typename GetM>
+template <int MU, bool FWD, bool G5IN, typename GetM>
 __device__ __forceinline__ void hop_dir_dot2_raw(float2 acc[4][3], const RawDir& r, GetM&& getM) {
   using P = Proj<MU, FWD>;
+  constexpr int a1 = G5IN ? (P::A1 ^ 2) : P::A1, b1 = G5IN ? (P::B1 ^ 2) : P::B1;
+  constexpr int a2 = G5IN ? (P::A2 ^ 2) : P::A2, b2 = G5IN ? (P::B2 ^ 2) : P::B2;
   h2v w1[3], w2[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    w1[c] = combine_h<P::C1>(as_h2v(r.v[c]), P::C1 == 4 ? h2v{} : as_h2v(r.v[3 + c]));
-    w2[c] = combine_h<P::C2>(as_h2v(r.v[6 + c]), P::C2 == 4 ? h2v{} : as_h2v(r.v[9 + c]));
+    w1[c] = combine_h<P::C1>(as_h2v(r.v[a1][c]), P::C1 == 4 ? h2v{} : as_h2v(r.v[b1][c]));
+    w2[c] = combine_h<P::C2>(as_h2v(r.v[a2][c]), P::C2 == 4 ? h2v{} : as_h2v(r.v[b2][c]));
   }
   const float sc = P::C1 == 4 ? r.n + r.n : r.n;
   float2 h1[3], h2[3];
