@@ -308,7 +308,11 @@ typedef struct {
   double delta;          /* reliable-update trigger (residual drop factor), 0 = default */
   int use_graph;         /* capture solver iterations in a HIP graph */
   int use_learned;       /* apply the trained learned preconditioner (flexible outer) */
+  int method;            /* II_METHOD_CG: CG on Dhat^dag Dhat (Solver_CG on Fopr_DdagD);
+                            II_METHOD_BICGSTAB: BiCGStab on Dhat itself (no reference
+                            counterpart; sloppy precisions, batched) */
 } ii_prop_params;
+enum { II_METHOD_CG = 0, II_METHOD_BICGSTAB = 1 };
 
 /* Full solve D x = b for every rhs of a full field through the even/odd
  * Schur system (Inverter_WilsonLike::invert, eoUtils.cpp:43-80) with a

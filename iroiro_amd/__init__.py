@@ -912,14 +912,17 @@ class PropagatorSolver:
     on the device, stopping on the FP64 true residual |b - D x|/|b| per rhs.
 
     precision: 'double' | 'single' | 'half' | 'bf16' (inner storage);
-    batched: all rhs in one fused multi-rhs solve."""
+    batched: all rhs in one fused multi-rhs solve;
+    method: 'cg' (CG on Dhat^dag Dhat, the reference's Solver_CG on
+    Fopr_DdagD) or 'bicgstab' (BiCGStab on Dhat, sloppy precisions only)."""
 
     def __init__(self, ctx, kappa, tol=1e-10, precision="double", batched=False, maxiter=100000,
-                 delta=0.0, use_graph=False, use_learned=False):
+                 delta=0.0, use_graph=False, use_learned=False, method="cg"):
         self.ctx = ctx
         self.kappa = float(kappa)
         self.params = _lib.PropParamsC(tol, maxiter, PRECISIONS[precision], int(batched),
-                                       float(delta), int(use_graph), int(use_learned))
+                                       float(delta), int(use_graph), int(use_learned),
+                                       _lib.II_METHOD[method])
 
     def solve(self, src, sol):
         outs = (_lib.SolverOutputC * src.nrhs)()

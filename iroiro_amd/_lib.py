@@ -72,7 +72,11 @@ class SolverOutputC(ctypes.Structure):
 class PropParamsC(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("precision", ctypes.c_int),
                 ("batched", ctypes.c_int), ("delta", ctypes.c_double),
-                ("use_graph", ctypes.c_int), ("use_learned", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("use_learned", ctypes.c_int),
+                ("method", ctypes.c_int)]
+
+
+II_METHOD = {"cg": 0, "bicgstab": 1}
 
 
 class TrainParamsC(ctypes.Structure):

@@ -20,21 +20,23 @@
 namespace ii {
 
 // The epilogue (HopEpilogue) of one site's output o of rhs-offset accessors
-// dr (the dv field) and outr; returns the site's partial sum.  16- and
-// 32-bit fields: the 24-term site sums in FP32, the sums over sites in FP64.
+// dr (the dv field) and outr; the site's partial sums go to ds[0..2].  xs
+// holds the XPAY term the kernel added (kEpiTS).  16- and 32-bit fields: the
+// 24-term site sums in FP32, the sums over sites in FP64.
 template <int EPI, typename R, typename C, typename SpX, typename StOut>
-__device__ __forceinline__ double hop_epilogue(C o[12], int hs, const SpX& dr, const StOut& outr,
-                                               double alpha) {
-  double dsum = 0.0;
+__device__ __forceinline__ void hop_epilogue(C o[12], const C* xs, int hs, const SpX& dr,
+                                             const StOut& outr, double alpha, double ds[3]) {
   constexpr bool kF64 = std::is_same<R, double>::value;
   if constexpr (EPI == kEpiDot) {
     if constexpr (kF64) {
-      const double ds = (double)dr.scale(hs);
+      const double sc = (double)dr.scale(hs);
+      double sum = 0.0;
 #pragma unroll
       for (int k = 0; k < 12; ++k) {
         auto dvv = dr.ld(hs, k);
-        dsum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * ds;
+        sum += ((double)dvv.x * (double)o[k].x + (double)dvv.y * (double)o[k].y) * sc;
       }
+      ds[0] = sum;
     } else {
       float sd = 0.0f;
 #pragma unroll
@@ -43,7 +45,7 @@ __device__ __forceinline__ double hop_epilogue(C o[12], int hs, const SpX& dr, c
         sd = fmaf((float)dvv.x, o[k].x, sd);
         sd = fmaf((float)dvv.y, o[k].y, sd);
       }
-      dsum = (double)sd * (double)dr.scale(hs);
+      ds[0] = (double)sd * (double)dr.scale(hs);
     }
   } else if constexpr (EPI == kEpiNorm) {
     R sd = R(0);
@@ -52,22 +54,46 @@ __device__ __forceinline__ double hop_epilogue(C o[12], int hs, const SpX& dr, c
       sd = fma(o[k].x, o[k].x, sd);
       sd = fma(o[k].y, o[k].y, sd);
     }
-    dsum = (double)sd;
+    ds[0] = (double)sd;
   } else if constexpr (EPI == kEpiRUpd) {
-    const R ds = (R)dr.scale(hs), al = (R)alpha;
+    const R sc = (R)dr.scale(hs), al = (R)alpha;
     R sd = R(0);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
       auto dvv = dr.ld(hs, k);
-      o[k].x = fma(-al, o[k].x, (R)dvv.x * ds);
-      o[k].y = fma(-al, o[k].y, (R)dvv.y * ds);
+      o[k].x = fma(-al, o[k].x, (R)dvv.x * sc);
+      o[k].y = fma(-al, o[k].y, (R)dvv.y * sc);
       sd = fma(o[k].x, o[k].x, sd);
       sd = fma(o[k].y, o[k].y, sd);
     }
-    dsum = (double)sd;
+    ds[0] = (double)sd;
+  } else if constexpr (EPI == kEpiCDot) {
+    // sum conj(d) o: re = d.x o.x + d.y o.y, im = d.x o.y - d.y o.x
+    R re = R(0), im = R(0);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      auto dvv = dr.ld(hs, k);
+      const R dx = (R)dvv.x, dy = (R)dvv.y;
+      re = fma(dx, o[k].x, fma(dy, o[k].y, re));
+      im = fma(dx, o[k].y, fma(-dy, o[k].x, im));
+    }
+    const double sc = (double)dr.scale(hs);
+    ds[0] = (double)re * sc;
+    ds[1] = (double)im * sc;
+  } else if constexpr (EPI == kEpiTS) {
+    // sum conj(o) x and |o|^2
+    R re = R(0), im = R(0), nn = R(0);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      re = fma(o[k].x, xs[k].x, fma(o[k].y, xs[k].y, re));
+      im = fma(o[k].x, xs[k].y, fma(-o[k].y, xs[k].x, im));
+      nn = fma(o[k].x, o[k].x, fma(o[k].y, o[k].y, nn));
+    }
+    ds[0] = (double)re;
+    ds[1] = (double)im;
+    ds[2] = (double)nn;
   }
   outr.st(hs, o);
-  return dsum;
 }
 
 // BS threads per block (kBlock for the variants with partial sums, which are
@@ -86,7 +112,7 @@ k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
   const int hs = base + threadIdx.x;
   const int rhs = blockIdx.y;
   if (skip && skip[rhs * skip_stride]) return;  // uniform per block
-  double dsum = 0.0;
+  double dsum[3] = {0.0, 0.0, 0.0};
   if (hs < g.Vh) {
     const SpIn inr = in.rhs(rhs);
     C acc[4][3];
@@ -98,23 +124,29 @@ k_hop(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
 #pragma unroll
       for (int c = 0; c < 3; ++c) o[3 * so + c] = cscale(acc[s][c], a);
     }
+    C xsv[12];
     if (XPAY) {
       const SpX xr = x.rhs(rhs);
       const auto xs = xr.scale(hs);
 #pragma unroll
       for (int k = 0; k < 12; ++k) {
         auto xv = xr.ld(hs, k);
-        o[k].x += (typename SpIn::R)(xv.x * xs);
-        o[k].y += (typename SpIn::R)(xv.y * xs);
+        xsv[k].x = (typename SpIn::R)(xv.x * xs);
+        xsv[k].y = (typename SpIn::R)(xv.y * xs);
+        o[k].x += xsv[k].x;
+        o[k].y += xsv[k].y;
       }
     }
-    dsum = hop_epilogue<EPI, typename SpIn::R>(o, hs, dv.rhs(rhs), out.rhs(rhs),
-                                               EPI == kEpiRUpd ? coef[rhs * coef_stride] : 0.0);
+    hop_epilogue<EPI, typename SpIn::R>(o, xsv, hs, dv.rhs(rhs), out.rhs(rhs),
+                                        EPI == kEpiRUpd ? coef[rhs * coef_stride] : 0.0, dsum);
   }
-  if (EPI != kEpiStore) {
-    double s = block_sum(dsum);
+  constexpr int NS = epi_sums(EPI);
+  const size_t plane = (size_t)gridDim.y * nblk;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    double s = block_sum(dsum[j]);
     // indexed by the site range, so sums do not depend on the traversal
-    if (threadIdx.x == 0) partials[(size_t)rhs * nblk + base / kBlock] = s;
+    if (threadIdx.x == 0) partials[j * plane + (size_t)rhs * nblk + base / kBlock] = s;
   }
 }
 
@@ -192,7 +224,7 @@ k_hop_mrhs(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
   static_assert(TS == 64, "one wavefront per rhs row");
   const int rhs = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + ty);
   const bool live = rhs < nrhs && !(skip && skip[rhs * skip_stride]);
-  double dsum = 0.0;
+  double dsum[3] = {0.0, 0.0, 0.0};
   if (live && hs < g.Vh) {
     const SpIn inr = in.rhs(rhs);
     C acc[4][3];
@@ -242,22 +274,29 @@ k_hop_mrhs(Geom g, SpIn in, Gauge gauge, StOut out, SpX x, SpX dv, int out_par,
 #pragma unroll
       for (int c = 0; c < 3; ++c) o[3 * so + c] = cscale(acc[s][c], a);
     }
+    C xsv[12];
     if (XPAY) {
       const SpX xr = x.rhs(rhs);
       const auto xs = xr.scale(hs);
 #pragma unroll
       for (int k = 0; k < 12; ++k) {
         auto xv = xr.ld(hs, k);
-        o[k].x += (typename SpIn::R)(xv.x * xs);
-        o[k].y += (typename SpIn::R)(xv.y * xs);
+        xsv[k].x = (typename SpIn::R)(xv.x * xs);
+        xsv[k].y = (typename SpIn::R)(xv.y * xs);
+        o[k].x += xsv[k].x;
+        o[k].y += xsv[k].y;
       }
     }
-    dsum = hop_epilogue<EPI, typename SpIn::R>(o, hs, dv.rhs(rhs), out.rhs(rhs),
-                                               EPI == kEpiRUpd ? coef[rhs * coef_stride] : 0.0);
+    hop_epilogue<EPI, typename SpIn::R>(o, xsv, hs, dv.rhs(rhs), out.rhs(rhs),
+                                        EPI == kEpiRUpd ? coef[rhs * coef_stride] : 0.0, dsum);
   }
-  if (EPI != kEpiStore) {
-    for (int off = TS / 2; off > 0; off >>= 1) dsum += __shfl_down(dsum, off, TS);
-    if (tx == 0 && live) partials[(size_t)rhs * ntile + blockIdx.x] = dsum;
+  constexpr int NS = epi_sums(EPI);
+  const size_t plane = (size_t)nrhs * ntile;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    double v = dsum[j];
+    for (int off = TS / 2; off > 0; off >>= 1) v += __shfl_down(v, off, TS);
+    if (tx == 0 && live) partials[j * plane + (size_t)rhs * ntile + blockIdx.x] = v;
   }
 }
 
@@ -324,13 +363,15 @@ ii_status dispatch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, cons
                        const SpX& x, const SpX& dv, const HopSpec& h) {
   using R = typename SpIn::R;
   const R a = (R)h.a;
-  if (h.epi < kEpiStore || h.epi > kEpiRUpd)
+  if (h.epi < kEpiStore || h.epi > kEpiTS)
     return set_error(ctx, II_ERR_BAD_ARG, "unknown hop epilogue");
+  if (h.epi == kEpiTS && !h.xpay)
+    return set_error(ctx, II_ERR_BAD_ARG, "kEpiTS needs the XPAY term");
   if (h.epi == kEpiRUpd && !h.coef)
     return set_error(ctx, II_ERR_BAD_ARG, "residual-update hop without its scalars");
-  const int sel = (h.g5in ? 16 : 0) | (h.g5out ? 8 : 0) | (h.xpay ? 4 : 0) | h.epi;
+  const int sel = (h.g5in ? 32 : 0) | (h.g5out ? 16 : 0) | (h.xpay ? 8 : 0) | h.epi;
 #define II_CASE(A, B, CC, E)                                                                 \
-  case ((A) ? 16 : 0) | ((B) ? 8 : 0) | ((CC) ? 4 : 0) | (E):                                \
+  case ((A) ? 32 : 0) | ((B) ? 16 : 0) | ((CC) ? 8 : 0) | (E):                               \
     return launch_hop<A, B, CC, E>(ctx, in, gauge, out, x, dv, a, h);
   switch (sel) {
     II_CASE(false, false, false, kEpiStore)
@@ -338,6 +379,8 @@ ii_status dispatch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, cons
     II_CASE(false, false, true, kEpiDot)
     II_CASE(false, false, true, kEpiNorm)
     II_CASE(false, false, true, kEpiRUpd)
+    II_CASE(false, false, true, kEpiCDot)
+    II_CASE(false, false, true, kEpiTS)
     II_CASE(true, true, false, kEpiStore)
     II_CASE(true, true, true, kEpiStore)
     II_CASE(true, true, true, kEpiDot)

@@ -17,7 +17,16 @@ namespace ii {
 //   kEpiRUpd   r = dv - alpha o, alpha = coef[rhs * coef_stride] (a device
 //              scalar); store r in place of o; partial sums of |r|^2
 //              (the CG residual update fused into the last hop of A p)
-enum HopEpilogue { kEpiStore = 0, kEpiDot = 1, kEpiNorm = 2, kEpiRUpd = 3 };
+//   kEpiCDot   store o; partial sums of the complex <dv, o> = sum conj(dv) o
+//              (re, im)                             (BiCGStab <r0, A p>)
+//   kEpiTS     XPAY only: store o; partial sums of the complex <o, x> and of
+//              |o|^2 from the x it adds             (BiCGStab omega = <t, s>/|t|^2)
+// Several sums are stored as planes: sum j of (rhs, block) at
+// partials[j * plane + rhs * nblocks + block], plane = nrhs * nblocks.
+enum HopEpilogue {
+  kEpiStore = 0, kEpiDot = 1, kEpiNorm = 2, kEpiRUpd = 3, kEpiCDot = 4, kEpiTS = 5
+};
+constexpr int epi_sums(int epi) { return epi == kEpiStore ? 0 : epi == kEpiCDot ? 2 : epi == kEpiTS ? 3 : 1; }
 
 struct HopSpec {
   int out_par = 0;

@@ -1700,6 +1700,523 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
   return II_OK;
 }
 
+// ============================================================ BiCGStab
+// Batched mixed-precision BiCGStab on Dhat (van der Vorst; no reference
+// counterpart: the reference solves Dhat^dag Dhat with Solver_CG).  Per
+// iteration and rhs, with complex scalars (Dhat is not Hermitian):
+//   v = Dhat p, <r0, v>         (hops; kEpiCDot)     alpha = rho / <r0, v>
+//   s = r - alpha v
+//   t = Dhat s, <t, s>, |t|^2   (hops; kEpiTS)       omega = <t, s> / |t|^2
+//   y += alpha p + omega s ; r = s - omega t ; <r0, r>, |r|^2
+//   beta = (<r0, r> / rho)(alpha / omega) ; rho = <r0, r> ; p = r + beta (p - omega v)
+// with FP64 reliable updates as in the CG solve: fold y into the FP64
+// solution, rho_true = bhat - Dhat y in FP64, r = rho_true; if the true
+// residual is far above the recursive one, restart r0 = p = r.
+struct BiCoef {
+  double rho[2], alpha[2], omega[2], beta[2];  // complex (re, im)
+};
+
+__device__ __forceinline__ void cdiv(double ar, double ai, double br, double bi, double& cr,
+                                     double& ci) {
+  const double d = br * br + bi * bi;
+  cr = (ar * br + ai * bi) / d;
+  ci = (ai * br - ar * bi) / d;
+}
+// sum over i < nb of plane j of rhs blockIdx.x
+__device__ __forceinline__ double plane_sum(const double* partials, size_t pl, int j, int nb) {
+  const double* p = partials + j * pl + (size_t)blockIdx.x * nb;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[i];
+  return block_sum(s);
+}
+
+__global__ void k_bi_init(const RUScal* __restrict__ sc, BiCoef* __restrict__ bc) {
+  if (threadIdx.x) return;
+  BiCoef z{};
+  z.rho[0] = sc[blockIdx.x].rr;  // r0 = r
+  bc[blockIdx.x] = z;
+}
+// alpha = rho / <r0, v>
+__global__ void k_bi_alpha(const double* __restrict__ partials, int nb, size_t pl,
+                           RUScal* __restrict__ sc, BiCoef* __restrict__ bc) {
+  RUScal& S = sc[blockIdx.x];
+  if (S.done) return;
+  const double re = plane_sum(partials, pl, 0, nb);
+  const double im = plane_sum(partials, pl, 1, nb);
+  if (threadIdx.x == 0) {
+    BiCoef& B = bc[blockIdx.x];
+    cdiv(B.rho[0], B.rho[1], re, im, B.alpha[0], B.alpha[1]);
+    if (!finite(B.alpha[0]) || !finite(B.alpha[1])) { S.nan = 1; S.done = 1; S.skip_ru = 1; }
+  }
+}
+// omega = <t, s> / |t|^2
+__global__ void k_bi_omega(const double* __restrict__ partials, int nb, size_t pl,
+                           RUScal* __restrict__ sc, BiCoef* __restrict__ bc) {
+  RUScal& S = sc[blockIdx.x];
+  if (S.done) return;
+  const double re = plane_sum(partials, pl, 0, nb);
+  const double im = plane_sum(partials, pl, 1, nb);
+  const double tt = plane_sum(partials, pl, 2, nb);
+  if (threadIdx.x == 0) {
+    BiCoef& B = bc[blockIdx.x];
+    B.omega[0] = re / tt;
+    B.omega[1] = im / tt;
+    if (!finite(B.omega[0]) || !finite(B.omega[1])) { S.nan = 1; S.done = 1; S.skip_ru = 1; }
+  }
+}
+// after the r update: beta, rho, |r|^2 and the reliable-update decision
+__global__ void k_bi_beta(const double* __restrict__ partials, int nb, size_t pl,
+                          RUScal* __restrict__ sc, BiCoef* __restrict__ bc) {
+  RUScal& S = sc[blockIdx.x];
+  if (S.done) return;
+  const double re = plane_sum(partials, pl, 0, nb);
+  const double im = plane_sum(partials, pl, 1, nb);
+  const double rr = plane_sum(partials, pl, 2, nb);
+  if (threadIdx.x == 0) {
+    BiCoef& B = bc[blockIdx.x];
+    double qr, qi, ar, ai;
+    cdiv(re, im, B.rho[0], B.rho[1], qr, qi);                    // rho_new / rho
+    cdiv(B.alpha[0], B.alpha[1], B.omega[0], B.omega[1], ar, ai);  // alpha / omega
+    B.beta[0] = qr * ar - qi * ai;
+    B.beta[1] = qr * ai + qi * ar;
+    B.rho[0] = re;
+    B.rho[1] = im;
+    S.rr = rr;
+    S.count += 1;
+    if (!finite(rr) || !finite(B.beta[0]) || !finite(B.beta[1])) {
+      S.nan = 1; S.done = 1; S.skip_ru = 1;
+      return;
+    }
+    const bool upd = (rr < S.delta2 * S.rmax2) || (rr < S.target2);
+    S.skip_ru = upd ? 0 : 1;
+  }
+}
+// after a reliable update: |r|^2 and rho = <r0, r> of the new residual; a
+// true residual far above the recursive one restarts r0 = p = r (keep_p = 0)
+__global__ void k_bi_ru(const double* __restrict__ partials, int nb, size_t pl,
+                        RUScal* __restrict__ sc, BiCoef* __restrict__ bc) {
+  RUScal& S = sc[blockIdx.x];
+  if (S.skip_ru) return;
+  const double re = plane_sum(partials, pl, 0, nb);
+  const double im = plane_sum(partials, pl, 1, nb);
+  const double rr = plane_sum(partials, pl, 2, nb);
+  if (threadIdx.x == 0) {
+    BiCoef& B = bc[blockIdx.x];
+    if (rr > 16.0 * S.rr) {
+      S.keep_p = 0;
+      S.restarts += 1;
+      B.rho[0] = rr;
+      B.rho[1] = 0.0;
+    } else {
+      B.rho[0] = re;
+      B.rho[1] = im;
+    }
+    S.rr = rr;
+    S.rmax2 = rr;
+    // r is the (sloppy) true residual itself: test again below 0.81 tol^2
+    const double t = 0.81 * S.tol2;
+    S.target2 = t < S.target2 ? t : S.target2;
+    S.updates += 1;
+    S.skip_ru = 1;
+    if (!finite(rr) || !finite(B.rho[0])) { S.nan = 1; S.done = 1; }
+  }
+}
+
+// s = r - alpha v
+template <typename Sp, typename St>
+__global__ void __launch_bounds__(kBlock)
+k_bi_s(Geom g, Sp r, Sp v, St s, const RUScal* __restrict__ sc, const BiCoef* __restrict__ bc) {
+  const int rhs = blockIdx.y;
+  if (sc[rhs].done) return;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (hs >= g.Vh) return;
+  const double ar = bc[rhs].alpha[0], ai = bc[rhs].alpha[1];
+  double rv[24], vv[24];
+  ld_site(r.rhs(rhs), hs, rv);
+  ld_site(v.rhs(rhs), hs, vv);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const double x = vv[2 * k], y = vv[2 * k + 1];
+    rv[2 * k] -= ar * x - ai * y;
+    rv[2 * k + 1] -= ar * y + ai * x;
+  }
+  st_site(s.rhs(rhs), hs, rv);
+}
+// y += alpha p + omega s ; r = s - omega t ; partial <r0, r> (re, im), |r|^2
+template <typename Sp, typename St, typename SpY, typename StY>
+__global__ void __launch_bounds__(kBlock)
+k_bi_xr(Geom g, Sp p, Sp s, Sp t, Sp r0, St rw, SpY yr, StY yw, const RUScal* __restrict__ sc,
+        const BiCoef* __restrict__ bc, double* __restrict__ partials) {
+  const int rhs = blockIdx.y;
+  if (sc[rhs].done) return;
+  const double ar = bc[rhs].alpha[0], ai = bc[rhs].alpha[1];
+  const double wr = bc[rhs].omega[0], wi = bc[rhs].omega[1];
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  if (hs < g.Vh) {
+    double pv[24], sv[24], yv[24];
+    ld_site(p.rhs(rhs), hs, pv);
+    ld_site(s.rhs(rhs), hs, sv);
+    ld_site(yr.rhs(rhs), hs, yv);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      yv[2 * k] += ar * pv[2 * k] - ai * pv[2 * k + 1] + wr * sv[2 * k] - wi * sv[2 * k + 1];
+      yv[2 * k + 1] += ar * pv[2 * k + 1] + ai * pv[2 * k] + wr * sv[2 * k + 1] + wi * sv[2 * k];
+    }
+    st_site(yw.rhs(rhs), hs, yv);
+    double tv[24];
+    ld_site(t.rhs(rhs), hs, tv);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      sv[2 * k] -= wr * tv[2 * k] - wi * tv[2 * k + 1];
+      sv[2 * k + 1] -= wr * tv[2 * k + 1] + wi * tv[2 * k];
+    }
+    st_site(rw.rhs(rhs), hs, sv);
+    ld_site(r0.rhs(rhs), hs, pv);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const double x = pv[2 * k], y = pv[2 * k + 1], u = sv[2 * k], w = sv[2 * k + 1];
+      d0 += x * u + y * w;
+      d1 += x * w - y * u;
+      d2 += u * u + w * w;
+    }
+  }
+  const size_t pl = (size_t)gridDim.y * gridDim.x, o = (size_t)rhs * gridDim.x + blockIdx.x;
+  d0 = block_sum(d0);
+  if (threadIdx.x == 0) partials[o] = d0;
+  d1 = block_sum(d1);
+  if (threadIdx.x == 0) partials[pl + o] = d1;
+  d2 = block_sum(d2);
+  if (threadIdx.x == 0) partials[2 * pl + o] = d2;
+}
+// p = r + beta (p - omega v)
+template <typename Sp, typename St>
+__global__ void __launch_bounds__(kBlock)
+k_bi_p(Geom g, Sp r, Sp p, Sp v, St pw, const RUScal* __restrict__ sc,
+       const BiCoef* __restrict__ bc) {
+  const int rhs = blockIdx.y;
+  if (sc[rhs].done) return;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (hs >= g.Vh) return;
+  const double br = bc[rhs].beta[0], bi = bc[rhs].beta[1];
+  const double wr = bc[rhs].omega[0], wi = bc[rhs].omega[1];
+  double rv[24], pv[24], vv[24];
+  ld_site(r.rhs(rhs), hs, rv);
+  ld_site(p.rhs(rhs), hs, pv);
+  ld_site(v.rhs(rhs), hs, vv);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const double qx = pv[2 * k] - (wr * vv[2 * k] - wi * vv[2 * k + 1]);
+    const double qy = pv[2 * k + 1] - (wr * vv[2 * k + 1] + wi * vv[2 * k]);
+    rv[2 * k] += br * qx - bi * qy;
+    rv[2 * k + 1] += br * qy + bi * qx;
+  }
+  st_site(pw.rhs(rhs), hs, rv);
+}
+// <r0, r> (re, im) and |r|^2 partials, masked by skip
+template <typename Sp>
+__global__ void __launch_bounds__(kBlock)
+k_bi_rdots(Geom g, Sp r0, Sp r, const int* __restrict__ skip, int skip_stride,
+           double* __restrict__ partials) {
+  const int rhs = blockIdx.y;
+  if (skip && skip[rhs * skip_stride]) return;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  if (hs < g.Vh) {
+    double a[24], b[24];
+    ld_site(r0.rhs(rhs), hs, a);
+    ld_site(r.rhs(rhs), hs, b);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const double x = a[2 * k], y = a[2 * k + 1], u = b[2 * k], w = b[2 * k + 1];
+      d0 += x * u + y * w;
+      d1 += x * w - y * u;
+      d2 += u * u + w * w;
+    }
+  }
+  const size_t pl = (size_t)gridDim.y * gridDim.x, o = (size_t)rhs * gridDim.x + blockIdx.x;
+  d0 = block_sum(d0);
+  if (threadIdx.x == 0) partials[o] = d0;
+  d1 = block_sum(d1);
+  if (threadIdx.x == 0) partials[pl + o] = d1;
+  d2 = block_sum(d2);
+  if (threadIdx.x == 0) partials[2 * pl + o] = d2;
+}
+
+template <int P>
+struct BiBlas {
+  using A = Acc<P>;
+  static ii_status s(ii_context* ctx, const SF& r, const SF& v, const SF& so, const RUScal* sc,
+                     const BiCoef* bc, int nrhs) {
+    const Geom& g = ctx->g;
+    hipLaunchKernelGGL((k_bi_s<typename A::Sp, typename A::St>), dim3(blocks_for(g.Vh), nrhs),
+                       dim3(kBlock), 0, ctx->stream, g, A::sp(r, g.stride), A::sp(v, g.stride),
+                       A::st(so, g.stride), sc, bc);
+    II_HIP(ctx, hipGetLastError());
+    return II_OK;
+  }
+  static ii_status xr(ii_context* ctx, const SF& p, const SF& s, const SF& t, const SF& r0,
+                      const SF& r, const SF& y, const RUScal* sc, const BiCoef* bc, int nrhs,
+                      double* partials) {
+    const Geom& g = ctx->g;
+    using Y = Acc<II_PREC_SINGLE>;
+    hipLaunchKernelGGL((k_bi_xr<typename A::Sp, typename A::St, typename Y::Sp, typename Y::St>),
+                       dim3(blocks_for(g.Vh), nrhs), dim3(kBlock), 0, ctx->stream, g,
+                       A::sp(p, g.stride), A::sp(s, g.stride), A::sp(t, g.stride),
+                       A::sp(r0, g.stride), A::st(r, g.stride), Y::sp(y, g.stride),
+                       Y::st(y, g.stride), sc, bc, partials);
+    II_HIP(ctx, hipGetLastError());
+    return II_OK;
+  }
+  static ii_status p(ii_context* ctx, const SF& r, const SF& pin, const SF& v, const RUScal* sc,
+                     const BiCoef* bc, int nrhs) {
+    const Geom& g = ctx->g;
+    hipLaunchKernelGGL((k_bi_p<typename A::Sp, typename A::St>), dim3(blocks_for(g.Vh), nrhs),
+                       dim3(kBlock), 0, ctx->stream, g, A::sp(r, g.stride), A::sp(pin, g.stride),
+                       A::sp(v, g.stride), A::st(pin, g.stride), sc, bc);
+    II_HIP(ctx, hipGetLastError());
+    return II_OK;
+  }
+  static ii_status rdots(ii_context* ctx, const SF& r0, const SF& r, const int* skip, int ss,
+                         int nrhs, double* partials) {
+    const Geom& g = ctx->g;
+    hipLaunchKernelGGL((k_bi_rdots<typename A::Sp>), dim3(blocks_for(g.Vh), nrhs), dim3(kBlock),
+                       0, ctx->stream, g, A::sp(r0, g.stride), A::sp(r, g.stride), skip, ss,
+                       partials);
+    II_HIP(ctx, hipGetLastError());
+    return II_OK;
+  }
+};
+
+template <int P>
+ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_params* prm,
+                                const ii_field* src, ii_field* sol, ii_solver_output* outs) {
+  const Geom& g = ctx->g;
+  using BB = BiBlas<P>;
+  const int nrhs_all = src->nrhs;
+  if (!ctx->gauge_lp[P]) II_CHECK(build_lowprec_gauge(ctx, P));
+  const int batch = prm->batched ? nrhs_all : 1;
+  const double tol = prm->tol > 0 ? prm->tol : 1e-10;
+  // BiCGStab drops ~3 decades in 10 iterations: a reliable update every
+  // decade (0.1, the CG default) costs more than the FP16 drift it removes;
+  // at 32^3x64 0.1 / 0.03 / 0.01 / 0.001 gave 8 / 6 / 4 / 4 updates in
+  // 28 / 28 / 30 / 36 iterations, 0.216 / 0.204 / 0.197 / 0.218 s
+  const double delta = prm->delta > 0 ? prm->delta : 0.01;
+  const int maxiter = prm->maxiter > 0 ? prm->maxiter : 100000;
+  const size_t he = half_elems(g);
+  const int nb = hop_blocks(g);                   // BLAS partials per rhs
+  const int nh = hop_dot_parts(g, batch, P);      // hop partials per rhs
+  const int nhd = hop_dot_parts(g, batch, II_PREC_DOUBLE);
+  II_CHECK(ensure_partials(ctx, (size_t)3 * std::max(std::max(nb, nh), nhd) * batch + 16));
+  II_CHECK(ensure_ru_scalars(ctx, batch));
+  double* part = ctx->partials;
+  RUScal* sc = dev_scal(ctx);
+  const int ss = kScalInts;
+  const int* skip_it = &sc[0].done;
+  const int* skip_ru = &sc[0].skip_ru;
+  void* v = nullptr;
+  II_CHECK(workspace(ctx, 40, sizeof(double) * 3 * batch + 64, &v));
+  double* aux = (double*)v;
+  double* one = aux + 3 * batch;
+  II_CHECK(workspace(ctx, 63, sizeof(BiCoef) * batch, &v));
+  BiCoef* bc = (BiCoef*)v;
+  SF be, bo, bh, y, q64, t64;
+  II_CHECK(fieldP(ctx, 41, 0, batch, &be));
+  II_CHECK(fieldP(ctx, 42, 0, batch, &bo));
+  II_CHECK(fieldP(ctx, 43, 0, batch, &bh));
+  II_CHECK(fieldP(ctx, 44, 0, batch, &y));
+  II_CHECK(fieldP(ctx, 46, 0, batch, &q64));
+  II_CHECK(fieldP(ctx, 47, 0, batch, &t64));
+  SF p, r, r0, vv, w, s, t, ys;
+  II_CHECK(fieldP(ctx, 50, P, batch, &p));
+  II_CHECK(fieldP(ctx, 51, P, batch, &r));
+  II_CHECK(fieldP(ctx, 52, P, batch, &r0));
+  II_CHECK(fieldP(ctx, 53, P, batch, &vv));
+  II_CHECK(fieldP(ctx, 54, P, batch, &w));
+  II_CHECK(fieldP(ctx, 72, P, batch, &s));
+  II_CHECK(fieldP(ctx, 73, P, batch, &t));
+  II_CHECK(fieldP(ctx, 55, II_PREC_SINGLE, batch, &ys));
+  HopSpec h;
+  for (int rb0 = 0; rb0 < nrhs_all; rb0 += batch) {
+    const int nr = std::min(batch, nrhs_all - rb0);
+    const size_t plh = (size_t)nr * nh, plb = (size_t)nr * nb;
+    II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (int k = 0; k < nr; ++k) {
+      const double2* base = src->data + (size_t)(rb0 + k) * 2 * he;
+      II_CHECK(copy64(ctx, (double2*)be.v + k * he, base));
+      II_CHECK(copy64(ctx, (double2*)bo.v + k * he, base + he));
+    }
+    II_CHECK(dot64(ctx, (double2*)be.v, (double2*)be.v, aux, nr));
+    II_CHECK(dot64(ctx, (double2*)bo.v, (double2*)bo.v, aux + batch, nr));
+    // bhat = b_e + kappa H_eo b_o   (eoUtils.cpp:54-57)
+    h = HopSpec();
+    h.out_par = 0;
+    h.a = kappa;
+    h.xpay = true;
+    h.nrhs = nr;
+    II_CHECK(hop64(ctx, h, (double2*)bo.v, (double2*)bh.v, (double2*)be.v, nullptr));
+    II_CHECK(zero64(ctx, (double2*)y.v, nr));
+    II_CHECK(dot64(ctx, (double2*)bh.v, (double2*)bh.v, aux + 2 * batch, nr));
+    {
+      std::vector<double> hb(2 * batch);
+      II_HIP(ctx, hipMemcpyAsync(hb.data(), aux, 2 * batch * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+      II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      for (int k = 0; k < nr; ++k) hb[k] += hb[batch + k];
+      II_HIP(ctx, hipMemcpyAsync(aux, hb.data(), nr * sizeof(double), hipMemcpyHostToDevice,
+                                 ctx->stream));
+      const double unit = 1.0;
+      II_HIP(ctx, hipMemcpyAsync(one, &unit, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    hipLaunchKernelGGL(k_scal_init, dim3(nr), dim3(64), 0, ctx->stream, sc, aux + 2 * batch, aux,
+                       0.0, tol, delta, 1);
+    hipLaunchKernelGGL(k_bi_init, dim3(nr), dim3(64), 0, ctx->stream, sc, bc);
+    II_HIP(ctx, hipGetLastError());
+    // r = r0 = p = bhat (x0 = 0), ys = 0
+    II_CHECK(convertP(ctx, 0, P, bh, r, nr));
+    II_CHECK(convertP(ctx, P, P, r, r0, nr));
+    II_CHECK(convertP(ctx, P, P, r, p, nr));
+    II_CHECK(convertP(ctx, 0, II_PREC_SINGLE, y, ys, nr));  // zeros
+    auto iteration = [&]() -> ii_status {
+      HopSpec a;
+      a.nrhs = nr;
+      a.skip = skip_it;
+      a.skip_stride = ss;
+      a.out_par = 1;
+      a.a = 1.0;
+      II_CHECK(hopP(ctx, P, a, p, w, nullptr, nullptr));
+      a.out_par = 0;
+      a.a = -kappa * kappa;
+      a.xpay = true;
+      a.epi = kEpiCDot;
+      a.partials = part;
+      II_CHECK(hopP(ctx, P, a, w, vv, &p, &r0));  // v = Dhat p, <r0, v>
+      hipLaunchKernelGGL(k_bi_alpha, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nh, plh, sc,
+                         bc);
+      II_CHECK(BB::s(ctx, r, vv, s, sc, bc, nr));
+      a.out_par = 1;
+      a.a = 1.0;
+      a.xpay = false;
+      a.epi = kEpiStore;
+      II_CHECK(hopP(ctx, P, a, s, w, nullptr, nullptr));
+      a.out_par = 0;
+      a.a = -kappa * kappa;
+      a.xpay = true;
+      a.epi = kEpiTS;
+      II_CHECK(hopP(ctx, P, a, w, t, &s, nullptr));  // t = Dhat s, <t, s>, |t|^2
+      hipLaunchKernelGGL(k_bi_omega, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nh, plh, sc,
+                         bc);
+      II_CHECK(BB::xr(ctx, p, s, t, r0, r, ys, sc, bc, nr, part));
+      hipLaunchKernelGGL(k_bi_beta, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, plb, sc,
+                         bc);
+      return BB::p(ctx, r, p, vv, sc, bc, nr);
+    };
+    int its = 0, chunk = 1;
+    bool all_done = false;
+    while (its < maxiter && !all_done) {
+      for (int i = 0; i < chunk; ++i) II_CHECK(iteration());
+      its += chunk;
+      II_CHECK(read_scalars(ctx, nr));
+      RUScal* H = host_scal(ctx);
+      bool any_upd = false;
+      all_done = true;
+      for (int k = 0; k < nr; ++k) {
+        if (H[k].nan)
+          return set_error(ctx, II_ERR_NAN,
+                           "non-finite BiCGStab scalar (rhs " + std::to_string(rb0 + k) +
+                               ", iteration " + std::to_string(H[k].count) + ")");
+        if (!H[k].done) all_done = false;
+        if (!H[k].done && !H[k].skip_ru) any_upd = true;
+      }
+      if (all_done) break;
+      if (!any_upd) {
+        chunk = std::min(chunk * 2, 8);
+        continue;
+      }
+      chunk = 1;
+      // reliable update for the flagged rhs: y64 += ys, ys = 0
+      hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(g.Vh), nr), dim3(kBlock), 0,
+                         ctx->stream, g, Acc<1>::sp(ys, g.stride), Acc<1>::st(ys, g.stride),
+                         (double2*)y.v, skip_ru, ss);
+      II_HIP(ctx, hipGetLastError());
+      // rho_true = bhat - Dhat y (FP64) and |rho_true|^2
+      h = HopSpec();
+      h.nrhs = nr;
+      h.skip = skip_ru;
+      h.skip_stride = ss;
+      h.out_par = 1;
+      h.a = 1.0;
+      II_CHECK(hop64(ctx, h, (double2*)y.v, (double2*)t64.v, nullptr, nullptr));
+      h.out_par = 0;
+      h.a = -kappa * kappa;
+      h.xpay = true;
+      h.epi = kEpiRUpd;
+      h.coef = one;
+      h.coef_stride = 0;
+      h.partials = part;
+      II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)q64.v, (double2*)y.v,
+                     (const double2*)bh.v));
+      hipLaunchKernelGGL(k_fin_rho, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nhd, sc);
+      // r = rho_true in the sloppy precision; rho = <r0, r>; restart if broken
+      II_CHECK(convertP(ctx, 0, P, q64, r, nr, skip_ru, ss));
+      II_CHECK(BB::rdots(ctx, r0, r, skip_ru, ss, nr, part));
+      hipLaunchKernelGGL(k_bi_ru, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, plb, sc, bc);
+      II_HIP(ctx, hipGetLastError());
+      II_CHECK(convertP(ctx, P, P, r, r0, nr, &sc[0].keep_p, ss));
+      II_CHECK(convertP(ctx, P, P, r, p, nr, &sc[0].keep_p, ss));
+    }
+    hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(g.Vh), nr), dim3(kBlock), 0,
+                       ctx->stream, g, Acc<1>::sp(ys, g.stride), Acc<1>::st(ys, g.stride),
+                       (double2*)y.v, nullptr, 0);
+    II_HIP(ctx, hipGetLastError());
+    // x_o = b_o + kappa H_oe y   (eoUtils.cpp:67)
+    h = HopSpec();
+    h.out_par = 1;
+    h.a = kappa;
+    h.xpay = true;
+    h.nrhs = nr;
+    II_CHECK(hop64(ctx, h, (double2*)y.v, (double2*)t64.v, (double2*)bo.v, nullptr));
+    for (int k = 0; k < nr; ++k) {
+      double2* base = sol->data + (size_t)(rb0 + k) * 2 * he;
+      II_CHECK(copy64(ctx, base, (double2*)y.v + k * he));
+      II_CHECK(copy64(ctx, base + he, (double2*)t64.v + k * he));
+    }
+    II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    II_CHECK(read_scalars(ctx, nr));
+    float ms = 0.f;
+    II_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    RUScal* H = host_scal(ctx);
+    bool failed = false;
+    for (int k = 0; k < nr; ++k) {
+      ii_solver_output& o = outs[rb0 + k];
+      o.iterations = H[k].count;
+      o.inner_iterations = H[k].count;
+      o.restarts = H[k].updates;
+      o.diff = std::sqrt(H[k].rho2 / H[k].b2);
+      o.timing_ms = ms;
+      o.true_residual = -1.0;
+      if (!H[k].done) failed = true;
+    }
+    if (failed)
+      return set_error(ctx, II_ERR_NOT_CONVERGED, "BiCGStab solve did not reach the tolerance");
+  }
+  return II_OK;
+}
+
+ii_status propagator_bicgstab(ii_context* ctx, double kappa, const ii_prop_params* prm,
+                              const ii_field* src, ii_field* sol, ii_solver_output* outs) {
+  switch (prm->precision) {
+    case II_PREC_SINGLE: return propagator_bicgstab_t<1>(ctx, kappa, prm, src, sol, outs);
+    case II_PREC_HALF: return propagator_bicgstab_t<2>(ctx, kappa, prm, src, sol, outs);
+    case II_PREC_BF16: return propagator_bicgstab_t<3>(ctx, kappa, prm, src, sol, outs);
+    default:
+      return set_error(ctx, II_ERR_BAD_ARG,
+                       "BiCGStab runs in a sloppy precision (single, half, bf16) with FP64 "
+                       "reliable updates");
+  }
+}
+
 // true residual |b - D x|/|b| per rhs of full fields (FP64)
 ii_status full_residual(ii_context* ctx, double kappa, const ii_field* src, const ii_field* sol,
                         double* res) {
@@ -1985,8 +2502,12 @@ ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_param
   ii_status st;
   if (p->use_learned) {
     st = learned_propagator(ctx, kappa, p, src, sol, out);
-  } else {
+  } else if (p->method == II_METHOD_BICGSTAB) {
+    st = propagator_bicgstab(ctx, kappa, p, src, sol, out);
+  } else if (p->method == II_METHOD_CG) {
     st = propagator(ctx, kappa, p, src, sol, out);
+  } else {
+    return set_error(ctx, II_ERR_BAD_ARG, "unknown propagator method");
   }
   if (st != II_OK && st != II_ERR_NOT_CONVERGED) return st;
   std::string msg = ctx->err;
