@@ -88,7 +88,7 @@ struct ii_context {
   int tbc_sign = 1;               // sign of the t-links on the last time slice (r12)
   bool r12_ok = false;
   // workspaces, by slot index (see ii::workspace)
-  static constexpr int kSlots = 80;
+  static constexpr int kSlots = 96;
   ii::Buffer work[kSlots];
   double* partials = nullptr;     // reduction partial sums
   size_t partials_len = 0;

@@ -1736,24 +1736,38 @@ __global__ void k_bi_init(const RUScal* __restrict__ sc, BiCoef* __restrict__ bc
   z.rho[0] = sc[blockIdx.x].rr;  // r0 = r
   bc[blockIdx.x] = z;
 }
-// alpha = rho / <r0, v>
+// alpha = rho / <r0, v>; kept with omega in ahist[(rhs * astride + slot) * 4
+// + 0..3] = (alpha, omega) for the deferred solution update (0 once an rhs
+// has finished)
 __global__ void k_bi_alpha(const double* __restrict__ partials, int nb, size_t pl,
-                           RUScal* __restrict__ sc, BiCoef* __restrict__ bc) {
+                           RUScal* __restrict__ sc, BiCoef* __restrict__ bc,
+                           double* __restrict__ ahist, int aidx, int astride) {
   RUScal& S = sc[blockIdx.x];
-  if (S.done) return;
+  double* ah = ahist + ((size_t)blockIdx.x * astride + aidx) * 4;
+  if (S.done) {
+    if (threadIdx.x == 0) ah[0] = ah[1] = 0.0;
+    return;
+  }
   const double re = plane_sum(partials, pl, 0, nb);
   const double im = plane_sum(partials, pl, 1, nb);
   if (threadIdx.x == 0) {
     BiCoef& B = bc[blockIdx.x];
     cdiv(B.rho[0], B.rho[1], re, im, B.alpha[0], B.alpha[1]);
     if (!finite(B.alpha[0]) || !finite(B.alpha[1])) { S.nan = 1; S.done = 1; S.skip_ru = 1; }
+    ah[0] = S.done ? 0.0 : B.alpha[0];
+    ah[1] = S.done ? 0.0 : B.alpha[1];
   }
 }
 // omega = <t, s> / |t|^2
 __global__ void k_bi_omega(const double* __restrict__ partials, int nb, size_t pl,
-                           RUScal* __restrict__ sc, BiCoef* __restrict__ bc) {
+                           RUScal* __restrict__ sc, BiCoef* __restrict__ bc,
+                           double* __restrict__ ahist, int aidx, int astride) {
   RUScal& S = sc[blockIdx.x];
-  if (S.done) return;
+  double* ah = ahist + ((size_t)blockIdx.x * astride + aidx) * 4;
+  if (S.done) {
+    if (threadIdx.x == 0) ah[0] = ah[1] = ah[2] = ah[3] = 0.0;
+    return;
+  }
   const double re = plane_sum(partials, pl, 0, nb);
   const double im = plane_sum(partials, pl, 1, nb);
   const double tt = plane_sum(partials, pl, 2, nb);
@@ -1762,6 +1776,9 @@ __global__ void k_bi_omega(const double* __restrict__ partials, int nb, size_t p
     B.omega[0] = re / tt;
     B.omega[1] = im / tt;
     if (!finite(B.omega[0]) || !finite(B.omega[1])) { S.nan = 1; S.done = 1; S.skip_ru = 1; }
+    if (S.done) ah[0] = ah[1] = 0.0;
+    ah[2] = S.done ? 0.0 : B.omega[0];
+    ah[3] = S.done ? 0.0 : B.omega[1];
   }
 }
 // after the r update: beta, rho, |r|^2 and the reliable-update decision
@@ -1842,29 +1859,20 @@ k_bi_s(Geom g, Sp r, Sp v, St s, const RUScal* __restrict__ sc, const BiCoef* __
   }
   st_site(s.rhs(rhs), hs, rv);
 }
-// y += alpha p + omega s ; r = s - omega t ; partial <r0, r> (re, im), |r|^2
-template <typename Sp, typename St, typename SpY, typename StY>
+// r = s - omega t ; partial <r0, r> (re, im), |r|^2 (the solution update
+// alpha p + omega s is deferred to k_bi_flush)
+template <typename Sp, typename St>
 __global__ void __launch_bounds__(kBlock)
-k_bi_xr(Geom g, Sp p, Sp s, Sp t, Sp r0, St rw, SpY yr, StY yw, const RUScal* __restrict__ sc,
-        const BiCoef* __restrict__ bc, double* __restrict__ partials) {
+k_bi_r(Geom g, Sp s, Sp t, Sp r0, St rw, const RUScal* __restrict__ sc,
+       const BiCoef* __restrict__ bc, double* __restrict__ partials) {
   const int rhs = blockIdx.y;
   if (sc[rhs].done) return;
-  const double ar = bc[rhs].alpha[0], ai = bc[rhs].alpha[1];
   const double wr = bc[rhs].omega[0], wi = bc[rhs].omega[1];
   const int hs = blockIdx.x * blockDim.x + threadIdx.x;
   double d0 = 0.0, d1 = 0.0, d2 = 0.0;
   if (hs < g.Vh) {
-    double pv[24], sv[24], yv[24];
-    ld_site(p.rhs(rhs), hs, pv);
+    double sv[24], tv[24];
     ld_site(s.rhs(rhs), hs, sv);
-    ld_site(yr.rhs(rhs), hs, yv);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      yv[2 * k] += ar * pv[2 * k] - ai * pv[2 * k + 1] + wr * sv[2 * k] - wi * sv[2 * k + 1];
-      yv[2 * k + 1] += ar * pv[2 * k + 1] + ai * pv[2 * k] + wr * sv[2 * k + 1] + wi * sv[2 * k];
-    }
-    st_site(yw.rhs(rhs), hs, yv);
-    double tv[24];
     ld_site(t.rhs(rhs), hs, tv);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
@@ -1872,10 +1880,10 @@ k_bi_xr(Geom g, Sp p, Sp s, Sp t, Sp r0, St rw, SpY yr, StY yw, const RUScal* __
       sv[2 * k + 1] -= wr * tv[2 * k + 1] + wi * tv[2 * k];
     }
     st_site(rw.rhs(rhs), hs, sv);
-    ld_site(r0.rhs(rhs), hs, pv);
+    ld_site(r0.rhs(rhs), hs, tv);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      const double x = pv[2 * k], y = pv[2 * k + 1], u = sv[2 * k], w = sv[2 * k + 1];
+      const double x = tv[2 * k], y = tv[2 * k + 1], u = sv[2 * k], w = sv[2 * k + 1];
       d0 += x * u + y * w;
       d1 += x * w - y * u;
       d2 += u * u + w * w;
@@ -1889,6 +1897,67 @@ k_bi_xr(Geom g, Sp p, Sp s, Sp t, Sp r0, St rw, SpY yr, StY yw, const RUScal* __
   d2 = block_sum(d2);
   if (threadIdx.x == 0) partials[2 * pl + o] = d2;
 }
+
+// Deferred BiCGStab solution update: y += sum_j (alpha_j p_j + omega_j s_j)
+// over up to kBiDefer stored iterations (coefficients from ahist as written
+// by k_bi_alpha / k_bi_omega); with y64, the rhs not masked by skip fold the
+// sum into the FP64 solution and restart y at 0 (as k_cg_flush).
+constexpr int kBiDefer = 4;
+template <typename SpP>
+struct BiDirs {
+  SpP p[kBiDefer], s[kBiDefer];
+  int slot[kBiDefer];
+  int n;
+};
+template <typename SpP, typename SpY, typename StY>
+__global__ void __launch_bounds__(kBlock)
+k_bi_flush(Geom g, BiDirs<SpP> d, const double* __restrict__ ahist, int astride, SpY yr, StY yw,
+           double2* __restrict__ y64, const int* __restrict__ skip, int skip_stride) {
+  const int rhs = blockIdx.y;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  if (hs >= g.Vh) return;
+  const bool fold = y64 && !(skip && skip[rhs * skip_stride]);
+  double yv[24];
+  ld_site(yr.rhs(rhs), hs, yv);
+#pragma unroll
+  for (int j = 0; j < kBiDefer; ++j) {
+    if (j < d.n) {
+      const double* c = ahist + ((size_t)rhs * astride + d.slot[j]) * 4;
+      const double ar = c[0], ai = c[1], wr = c[2], wi = c[3];
+      double v[24];
+      if (ar != 0.0 || ai != 0.0) {
+        ld_site(d.p[j].rhs(rhs), hs, v);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          yv[2 * k] += ar * v[2 * k] - ai * v[2 * k + 1];
+          yv[2 * k + 1] += ar * v[2 * k + 1] + ai * v[2 * k];
+        }
+      }
+      if (wr != 0.0 || wi != 0.0) {
+        ld_site(d.s[j].rhs(rhs), hs, v);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          yv[2 * k] += wr * v[2 * k] - wi * v[2 * k + 1];
+          yv[2 * k + 1] += wr * v[2 * k + 1] + wi * v[2 * k];
+        }
+      }
+    }
+  }
+  if (fold) {
+    double2* y = y64 + (size_t)rhs * 12 * g.stride;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      double2 o = y[(size_t)k * g.stride + hs];
+      o.x += yv[2 * k];
+      o.y += yv[2 * k + 1];
+      y[(size_t)k * g.stride + hs] = o;
+    }
+#pragma unroll
+    for (int k = 0; k < 24; ++k) yv[k] = 0.0;
+  }
+  st_site(yw.rhs(rhs), hs, yv);
+}
+
 // p = r + beta (p - omega v)
 template <typename Sp, typename St>
 __global__ void __launch_bounds__(kBlock)
@@ -1955,25 +2024,40 @@ struct BiBlas {
     II_HIP(ctx, hipGetLastError());
     return II_OK;
   }
-  static ii_status xr(ii_context* ctx, const SF& p, const SF& s, const SF& t, const SF& r0,
-                      const SF& r, const SF& y, const RUScal* sc, const BiCoef* bc, int nrhs,
-                      double* partials) {
+  static ii_status r(ii_context* ctx, const SF& s, const SF& t, const SF& r0, const SF& rr,
+                     const RUScal* sc, const BiCoef* bc, int nrhs, double* partials) {
     const Geom& g = ctx->g;
-    using Y = Acc<II_PREC_SINGLE>;
-    hipLaunchKernelGGL((k_bi_xr<typename A::Sp, typename A::St, typename Y::Sp, typename Y::St>),
-                       dim3(blocks_for(g.Vh), nrhs), dim3(kBlock), 0, ctx->stream, g,
-                       A::sp(p, g.stride), A::sp(s, g.stride), A::sp(t, g.stride),
-                       A::sp(r0, g.stride), A::st(r, g.stride), Y::sp(y, g.stride),
-                       Y::st(y, g.stride), sc, bc, partials);
+    hipLaunchKernelGGL((k_bi_r<typename A::Sp, typename A::St>), dim3(blocks_for(g.Vh), nrhs),
+                       dim3(kBlock), 0, ctx->stream, g, A::sp(s, g.stride), A::sp(t, g.stride),
+                       A::sp(r0, g.stride), A::st(rr, g.stride), sc, bc, partials);
     II_HIP(ctx, hipGetLastError());
     return II_OK;
   }
-  static ii_status p(ii_context* ctx, const SF& r, const SF& pin, const SF& v, const RUScal* sc,
-                     const BiCoef* bc, int nrhs) {
+  static ii_status flush(ii_context* ctx, const SF* rp, const SF* rs, const int* slots, int n,
+                         const double* ahist, int astride, const SF& y, int nrhs, double2* y64,
+                         const int* skip, int ss) {
+    const Geom& g = ctx->g;
+    using Y = Acc<II_PREC_SINGLE>;
+    BiDirs<typename A::Sp> d{};
+    d.n = n;
+    for (int j = 0; j < kBiDefer; ++j) {
+      const int sl = slots[j < n ? j : 0];
+      d.p[j] = A::sp(rp[sl], g.stride);
+      d.s[j] = A::sp(rs[sl], g.stride);
+      d.slot[j] = sl;
+    }
+    hipLaunchKernelGGL((k_bi_flush<typename A::Sp, typename Y::Sp, typename Y::St>),
+                       dim3(blocks_for(g.Vh), nrhs), dim3(kBlock), 0, ctx->stream, g, d, ahist,
+                       astride, Y::sp(y, g.stride), Y::st(y, g.stride), y64, skip, ss);
+    II_HIP(ctx, hipGetLastError());
+    return II_OK;
+  }
+  static ii_status p(ii_context* ctx, const SF& r, const SF& pin, const SF& v, const SF& pout,
+                     const RUScal* sc, const BiCoef* bc, int nrhs) {
     const Geom& g = ctx->g;
     hipLaunchKernelGGL((k_bi_p<typename A::Sp, typename A::St>), dim3(blocks_for(g.Vh), nrhs),
                        dim3(kBlock), 0, ctx->stream, g, A::sp(r, g.stride), A::sp(pin, g.stride),
-                       A::sp(v, g.stride), A::st(pin, g.stride), sc, bc);
+                       A::sp(v, g.stride), A::st(pout, g.stride), sc, bc);
     II_HIP(ctx, hipGetLastError());
     return II_OK;
   }
@@ -1999,9 +2083,10 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
   const double tol = prm->tol > 0 ? prm->tol : 1e-10;
   // BiCGStab drops ~3 decades in 10 iterations: a reliable update every
   // decade (0.1, the CG default) costs more than the FP16 drift it removes;
-  // at 32^3x64 0.1 / 0.03 / 0.01 / 0.001 gave 8 / 6 / 4 / 4 updates in
-  // 28 / 28 / 30 / 36 iterations, 0.216 / 0.204 / 0.197 / 0.218 s
-  const double delta = prm->delta > 0 ? prm->delta : 0.01;
+  // FP16 at 32^3x64: 0.1 / 0.03 / 0.01 / 0.001 gave 8 / 6 / 4 / 4 updates in
+  // 28 / 28 / 30 / 36 iterations, 0.216 / 0.204 / 0.197 / 0.218 s.  BF16
+  // (8-bit mantissa) drifts faster: 0.01 took 38-41 iterations, 0.1 27-28.
+  const double delta = prm->delta > 0 ? prm->delta : (P == II_PREC_BF16 ? 0.1 : 0.01);
   const int maxiter = prm->maxiter > 0 ? prm->maxiter : 100000;
   const size_t he = half_elems(g);
   const int nb = hop_blocks(g);                   // BLAS partials per rhs
@@ -2027,15 +2112,23 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
   II_CHECK(fieldP(ctx, 44, 0, batch, &y));
   II_CHECK(fieldP(ctx, 46, 0, batch, &q64));
   II_CHECK(fieldP(ctx, 47, 0, batch, &t64));
-  SF p, r, r0, vv, w, s, t, ys;
-  II_CHECK(fieldP(ctx, 50, P, batch, &p));
+  SF r, r0, vv, w, t, ys;
   II_CHECK(fieldP(ctx, 51, P, batch, &r));
   II_CHECK(fieldP(ctx, 52, P, batch, &r0));
   II_CHECK(fieldP(ctx, 53, P, batch, &vv));
   II_CHECK(fieldP(ctx, 54, P, batch, &w));
-  II_CHECK(fieldP(ctx, 72, P, batch, &s));
   II_CHECK(fieldP(ctx, 73, P, batch, &t));
   II_CHECK(fieldP(ctx, 55, II_PREC_SINGLE, batch, &ys));
+  // p and s of the last kBiDefer iterations stay in rings of kBiDefer + 1
+  // fields (their alpha, omega in ahist) until k_bi_flush adds them to ys
+  constexpr int nring = kBiDefer + 1;
+  SF rp[nring], rsr[nring];
+  for (int i = 0; i < nring; ++i) {
+    II_CHECK(fieldP(ctx, i == 0 ? 50 : 73 + i, P, batch, &rp[i]));
+    II_CHECK(fieldP(ctx, i == 0 ? 72 : 77 + i, P, batch, &rsr[i]));
+  }
+  II_CHECK(workspace(ctx, 62, sizeof(double) * 4 * nring * batch, &v));
+  double* ahist = (double*)v;
   HopSpec h;
   for (int rb0 = 0; rb0 < nrhs_all; rb0 += batch) {
     const int nr = std::min(batch, nrhs_all - rb0);
@@ -2076,9 +2169,20 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
     // r = r0 = p = bhat (x0 = 0), ys = 0
     II_CHECK(convertP(ctx, 0, P, bh, r, nr));
     II_CHECK(convertP(ctx, P, P, r, r0, nr));
-    II_CHECK(convertP(ctx, P, P, r, p, nr));
+    II_CHECK(convertP(ctx, P, P, r, rp[0], nr));
     II_CHECK(convertP(ctx, 0, II_PREC_SINGLE, y, ys, nr));  // zeros
-    auto iteration = [&]() -> ii_status {
+    int pos = 0;            // ring slot of this iteration's p and s
+    std::vector<int> pend;  // slots whose alpha p + omega s is not yet in ys
+    auto flush = [&](double2* y64, const int* skip) -> ii_status {
+      if (pend.empty() && !y64) return II_OK;
+      II_CHECK(BB::flush(ctx, rp, rsr, pend.empty() ? &pos : pend.data(), (int)pend.size(), ahist,
+                         nring, ys, nr, y64, skip, ss));
+      pend.clear();
+      return II_OK;
+    };
+    auto iteration = [&](int cur) -> ii_status {
+      const SF& p = rp[cur];
+      const SF& s = rsr[cur];
       HopSpec a;
       a.nrhs = nr;
       a.skip = skip_it;
@@ -2093,7 +2197,7 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
       a.partials = part;
       II_CHECK(hopP(ctx, P, a, w, vv, &p, &r0));  // v = Dhat p, <r0, v>
       hipLaunchKernelGGL(k_bi_alpha, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nh, plh, sc,
-                         bc);
+                         bc, ahist, cur, nring);
       II_CHECK(BB::s(ctx, r, vv, s, sc, bc, nr));
       a.out_par = 1;
       a.a = 1.0;
@@ -2106,16 +2210,21 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
       a.epi = kEpiTS;
       II_CHECK(hopP(ctx, P, a, w, t, &s, nullptr));  // t = Dhat s, <t, s>, |t|^2
       hipLaunchKernelGGL(k_bi_omega, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nh, plh, sc,
-                         bc);
-      II_CHECK(BB::xr(ctx, p, s, t, r0, r, ys, sc, bc, nr, part));
+                         bc, ahist, cur, nring);
+      II_CHECK(BB::r(ctx, s, t, r0, r, sc, bc, nr, part));
       hipLaunchKernelGGL(k_bi_beta, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, plb, sc,
                          bc);
-      return BB::p(ctx, r, p, vv, sc, bc, nr);
+      return BB::p(ctx, r, p, vv, rp[(cur + 1) % nring], sc, bc, nr);
     };
     int its = 0, chunk = 1;
     bool all_done = false;
     while (its < maxiter && !all_done) {
-      for (int i = 0; i < chunk; ++i) II_CHECK(iteration());
+      for (int i = 0; i < chunk; ++i) {
+        II_CHECK(iteration(pos));
+        pend.push_back(pos);
+        pos = (pos + 1) % nring;
+        if ((int)pend.size() == kBiDefer) II_CHECK(flush(nullptr, nullptr));
+      }
       its += chunk;
       II_CHECK(read_scalars(ctx, nr));
       RUScal* H = host_scal(ctx);
@@ -2135,11 +2244,8 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
         continue;
       }
       chunk = 1;
-      // reliable update for the flagged rhs: y64 += ys, ys = 0
-      hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(g.Vh), nr), dim3(kBlock), 0,
-                         ctx->stream, g, Acc<1>::sp(ys, g.stride), Acc<1>::st(ys, g.stride),
-                         (double2*)y.v, skip_ru, ss);
-      II_HIP(ctx, hipGetLastError());
+      // reliable update for the flagged rhs: y64 += ys + pending, ys = 0
+      II_CHECK(flush((double2*)y.v, skip_ru));
       // rho_true = bhat - Dhat y (FP64) and |rho_true|^2
       h = HopSpec();
       h.nrhs = nr;
@@ -2164,12 +2270,9 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
       hipLaunchKernelGGL(k_bi_ru, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, plb, sc, bc);
       II_HIP(ctx, hipGetLastError());
       II_CHECK(convertP(ctx, P, P, r, r0, nr, &sc[0].keep_p, ss));
-      II_CHECK(convertP(ctx, P, P, r, p, nr, &sc[0].keep_p, ss));
+      II_CHECK(convertP(ctx, P, P, r, rp[pos], nr, &sc[0].keep_p, ss));
     }
-    hipLaunchKernelGGL((k_accum<SpF32, StF32>), dim3(blocks_for(g.Vh), nr), dim3(kBlock), 0,
-                       ctx->stream, g, Acc<1>::sp(ys, g.stride), Acc<1>::st(ys, g.stride),
-                       (double2*)y.v, nullptr, 0);
-    II_HIP(ctx, hipGetLastError());
+    II_CHECK(flush((double2*)y.v, nullptr));
     // x_o = b_o + kappa H_oe y   (eoUtils.cpp:67)
     h = HopSpec();
     h.out_par = 1;
