@@ -5,9 +5,10 @@ at beta = 6.0 (200 sweeps from a cold start, fixed seed), kappa = 0.1355,
 antiperiodic time boundary, the 12 point-source right-hand sides of a
 propagator (delta at the origin, one per spin x colour).  One step = one
 12-rhs propagator solve to the FP64 true residual |b - D x|/|b| <= 1e-10 per
-rhs with the optimized method (batched multi-rhs CG, 16-bit storage with
-per-site norms and reconstruct-12 links, FP32 arithmetic, FP64 reliable
-updates), sources already resident in HBM.
+rhs with the optimized method (batched multi-rhs BiCGStab on the even/odd
+Schur operator, 16-bit storage with per-site norms and reconstruct-12 links,
+FP32 arithmetic, FP64 reliable updates; --method cg selects the batched mixed
+CG on Dhat^dag Dhat instead), sources already resident in HBM.
 
 Reported beside it: the plain FP64 GPU CG (one rhs after the other, as the
 reference's Qprop_EvenOdd does) on the same problem, the FP64 even/odd hop
@@ -16,8 +17,8 @@ learned gauge-covariant preconditioner (trained on this gauge field from 64
 plain-CG pairs, then the same 12 columns through its flexible GCR; training
 time reported apart), and the reference CPU path (oracle/_ref, the compiled
 IroIro++) timed on this host.  `value` is the time of the fastest full-accuracy
-method, the batched mixed-precision CG; the learned method's time is in
-"learned".
+method, the batched mixed-precision BiCGStab; the batched mixed CG is in
+"other_mixed_method", the learned method's time in "learned".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -171,6 +172,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="half", choices=["single", "half", "bf16"])
+    ap.add_argument("--method", default="bicgstab", choices=["bicgstab", "cg"],
+                    help="optimized solver: BiCGStab on Dhat or CG on Dhat^dag Dhat")
+    ap.add_argument("--no-cg-mixed", action="store_true",
+                    help="skip the secondary mixed-precision CG measurement")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=20260601)
@@ -306,7 +311,8 @@ def main():
             % (fp64_s, its, fp64["max_true_residual"]))
 
     # ---- optimized solve: batched, mixed precision
-    solver = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=args.precision, batched=True)
+    solver = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=args.precision, batched=True,
+                                 method=args.method)
     for _ in range(args.warmup):
         solver.solve(src, sol)
     import ctypes  # noqa: F401
@@ -323,6 +329,23 @@ def main():
         "max true residual %.2e" % (ms_per_step / 1000, ["%.3f" % t for t in times],
                                    [o.Iterations for o in outs], [o.restarts for o in outs],
                                    max_res))
+
+    # ---- the other optimized method (batched mixed CG on Dhat^dag Dhat or
+    # BiCGStab), for comparison
+    other = None
+    if not args.no_cg_mixed:
+        om = "cg" if args.method == "bicgstab" else "bicgstab"
+        s2 = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=args.precision, batched=True,
+                                 method=om)
+        s2.solve(src, sol)
+        t0 = time.time()
+        o2 = s2.solve(src, sol)
+        dt2 = time.time() - t0
+        other = {"method": om, "s": dt2, "iterations_per_rhs": [o.Iterations for o in o2],
+                 "reliable_updates": [o.restarts for o in o2],
+                 "max_true_residual": max(o.true_residual for o in o2)}
+        log("batched mixed %s: %.3f s, iterations %s" % (om, dt2, other["iterations_per_rhs"]))
+        outs = solver.solve(src, sol)  # the headline method's solution for the contraction
 
     # ---- meson correlators of the solved propagator (stays on the device)
     import numpy as np
@@ -382,9 +405,13 @@ def main():
         "data": "synthetic: SU(3) heat-bath gauge field generated on the device, point sources",
         "config": {"workload": cfg["workload"], "lattice": list(dims), "beta": cfg["beta"],
                    "kappa": kappa, "nrhs": 12, "tol": 1e-10, "method":
-                   "batched 12-rhs CG on Dhat^dag Dhat, %s storage, fp64 reliable updates"
-                   % args.precision, "plaquette": plaq, "device": name},
+                   ("batched 12-rhs BiCGStab on Dhat" if args.method == "bicgstab" else
+                    "batched 12-rhs CG on Dhat^dag Dhat")
+                   + ", %s storage, fp64 reliable updates" % args.precision,
+                   "plaquette": plaq, "device": name},
+        "iterations_per_rhs": [o.Iterations for o in outs],
         "max_true_residual": max_res,
+        "other_mixed_method": other,
         "fp64_cg": fp64,
         "speedup_vs_fp64_cg": (fp64["s"] / (ms_per_step / 1000.0)) if fp64 else None,
         "roofline": {"bound": "hbm", "achieved": hop_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
