@@ -492,14 +492,14 @@ ii_status tune_hop(ii_context* ctx) {
 }
 
 ii_status hop64(ii_context* ctx, const HopSpec& h, const double2* in, double2* out,
-                const double2* x, const double2* dv) {
+                const double2* x, const double2* dv, int rs) {
   const Geom& g = ctx->g;
   if (h.nrhs == 1 && !ctx->hop_tuned) II_CHECK(tune_hop(ctx));
-  SpF64 sin{in, g.stride};
+  SpF64 sin{in, g.stride, rs};
   GaugeF64 gf{ctx->gauge64, g.stride};
   StF64 so{out, g.stride};
-  SpF64 sx{x ? x : in, g.stride};
-  SpF64 sd{dv ? dv : in, g.stride};
+  SpF64 sx{x ? x : in, g.stride, rs};
+  SpF64 sd{dv ? dv : in, g.stride, rs};
   return dispatch_hop(ctx, sin, gf, so, sx, sd, h);
 }
 

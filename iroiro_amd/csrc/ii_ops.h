@@ -51,8 +51,10 @@ struct LoField {
   float* nrm = nullptr;
 };
 
+// rs: elements between the rhs of in, x and dv (0: back to back, 12 * stride;
+// 24 * stride addresses one parity of full fields); out is back to back
 ii_status hop64(ii_context* ctx, const HopSpec& h, const double2* in, double2* out,
-                const double2* x, const double2* dv);
+                const double2* x, const double2* dv, int rs = 0);
 ii_status hop32(ii_context* ctx, const HopSpec& h, const float2* in, float2* out,
                 const float2* x, const float2* dv);
 ii_status hop_half(ii_context* ctx, const HopSpec& h, const LoField& in, const LoField& out,

@@ -46,6 +46,10 @@ constexpr int kScalInts = sizeof(RUScal) / sizeof(int);
 
 __device__ __forceinline__ bool finite(double v) { return isfinite(v); }
 
+// threads per block of the per-rhs scalar kernels of the propagator solves:
+// they sum ~16k tile partials per rhs at 32^3x64
+constexpr int kFinThreads = 1024;
+
 // pap -> alpha = rr/pap; with ahist, alpha is also kept at
 // ahist[rhs * astride + aidx] (0 for a finished rhs) for the deferred
 // solution update
@@ -448,7 +452,7 @@ ii_status ddagd_cg_fused(ii_context* ctx, int P, double kappa, const SF& p, cons
   h.epi = kEpiNorm;
   h.partials = partials;
   II_CHECK(hopP(ctx, P, h, t, q, &p, nullptr));  // q = Dhat p, |q|^2
-  hipLaunchKernelGGL(k_fin_pap, dim3(nrhs), dim3(kBlock), 0, ctx->stream, partials,
+  hipLaunchKernelGGL(k_fin_pap, dim3(nrhs), dim3(kFinThreads), 0, ctx->stream, partials,
                      hop_dot_parts(ctx->g, nrhs, P), sc, ahist, aidx, astride);
   h.out_par = 1;
   h.a = 1.0;
@@ -1522,14 +1526,14 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       if (defer) {
         II_CHECK(ddagd_cg_fused(ctx, P, kappa, p, rs, qs, ts, sc, part, nr, skip_it, ss, ahist,
                                 cur, nring));
-        hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
+        hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part,
                            hop_dot_parts(g, nr, P), sc, 1);
       } else {
         II_CHECK(ddagdP(ctx, P, kappa, p, ss_, qs, ts, &p, part, nr, skip_it, ss));
-        hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
+        hipLaunchKernelGGL(k_fin_pap, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part,
                            hop_dot_parts(g, nr, P), sc, nullptr, 0, 0);
         II_CHECK(cg_blas_P(ctx, P, p, ss_, rs, ys, ytype, sc, nr, part, nb));
-        hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc, 1);
+        hipLaunchKernelGGL(k_fin_rr, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part, nb, sc, 1);
       }
       return cg_p_P(ctx, P, rs, p, ring[(cur + 1) % nring], sc, nr);
     };
@@ -1619,7 +1623,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
         h.partials = part;
         II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)rho.v, (double2*)y.v,
                        (const double2*)bh.v));
-        hipLaunchKernelGGL(k_fin_rho, dim3(nr), dim3(kBlock), 0, ctx->stream, part,
+        hipLaunchKernelGGL(k_fin_rho, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part,
                            hop_dot_parts(g, nr, II_PREC_DOUBLE), sc);
         // new CG residual r = Dhat^dag rho and |r|^2.  The true residual rho
         // is FP64; r only restarts the sloppy recursion, so with a sloppy
@@ -1653,7 +1657,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
           hipLaunchKernelGGL(k_norm2_masked, dim3(nb, nr), dim3(kBlock), 0, ctx->stream, g,
                              (const double2*)r64.v, skip_ru, ss, part);
         }
-        hipLaunchKernelGGL(k_fin_ru, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, sc);
+        hipLaunchKernelGGL(k_fin_ru, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part, nb, sc);
         II_HIP(ctx, hipGetLastError());
         // restart the search direction where the recursion broke (keep_p == 0)
         if (lo) II_CHECK(convertP(ctx, P, P, rs, ring[pos], nr, &sc[0].keep_p, ss));
@@ -2196,7 +2200,7 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
       a.epi = kEpiCDot;
       a.partials = part;
       II_CHECK(hopP(ctx, P, a, w, vv, &p, &r0));  // v = Dhat p, <r0, v>
-      hipLaunchKernelGGL(k_bi_alpha, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nh, plh, sc,
+      hipLaunchKernelGGL(k_bi_alpha, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part, nh, plh, sc,
                          bc, ahist, cur, nring);
       II_CHECK(BB::s(ctx, r, vv, s, sc, bc, nr));
       a.out_par = 1;
@@ -2209,10 +2213,10 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
       a.xpay = true;
       a.epi = kEpiTS;
       II_CHECK(hopP(ctx, P, a, w, t, &s, nullptr));  // t = Dhat s, <t, s>, |t|^2
-      hipLaunchKernelGGL(k_bi_omega, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nh, plh, sc,
+      hipLaunchKernelGGL(k_bi_omega, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part, nh, plh, sc,
                          bc, ahist, cur, nring);
       II_CHECK(BB::r(ctx, s, t, r0, r, sc, bc, nr, part));
-      hipLaunchKernelGGL(k_bi_beta, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, plb, sc,
+      hipLaunchKernelGGL(k_bi_beta, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part, nb, plb, sc,
                          bc);
       return BB::p(ctx, r, p, vv, rp[(cur + 1) % nring], sc, bc, nr);
     };
@@ -2263,11 +2267,11 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
       h.partials = part;
       II_CHECK(hop64(ctx, h, (double2*)t64.v, (double2*)q64.v, (double2*)y.v,
                      (const double2*)bh.v));
-      hipLaunchKernelGGL(k_fin_rho, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nhd, sc);
+      hipLaunchKernelGGL(k_fin_rho, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part, nhd, sc);
       // r = rho_true in the sloppy precision; rho = <r0, r>; restart if broken
       II_CHECK(convertP(ctx, 0, P, q64, r, nr, skip_ru, ss));
       II_CHECK(BB::rdots(ctx, r0, r, skip_ru, ss, nr, part));
-      hipLaunchKernelGGL(k_bi_ru, dim3(nr), dim3(kBlock), 0, ctx->stream, part, nb, plb, sc, bc);
+      hipLaunchKernelGGL(k_bi_ru, dim3(nr), dim3(kFinThreads), 0, ctx->stream, part, nb, plb, sc, bc);
       II_HIP(ctx, hipGetLastError());
       II_CHECK(convertP(ctx, P, P, r, r0, nr, &sc[0].keep_p, ss));
       II_CHECK(convertP(ctx, P, P, r, rp[pos], nr, &sc[0].keep_p, ss));
@@ -2327,10 +2331,10 @@ ii_status full_residual(ii_context* ctx, double kappa, const ii_field* src, cons
   const size_t he = half_elems(g);
   const int n = src->nrhs;
   double2* w;
-  double* d;  // |b - D x|^2 per (rhs, parity) [0, 2n), |b|^2 [2n, 4n), 1.0 at [4n]
+  double* d;  // |b - D x|^2 per (parity, rhs) [0, 2n), |b|^2 per (rhs, parity) [2n, 4n), 1.0
   {
     void* p;
-    II_CHECK(workspace(ctx, 56, he * sizeof(double2), &p));
+    II_CHECK(workspace(ctx, 56, he * n * sizeof(double2), &p));
     w = (double2*)p;
     II_CHECK(workspace(ctx, 57, (4 * (size_t)n + 1) * sizeof(double), &p));
     d = (double*)p;
@@ -2338,32 +2342,32 @@ ii_status full_residual(ii_context* ctx, double kappa, const ii_field* src, cons
   const double unit = 1.0;
   II_HIP(ctx, hipMemcpyAsync(d + 4 * n, &unit, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   II_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const int nblk = hop_dot_parts(g, 1, II_PREC_DOUBLE);
-  II_CHECK(ensure_partials(ctx, (size_t)nblk + 16));
-  for (int k = 0; k < n; ++k) {
-    const double2* se = src->data + (size_t)k * 2 * he;
-    const double2* xe = sol->data + (size_t)k * 2 * he;
-    for (int par = 0; par < 2; ++par) {
-      // w = b_par - (x_par - kappa H x_other) = (b - D x)_par, |w|^2 in the
-      // hop's epilogue
-      HopSpec h;
-      h.out_par = par;
-      h.a = -kappa;
-      h.xpay = true;
-      h.epi = kEpiRUpd;
-      h.coef = d + 4 * n;
-      h.partials = ctx->partials;
-      II_CHECK(hop64(ctx, h, xe + (1 - par) * he, w, xe + par * he, se + par * he));
-      II_CHECK(finalize_sums(ctx, ctx->partials, nblk, d + 2 * k + par, 1));
-    }
-    II_CHECK(dot64(ctx, se, se, d + 2 * n + 2 * k, 2));
+  const int nblk = hop_dot_parts(g, n, II_PREC_DOUBLE);
+  II_CHECK(ensure_partials(ctx, (size_t)nblk * n + 16));
+  for (int par = 0; par < 2; ++par) {
+    // w = b_par - (x_par - kappa H x_other) = (b - D x)_par for all rhs at
+    // once (the full fields' parity par: rhs stride 24 * stride), |w|^2 in
+    // the hop's epilogue
+    HopSpec h;
+    h.out_par = par;
+    h.a = -kappa;
+    h.xpay = true;
+    h.epi = kEpiRUpd;
+    h.coef = d + 4 * n;
+    h.partials = ctx->partials;
+    h.nrhs = n;
+    II_CHECK(hop64(ctx, h, sol->data + (1 - par) * he, w, sol->data + par * he,
+                   src->data + par * he, (int)(2 * he)));
+    II_CHECK(finalize_sums(ctx, ctx->partials, nblk, d + par * n, n));
   }
+  // |b|^2 of every half field of the sources in one call
+  II_CHECK(dot64(ctx, src->data, src->data, d + 2 * n, 2 * n));
   std::vector<double> hh(4 * (size_t)n);
   II_HIP(ctx, hipMemcpyAsync(hh.data(), d, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
                              ctx->stream));
   II_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (int k = 0; k < n; ++k)
-    res[k] = std::sqrt((hh[2 * k] + hh[2 * k + 1]) / (hh[2 * n + 2 * k] + hh[2 * n + 2 * k + 1]));
+    res[k] = std::sqrt((hh[k] + hh[n + k]) / (hh[2 * n + 2 * k] + hh[2 * n + 2 * k + 1]));
   return II_OK;
 }
 

@@ -94,11 +94,14 @@ struct SpF64 {
   static constexpr bool kScaled = false;
   const double2* __restrict__ p;
   int stride;
+  int rs = 0;  // elements between rhs (0: 12 * stride, back to back)
   __device__ __forceinline__ C ld(int site, int comp) const {
     return ld_at(p, (uint32_t)(comp * stride), (uint32_t)site);
   }
   __device__ __forceinline__ R scale(int) const { return 1.0; }
-  __device__ __forceinline__ SpF64 rhs(int r) const { return SpF64{p + (size_t)r * 12 * stride, stride}; }
+  __device__ __forceinline__ SpF64 rhs(int r) const {
+    return SpF64{p + (size_t)r * (rs ? rs : 12 * stride), stride, rs};
+  }
 };
 struct StF64 {
   using R = double;
