@@ -206,6 +206,29 @@ def test_propagator_bicgstab(precision, batched):
     ctx.close()
 
 
+def test_bicgstab_fewer_iterations_than_cg_and_maxiter():
+    """On the same point sources BiCGStab (two Dhat per iteration) needs fewer
+    Dhat applications than CG on Dhat^dag Dhat (two Dhat and two Dhat^dag per
+    iteration); a maxiter below that count is reported as not converged."""
+    dims = (8, 8, 8, 8)
+    lat, u = _prop_setup(dims)
+    ctx = ii.Context(dims)
+    ctx.upload_gauge(u)
+    src = ii.DeviceField(ctx, "full", 12)
+    sol = ii.DeviceField(ctx, "full", 12)
+    src.point_sources((0, 0, 0, 0))
+    cg = ii.PropagatorSolver(ctx, 0.1355, precision="half", batched=True).solve(src, sol)
+    bi = ii.PropagatorSolver(ctx, 0.1355, precision="half", batched=True,
+                             method="bicgstab").solve(src, sol)
+    n_cg = max(o.Iterations for o in cg)
+    n_bi = max(o.Iterations for o in bi)
+    assert 2 * n_bi < 2 * n_cg, (n_bi, n_cg)
+    with pytest.raises(ii.IIError):
+        ii.PropagatorSolver(ctx, 0.1355, precision="half", batched=True, method="bicgstab",
+                            maxiter=max(2, n_bi // 3)).solve(src, sol)
+    ctx.close()
+
+
 def test_bicgstab_rejects_double():
     ctx = ii.Context((4, 4, 4, 4))
     ctx.unit_gauge()
@@ -340,8 +363,9 @@ def test_copy_bandwidth_runs():
     ctx.close()
 
 
-@pytest.mark.parametrize("precision", ["half", "bf16", "single"])
-def test_point_source_heatbath_16c32(precision):
+@pytest.mark.parametrize("precision,method", [("half", "cg"), ("bf16", "cg"), ("single", "cg"),
+                                              ("half", "bicgstab"), ("bf16", "bicgstab")])
+def test_point_source_heatbath_16c32(precision, method):
     """Point sources on a beta=6.0 heat-bath field at 16^3 x 32: far from the
     source the early Krylov vectors underflow to FP32 denormals (values
     ~kappa^40), the regime where a 16-bit site norm must not overflow."""
@@ -354,7 +378,7 @@ def test_point_source_heatbath_16c32(precision):
     sol = ii.DeviceField(ctx, "full", 12)
     src.point_sources((0, 0, 0, 0))
     outs = ii.PropagatorSolver(ctx, 0.1355, tol=1e-10, precision=precision,
-                               batched=True).solve(src, sol)
+                               batched=True, method=method).solve(src, sol)
     assert max(o.true_residual for o in outs) <= 1e-10
     # check one column against the oracle on the same (downloaded) field
     u = ctx.download_gauge()
