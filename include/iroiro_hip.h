@@ -306,7 +306,7 @@ typedef struct {
   int precision;         /* II_PREC_* of the inner solver */
   int batched;           /* nonzero: all rhs in one fused multi-rhs solve */
   double delta;          /* reliable-update trigger (residual drop factor), 0 = default */
-  int use_graph;         /* capture solver iterations in a HIP graph */
+  int use_graph;         /* capture solver iterations in a HIP graph (II_METHOD_CG) */
   int use_learned;       /* apply the trained learned preconditioner (flexible outer) */
   int method;            /* II_METHOD_CG: CG on Dhat^dag Dhat (Solver_CG on Fopr_DdagD);
                             II_METHOD_BICGSTAB: BiCGStab on Dhat itself (no reference
