@@ -54,10 +54,12 @@ CONFIGS = {
 }
 
 
-def measured_traffic(dims):
-    """HBM bytes per launch of the FP64 hop from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes summarised in profiles/wilson_hop_32c64.md (scaled by
-    the copy-kernel calibration stated there); None for other lattices."""
+def measured_traffic(dims, key="k_hop_fp64_32c64"):
+    """HBM bytes per launch of a hop kernel (key: k_hop_fp64_32c64, the FP64
+    hop, or k_hop_mrhs_half_32c64, the 12-rhs FP16 hop) from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/wilson_hop_32c64.md
+    (scaled by the copy-kernel calibration stated there); None for other
+    lattices."""
     if tuple(dims) != (32, 32, 32, 64):
         return None, None
     path = os.path.join(ROOT, "profiles", "wilson_hop_32c64.md")
@@ -65,8 +67,8 @@ def measured_traffic(dims):
         text = open(path).read()
     except OSError:
         return None, None
-    m = re.search(r"pmc k_hop_fp64_32c64: fetch_kib=(\d+) write_kib=(\d+) "
-                  r"fetch_scale=([0-9.]+) write_scale=([0-9.]+)", text)
+    m = re.search(r"pmc %s: fetch_kib=(\d+) write_kib=(\d+) "
+                  r"fetch_scale=([0-9.]+) write_scale=([0-9.]+)" % re.escape(key), text)
     if not m:
         return None, None
     fk, wk, fs, ws = int(m.group(1)), int(m.group(2)), float(m.group(3)), float(m.group(4))
@@ -292,6 +294,8 @@ def main():
                                   hop_lp_ms, copy_gbs))
 
     traffic, traffic_src = measured_traffic(dims)
+    lp_traffic, lp_traffic_src = (measured_traffic(dims, "k_hop_mrhs_half_32c64")
+                                  if args.precision == "half" else (None, None))
     force_ms, force_total_ms = ctx.bench_md_force(20)
     force_gbs = MD_FORCE_BYTES_PER_SITE * ctx.V / (force_ms * 1e-3) / 1e9
     log("MD force kernel %.3f ms = %.0f GB/s (%.1f%% of HBM peak)"
@@ -423,10 +427,15 @@ def main():
                      "avg_ms": hop_ms,
                      "gflops": HOP_FLOPS_PER_SITE * Vh / (hop_ms * 1e-3) / 1e9,
                      "copy_kernel_gbs": copy_gbs},
-        "roofline_batched_lowprec": {"kernel": "k_hop %s 12-rhs" % args.precision,
-                                     "avg_ms": hop_lp_ms,
-                                     "achieved": lp_bytes / (hop_lp_ms * 1e-3) / 1e9,
-                                     "unit": "GB/s", "bytes_per_site_per_rhs": 104 + 16},
+        "roofline_batched_lowprec": {
+            "kernel": "k_hop_mrhs %s 12-rhs (the solve's dominant kernel), V/2 x 12 site-rhs "
+                      "per launch, 120 B/site-rhs algorithmic" % args.precision,
+            "bound": "vector-memory path (rocprof counters: texture addresser ~85% busy)",
+            "avg_ms": hop_lp_ms,
+            "achieved": lp_bytes / (hop_lp_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+            "frac": lp_bytes / (hop_lp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "unit": "GB/s", "bytes_per_site_per_rhs": 104 + 16,
+            "traffic": lp_traffic, "traffic_source": lp_traffic_src},
         "md_force": {"kernel": "k_md_force, Dirac_Wilson_EvenOdd::md_force over V = %d sites, "
                                "1920 B/site algorithmic" % ctx.V,
                      "avg_ms": force_ms, "achieved": force_gbs, "unit": "GB/s",
