@@ -571,6 +571,57 @@ __device__ __forceinline__ void load_dir_raw(const SpIn& in, int nb, RawDir& r) 
   }
   r.n = in.scale(nb);
 }
+// The input spinor of a lane's own tile site (the 64 sites of the tile in
+// the input parity), kept in registers: a neighbour inside the tile is
+// taken from the lane that holds it (ds_bpermute) instead of memory.  With
+// 16-site rows every x neighbour and 3 of 4 y neighbours are inside.
+struct RawSite {
+  __half2 v[4][3];
+  float n;
+};
+template <typename SpIn>
+__device__ __forceinline__ void load_site_raw(const SpIn& in, int hs, RawSite& o) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const auto p0 = in.raw2(hs, c, 0), p1 = in.raw2(hs, c, 1);
+    o.v[0][c] = p0.lo;
+    o.v[1][c] = p0.hi;
+    o.v[2][c] = p1.lo;
+    o.v[3][c] = p1.hi;
+  }
+  o.n = in.scale(hs);
+}
+__device__ __forceinline__ __half2 shfl_h2(__half2 v, int lane) {
+  return __builtin_bit_cast(__half2, __shfl(__builtin_bit_cast(int, v), lane));
+}
+// load_dir_raw for the neighbour nb, from the owning lane when nb is one of
+// the tile's sites [hs0, hs0 + 64), else from memory
+template <int MU, bool FWD, bool G5IN, typename SpIn>
+__device__ __forceinline__ void load_dir_tile(const SpIn& in, int nb, int hs0, int lane,
+                                              const RawSite& own, RawDir& r) {
+  using P = Proj<MU, FWD>;
+  constexpr int a1 = G5IN ? (P::A1 ^ 2) : P::A1;
+  const int src = nb - hs0;
+  const bool inside = (unsigned)src < 64u;
+  if (__any(inside)) {
+    const int sl = inside ? src : lane;  // every active lane takes part
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (P::C1 == 4) {
+        r.v[a1][c] = shfl_h2(own.v[a1][c], sl);
+        r.v[a1 + 1][c] = shfl_h2(own.v[a1 + 1][c], sl);
+      } else {
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp) r.v[sp][c] = shfl_h2(own.v[sp][c], sl);
+      }
+    }
+    r.n = __shfl(own.n, sl);
+  }
+  if (!__all(inside)) {
+    if (!inside) load_dir_raw<MU, FWD, G5IN>(in, nb, r);
+  }
+}
+
 template <int MU, bool FWD, bool G5IN, typename GetM>
 __device__ __forceinline__ void hop_dir_dot2_raw(float2 acc[4][3], const RawDir& r, GetM&& getM) {
   using P = Proj<MU, FWD>;
