@@ -240,8 +240,8 @@ def test_bicgstab_rejects_double():
     ctx.close()
 
 
-@pytest.mark.parametrize("nrhs", [5, 13])
-def test_propagator_ragged_batches(nrhs):
+@pytest.mark.parametrize("nrhs,method", [(5, "cg"), (13, "cg"), (5, "bicgstab"), (13, "bicgstab")])
+def test_propagator_ragged_batches(nrhs, method):
     """Batched half-precision solves of 5 and 13 Gaussian sources (13: two
     rhs row-blocks of the multi-rhs kernel); every column against the
     oracle's FP64 operator and solution."""
@@ -257,7 +257,7 @@ def test_propagator_ragged_batches(nrhs):
     for k in range(nrhs):
         src.upload(k, bs[k])
     outs = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision="half",
-                               batched=True).solve(src, sol)
+                               batched=True, method=method).solve(src, sol)
     for k in range(nrhs):
         x = sol.download(k)
         r_oracle = lat.true_residual_full(u, kappa, x, bs[k])
