@@ -12,7 +12,8 @@
 // cos(py y) cos(pz z).
 //
 // k_meson_sites: one thread per site.  For each (c1, c2) it loads the 4x4
-// spin blocks of both propagators (16 complex each; once when q1 == q2) and
+// spin blocks of both propagators (16 complex each; one block serves both
+// when q1 == q2, the SAME instantiation) and
 // updates the 18 accumulators; the type tables are constexpr, so every
 // spin index and factor folds at compile time.  It writes one value (two
 // for the momentum projection) per type and site.  k_meson_slices then sums
@@ -88,10 +89,10 @@ __device__ __forceinline__ void meson_all(double* re, double* im, const double2 
 
 // mode 0: Re z; 1: z e^{i p.x} (re, im); 2: Re z cos cos cos.
 // vals: [k][2][stride], k = type (mode 0, 2) or 2 type + re/im (mode 1).
-template <int MODE>
+template <int MODE, bool SAME>
 __global__ void __launch_bounds__(kBlock)
 k_meson_sites(Geom g, const double2* __restrict__ q1, const double2* __restrict__ q2,
-              int same, double px, double py, double pz, double* __restrict__ vals) {
+              double px, double py, double pz, double* __restrict__ vals) {
   const int hs = blockIdx.x * blockDim.x + threadIdx.x;
   const int par = blockIdx.y;
   if (hs >= g.Vh) return;
@@ -104,25 +105,24 @@ k_meson_sites(Geom g, const double2* __restrict__ q1, const double2* __restrict_
   for (int T = 0; T < kMesonTypes; ++T) re[T] = im[T] = 0.0;
   for (int c1 = 0; c1 < 3; ++c1) {
     for (int c2 = 0; c2 < 3; ++c2) {
-      double2 a[4][4], b[4][4];
+      double2 a[4][4];
 #pragma unroll
       for (int s1 = 0; s1 < 4; ++s1)
 #pragma unroll
         for (int s = 0; s < 4; ++s)
           a[s1][s] = b1[(size_t)(c1 + 3 * s1) * colstride + (size_t)(3 * s + c2) * g.stride];
-      if (same) {
-#pragma unroll
-        for (int s1 = 0; s1 < 4; ++s1)
-#pragma unroll
-          for (int s = 0; s < 4; ++s) b[s1][s] = a[s1][s];
+      if constexpr (SAME) {
+        // q1 == q2: one block serves both operands (half the registers)
+        meson_all<0>(re, im, a, a, MODE == 1);
       } else {
+        double2 b[4][4];
 #pragma unroll
         for (int s1 = 0; s1 < 4; ++s1)
 #pragma unroll
           for (int s = 0; s < 4; ++s)
             b[s1][s] = b2[(size_t)(c1 + 3 * s1) * colstride + (size_t)(3 * s + c2) * g.stride];
+        meson_all<0>(re, im, a, b, MODE == 1);
       }
-      meson_all<0>(re, im, a, b, MODE == 1);
     }
   }
   const size_t vstride = 2 * (size_t)g.stride;
@@ -200,17 +200,19 @@ ii_status ii_meson_correlators(ii_context* ctx, const ii_field* q1, const ii_fie
   void* wo = nullptr;
   II_CHECK(workspace(ctx, 49, (size_t)nk * L * sizeof(double), &wo));
   const double px = mom ? mom[0] : 0.0, py = mom ? mom[1] : 0.0, pz = mom ? mom[2] : 0.0;
-  const int same = q1->data == q2->data ? 1 : 0;
+  const bool same = q1->data == q2->data;
   const dim3 grid(blocks_for(g.Vh), 2);
-  if (mode == 0)
-    hipLaunchKernelGGL(k_meson_sites<0>, grid, dim3(kBlock), 0, ctx->stream, g, q1->data,
-                       q2->data, same, px, py, pz, (double*)wv);
-  else if (mode == 1)
-    hipLaunchKernelGGL(k_meson_sites<1>, grid, dim3(kBlock), 0, ctx->stream, g, q1->data,
-                       q2->data, same, px, py, pz, (double*)wv);
-  else
-    hipLaunchKernelGGL(k_meson_sites<2>, grid, dim3(kBlock), 0, ctx->stream, g, q1->data,
-                       q2->data, same, px, py, pz, (double*)wv);
+#define II_MESON(M, S)                                                                       \
+  hipLaunchKernelGGL((k_meson_sites<M, S>), grid, dim3(kBlock), 0, ctx->stream, g, q1->data, \
+                     q2->data, px, py, pz, (double*)wv)
+  if (mode == 0) {
+    if (same) II_MESON(0, true); else II_MESON(0, false);
+  } else if (mode == 1) {
+    if (same) II_MESON(1, true); else II_MESON(1, false);
+  } else {
+    if (same) II_MESON(2, true); else II_MESON(2, false);
+  }
+#undef II_MESON
   II_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_meson_slices, dim3(nk, L), dim3(kBlock), 0, ctx->stream, g,
                      (const double*)wv, direction, (double*)wo);
