@@ -636,6 +636,27 @@ ii_status ensure_ru_scalars(ii_context* ctx, int n) {
   return ensure_scalars(ctx, 2 * n);
 }
 
+// When some rhs of a batch trigger a reliable update, every other rhs whose
+// |r| has come within a factor 4 of its own trigger (|r|^2 < 16 delta^2
+// rmax^2) joins it, so that the batch's rhs update together: one batched
+// FP64 residual per event instead of one more for each rhs that would
+// trigger an iteration or two later.  An early reliable update is still a
+// reliable update; the rhs's next trigger is measured from it.  Needs the
+// host copy of the scalars current (read_scalars) and no kernel in flight.
+ii_status join_updates(ii_context* ctx, int n) {
+  RUScal* H = host_scal(ctx);
+  bool joined = false;
+  for (int k = 0; k < n; ++k)
+    if (!H[k].done && H[k].skip_ru && H[k].rr < 16.0 * H[k].delta2 * H[k].rmax2) {
+      H[k].skip_ru = 0;
+      joined = true;
+    }
+  if (joined)
+    II_HIP(ctx, hipMemcpyAsync(ctx->scal, ctx->scal_host, n * sizeof(RUScal),
+                               hipMemcpyHostToDevice, ctx->stream));
+  return II_OK;
+}
+
 __global__ void k_scal_init(RUScal* sc, const double* __restrict__ rr, const double* __restrict__ bb,
                             double prec, double tol, double delta, int mode) {
   RUScal& S = sc[blockIdx.x];
@@ -1604,6 +1625,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       if (all_done) break;
       if (any_upd) {
         // reliable update, FP64, for the flagged rhs only
+        II_CHECK(join_updates(ctx, nr));
         if (lo) II_CHECK(flush_fold(skip_ru));
         // q = Dhat y ; rho = bhat - q ; |rho|^2
         h = HopSpec();
@@ -2248,6 +2270,7 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
         continue;
       }
       chunk = 1;
+      II_CHECK(join_updates(ctx, nr));
       // reliable update for the flagged rhs: y64 += ys + pending, ys = 0
       II_CHECK(flush((double2*)y.v, skip_ru));
       // rho_true = bhat - Dhat y (FP64) and |rho_true|^2
