@@ -305,7 +305,10 @@ typedef struct {
   int maxiter;           /* bound on total inner iterations per solve */
   int precision;         /* II_PREC_* of the inner solver */
   int batched;           /* nonzero: all rhs in one fused multi-rhs solve */
-  double delta;          /* reliable-update trigger (residual drop factor), 0 = default */
+  double delta;          /* reliable-update trigger: |r| below delta times its value at the
+                            last update (0 = default: 0.1 for CG, 0.01 for BiCGStab, 0.1
+                            for BiCGStab in BF16).  Batched: rhs within a factor 4 of
+                            their own trigger join the update of the rhs that trigger */
   int use_graph;         /* capture solver iterations in a HIP graph (II_METHOD_CG) */
   int use_learned;       /* apply the trained learned preconditioner (flexible outer) */
   int method;            /* II_METHOD_CG: CG on Dhat^dag Dhat (Solver_CG on Fopr_DdagD);
@@ -316,7 +319,8 @@ enum { II_METHOD_CG = 0, II_METHOD_BICGSTAB = 1 };
 
 /* Full solve D x = b for every rhs of a full field through the even/odd
  * Schur system (Inverter_WilsonLike::invert, eoUtils.cpp:43-80) with a
- * device CG on Dhat^dag Dhat; stops on the FP64 true full-D residual.
+ * device CG on Dhat^dag Dhat or (p->method) BiCGStab on Dhat; stops on the
+ * FP64 true full-D residual.
  * Sources and solutions stay on the device.  out: one entry per rhs. */
 ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_params* p,
                               const ii_field* src, ii_field* sol, ii_solver_output* out);
