@@ -702,15 +702,15 @@ __device__ __forceinline__ void wilson_hop_site(typename SpIn::C acc[4][3], cons
 // XCD-aware block order: hardware dispatches block b to XCD b % 8; the
 // remap gives each XCD a contiguous range of logical blocks so that
 // neighbouring sites (and the rhs of one site tile) share an L2.
-__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+__host__ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
   const int nx = 8;
   if (nblocks % nx) return b;
   return (b % nx) * (nblocks / nx) + b / nx;
 }
 
 // First half site of the logical block that physical block b handles.
-__device__ __forceinline__ int hop_block_base(const Geom& g, const HopOrder& o, int b, int nblk,
-                                              int bs) {
+__host__ __device__ __forceinline__ int hop_block_base(const Geom& g, const HopOrder& o, int b,
+                                                       int nblk, int bs) {
   if (o.mode == 1) {
     const int x = b & 7, j = b >> 3;
     const int bb = j % o.bpp;
