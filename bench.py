@@ -5,22 +5,24 @@ at beta = 6.0 (200 sweeps from a cold start, fixed seed), kappa = 0.1355,
 antiperiodic time boundary, the 12 point-source right-hand sides of a
 propagator (delta at the origin, one per spin x colour).  One step = one
 12-rhs propagator solve to the FP64 true residual |b - D x|/|b| <= 1e-10 per
-rhs with the optimized method (batched multi-rhs BiCGStab on the even/odd
-Schur operator, 16-bit storage with per-site norms and reconstruct-12 links,
-FP32 arithmetic, FP64 reliable updates; --method cg selects the batched mixed
-CG on Dhat^dag Dhat instead), sources already resident in HBM.
+rhs with mixed precision + batching + the learned preconditioner: the batched
+flexible GCR on the even/odd Schur operator with the gauge-covariant network
+(trained on this gauge field, bf16 activations and links, MFMA channel mixing)
+as preconditioner, FP32 Krylov vectors, FP64 true-residual restarts, each
+restart cycle replayed as a captured HIP graph; sources already resident in
+HBM.  The network's training (64 plain-CG pairs) is timed apart.
 
-Reported beside it: the plain FP64 GPU CG (one rhs after the other, as the
-reference's Qprop_EvenOdd does) on the same problem, the FP64 even/odd hop
-(Dslash) bandwidth against the HBM roofline with its measured HBM traffic, the
-learned gauge-covariant preconditioner (trained on this gauge field from 64
-plain-CG pairs, then the same 12 columns through its flexible GCR; training
-time reported apart), and the reference CPU path (oracle/_ref, the compiled
-IroIro++) timed on this host.  `value` is the time of the fastest full-accuracy
-method, the batched mixed-precision BiCGStab; the batched mixed CG is in
-"other_mixed_method", the learned method's time in "learned".
+Reported beside it, on the same problem: the plain FP64 GPU CG (one rhs after
+the other, as the reference's Qprop_EvenOdd does; warmed up like the timed
+method) with each rhs's error of the timed solution against it, the same
+flexible GCR without the network, the batched mixed BiCGStab and CG, the
+network's stage kernels against the HBM roofline (the solve's dominant kernel
+is `roofline`), the FP64 even/odd hop (Dslash) bandwidth (metric part 2,
+`dslash_fp64_roofline`) with its measured HBM traffic, and the reference CPU
+path (oracle/_ref, the compiled IroIro++) timed on this host on the same
+gauge field.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--method M]
 """
 import argparse
 import json
@@ -41,6 +43,20 @@ MD_FORCE_BYTES_PER_SITE = 1920
 # meson contraction of one 12-column FP64 propagator: 12 x 24 doubles read,
 # 18 channel values written and read back by the slice sums
 MESON_BYTES_PER_SITE = 12 * 24 * 8 + 2 * 18 * 8
+# algorithmic bytes of the network's four stage kernels per output site:
+# (per rhs, shared by all rhs).  Activations are bf16 complex (4 B), 48 B per
+# channel; r and z are FP32 half fields (96 B per site); each launch reads
+# the 8 transport matrices of its sites once (8 x 80 B) for every rhs.
+#   first: r at the even neighbours 96 + [L1|L2](odd) written 768
+#   mid<0>: L1(odd) 384 + r 96 + L2(even) written 384
+#   mid<1>: L2(even) 384 + L2(odd) 384 + L3(odd) written 384
+#   last: (L3|L2)(odd) 768 + L2(even) 384 + z written 96
+NET_STAGE_BYTES = [(96 + 768, 640), (384 + 96 + 384, 640), (3 * 384, 640),
+                   (768 + 384 + 96, 640)]
+# MFMA flops per output site and rhs (v_mfma_f32_16x16x32_bf16, 16384 flops,
+# 16 sites per instruction): first 3 x 8 x 2, mid 3 x 4 x 2 x 8 + identity
+# (3 x 4 x 1 or 3 x 4 x 2), last 3 x 4 x 8 + 3 x 2
+NET_STAGE_MFMA_FLOPS = [48 * 1024, (192 + 12) * 1024, (192 + 24) * 1024, (96 + 6) * 1024]
 
 CONFIGS = {
     "c5": dict(dims=(32, 32, 32, 64), beta=6.0, kappa=0.1355, sweeps=200,
@@ -54,15 +70,16 @@ CONFIGS = {
 }
 
 
-def measured_traffic(dims, key="k_hop_fp64_32c64"):
-    """HBM bytes per launch of a hop kernel (key: k_hop_fp64_32c64, the FP64
-    hop, or k_hop_mrhs_half_32c64, the 12-rhs FP16 hop) from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/wilson_hop_32c64.md
+def measured_traffic(dims, key="k_hop_fp64_32c64", fname="wilson_hop_32c64.md"):
+    """HBM bytes per launch of a kernel (key: k_hop_fp64_32c64, the FP64 hop;
+    k_hop_mrhs_half_32c64, the 12-rhs FP16 hop; k_bnet_dominant_32c64 in
+    learned_net_batched_32c64.md, the network's dominant stage) from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised under profiles/
     (scaled by the copy-kernel calibration stated there); None for other
-    lattices."""
+    lattices or when no summary holds the key."""
     if tuple(dims) != (32, 32, 32, 64):
         return None, None
-    path = os.path.join(ROOT, "profiles", "wilson_hop_32c64.md")
+    path = os.path.join(ROOT, "profiles", fname)
     try:
         text = open(path).read()
     except OSError:
@@ -72,18 +89,23 @@ def measured_traffic(dims, key="k_hop_fp64_32c64"):
     if not m:
         return None, None
     fk, wk, fs, ws = int(m.group(1)), int(m.group(2)), float(m.group(3)), float(m.group(4))
-    return (fk * fs + wk * ws) * 1024.0, "profiles/wilson_hop_32c64.md"
+    return (fk * fs + wk * ws) * 1024.0, "profiles/" + fname
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg, n_iter_per_rhs, nrhs=12):
+def cpu_baseline(ctx, cfg, n_iter_per_rhs, nrhs=12):
     """Times the reference CPU path (compiled IroIro++, oracle/_ref) on this
-    host: one EO hop and a bounded Solver_CG sample on the same lattice; the
-    12-rhs solve time is extrapolated from the measured seconds per CG
-    iteration and the per-rhs iteration count of the FP64 GPU solve."""
+    host, on this run's own heat-bath gauge field (downloaded from the device
+    with its boundary phase): one EO hop, then column 0 of the point-source
+    propagator through Solver_CG on Fopr_DdagD as Inverter_WilsonLike calls
+    it, stopped after about 12 iterations (a squared stopping precision
+    scaled from the FP64 GPU iteration count).  The 12-rhs solve time is
+    extrapolated from the measured seconds per CG iteration and the per-rhs
+    iteration count of the FP64 GPU solve."""
+    import tempfile
     ref = os.path.join(ROOT, "oracle", "_ref", "iroiro_ref")
     if not os.path.exists(ref):
         return None
@@ -91,15 +113,24 @@ def cpu_baseline(cfg, n_iter_per_rhs, nrhs=12):
     mass = repr(0.5 / cfg["kappa"] - 4.0)
     threads = int(os.environ.get("OMP_NUM_THREADS", "8"))
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    # a loose precision bounds the CG sample to a handful of iterations
-    cmd = [ref, "bench"] + dims + [mass, "2", "1e-3"]
-    t0 = time.time()
+    target = 12
+    prec = 10.0 ** (-20.0 * target / max(n_iter_per_rhs, target))
+    fd, path = tempfile.mkstemp(suffix=".gauge")
+    os.close(fd)
     try:
+        ctx.download_gauge().tofile(path)
+        cmd = [ref, "benchprop"] + dims + [mass, path, "2", repr(prec)]
+        t0 = time.time()
         out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600).stdout
+        wall = time.time() - t0
     except (subprocess.TimeoutExpired, OSError) as e:
         log("cpu baseline failed:", e)
         return None
-    wall = time.time() - t0
+    finally:
+        try:
+            os.unlink(path)
+        except OSError:
+            pass
     m = re.search(r"IIREF_BENCH (\{.*\})", out)
     if not m:
         return None
@@ -121,12 +152,15 @@ def cpu_baseline(cfg, n_iter_per_rhs, nrhs=12):
         "value": value_s, "unit": "s per 12-rhs propagator solve (extrapolated)",
         "cores": threads, "kind": "reference",
         "sample": ("compiled IroIro++ (oracle/_ref) on %s, %d OpenMP threads (its Dslash is "
-                   "single-threaded): EO hop %.1f ms, Solver_CG %d iterations in %.1f s "
-                   "(%.1f s/iteration) on a random SU(3) field of the same lattice; "
-                   "x %d iterations/rhs (FP64 GPU count) x %d rhs; nproc=%d; sample wall %.1f s"
+                   "single-threaded): EO hop %.1f ms; Solver_CG on Fopr_DdagD for column 0 of "
+                   "the point-source propagator on this run's heat-bath gauge field, %d "
+                   "iterations in %.1f s (%.2f s/iteration, stopping precision %.1e); x %d "
+                   "iterations/rhs (FP64 GPU count) x %d rhs; nproc=%d; sample wall %.1f s"
                    % (model or "host CPU", threads, r["hop_ms"], iters, r["cg_ms"] / 1000.0,
-                      per_iter_ms / 1000.0, n_iter_per_rhs, nrhs, os.cpu_count() or 0, wall)),
+                      per_iter_ms / 1000.0, prec, n_iter_per_rhs, nrhs, os.cpu_count() or 0,
+                      wall)),
         "hop_ms": r["hop_ms"], "s_per_cg_iteration": per_iter_ms / 1000.0,
+        "cg_iterations_timed": iters,
     }
 
 
@@ -173,11 +207,16 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="half", choices=["single", "half", "bf16"])
-    ap.add_argument("--method", default="bicgstab", choices=["bicgstab", "cg"],
-                    help="optimized solver: BiCGStab on Dhat or CG on Dhat^dag Dhat")
-    ap.add_argument("--no-cg-mixed", action="store_true",
-                    help="skip the secondary mixed-precision CG measurement")
+    ap.add_argument("--precision", default="half", choices=["single", "half", "bf16"],
+                    help="storage of the batched BiCGStab / CG methods")
+    ap.add_argument("--method", default="fgcr-learned",
+                    choices=["fgcr-learned", "fgcr", "bicgstab", "cg"],
+                    help="timed solver: flexible GCR with the learned preconditioner (default), "
+                         "the same without it, BiCGStab on Dhat or CG on Dhat^dag Dhat")
+    ap.add_argument("--nkrylov", type=int, default=8,
+                    help="flexible GCR directions per restart cycle")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip timing the other methods")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=20260601)
@@ -189,8 +228,6 @@ def main():
                     help="only time the MD force (Dirac_Wilson_EvenOdd::md_force)")
     ap.add_argument("--rhmc-only", action="store_true",
                     help="only time RationalSolver_CG and the RHMC force (Action_Nf)")
-    ap.add_argument("--no-learned", action="store_true",
-                    help="skip training and timing the learned preconditioner")
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--train-pairs", type=int, default=64)
     args = ap.parse_args()
@@ -254,6 +291,17 @@ def main():
             log("net layer %d (%d->%d): %.3f ms, %.0f GB/s, %.1f TFLOP/s" %
                 (l, cin, cout, t, rows["layer%d_%dto%d" % (l, cin, cout)]["alg_gbs"],
                  rows["layer%d_%dto%d" % (l, cin, cout)]["mfma_tflops"]))
+        # the batched evaluation the flexible solver uses: 12 rhs per launch
+        st = net.bench_batched(12, 10)
+        for i, t in enumerate(st):
+            nb = NET_STAGE_BYTES[i]
+            alg = ctx.Vh * (12 * nb[0] + nb[1])
+            flops = ctx.Vh * 12 * NET_STAGE_MFMA_FLOPS[i]
+            rows["batched_stage%d" % i] = {"ms": t, "alg_gbs": alg / (t * 1e-3) / 1e9,
+                                           "mfma_tflops": flops / (t * 1e-3) / 1e12}
+            log("batched net stage %d (12 rhs): %.3f ms, %.0f GB/s, %.0f TFLOP/s"
+                % (i, t, rows["batched_stage%d" % i]["alg_gbs"],
+                   rows["batched_stage%d" % i]["mfma_tflops"]))
         print(json.dumps({"net_only": rows, "lattice": list(dims)}), flush=True)
         ctx.close()
         return
@@ -277,79 +325,125 @@ def main():
 
     src = ii.DeviceField(ctx, "full", 12)
     sol = ii.DeviceField(ctx, "full", 12)
+    x64 = ii.DeviceField(ctx, "full", 12)
     src.point_sources((0, 0, 0, 0))
     tune = ctx.autotune()
     log("autotune:", tune)
 
-    # ---- Dslash roofline (FP64 hop, one launch = V/2 output sites)
+    # ---- metric (2): Dslash roofline (FP64 hop, one launch = V/2 output sites)
     Vh = ctx.Vh
     hop_ms = ctx.bench_hop("double", 1, 50)
     hop_gbs = FP64_HOP_BYTES_PER_SITE * Vh / (hop_ms * 1e-3) / 1e9
-    hop_lp_ms = ctx.bench_hop(args.precision, 12, 20)
-    lp_bytes = (104 + 192 / 12.0) * Vh * 12  # 16-bit spinor+norm in/out, r12 links shared
     copy_bytes = 2 << 30
     copy_gbs = 2 * copy_bytes / (ctx.bench_copy(copy_bytes, 10) * 1e-3) / 1e9
-    log("FP64 hop %.3f ms = %.0f GB/s (%.1f%% of HBM peak); %s 12-rhs hop %.3f ms; "
-        "plain copy %.0f GB/s" % (hop_ms, hop_gbs, 100 * hop_gbs / HBM_PEAK_GBS, args.precision,
-                                  hop_lp_ms, copy_gbs))
-
+    log("FP64 hop %.3f ms = %.0f GB/s (%.1f%% of HBM peak); plain copy %.0f GB/s"
+        % (hop_ms, hop_gbs, 100 * hop_gbs / HBM_PEAK_GBS, copy_gbs))
     traffic, traffic_src = measured_traffic(dims)
-    lp_traffic, lp_traffic_src = (measured_traffic(dims, "k_hop_mrhs_half_32c64")
-                                  if args.precision == "half" else (None, None))
     force_ms, force_total_ms = ctx.bench_md_force(20)
     force_gbs = MD_FORCE_BYTES_PER_SITE * ctx.V / (force_ms * 1e-3) / 1e9
     log("MD force kernel %.3f ms = %.0f GB/s (%.1f%% of HBM peak)"
         % (force_ms, force_gbs, 100 * force_gbs / HBM_PEAK_GBS))
 
-    # ---- plain FP64 GPU CG, rhs after rhs (reference Qprop_EvenOdd shape)
+    # ---- plain FP64 GPU CG, one rhs after the other (Qprop_EvenOdd's
+    # shape), warmed up like the optimized path; its solution is kept
     fp64 = None
     if not args.no_fp64:
         s64 = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision="double", batched=False)
+        s64.solve(src, x64)
         t0 = time.time()
-        outs = s64.solve(src, sol)
+        outs64 = s64.solve(src, x64)
         fp64_s = time.time() - t0
-        its = [o.Iterations for o in outs]
+        its = [o.Iterations for o in outs64]
         fp64 = {"s": fp64_s, "iterations_per_rhs": its,
-                "max_true_residual": max(o.true_residual for o in outs)}
+                "max_true_residual": max(o.true_residual for o in outs64)}
         log("FP64 CG 12 rhs: %.3f s, iterations %s, max true residual %.2e"
             % (fp64_s, its, fp64["max_true_residual"]))
 
-    # ---- optimized solve: batched, mixed precision
-    solver = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=args.precision, batched=True,
-                                 method=args.method)
+    # ---- the learned preconditioner, trained on this gauge field from
+    # plain-CG pairs (its cost is reported apart from the solve time)
+    net = None
+    train = None
+    if args.method == "fgcr-learned":
+        net = ii.LearnedPreconditioner(ctx)
+        t0 = time.time()
+        rep = net.train(kappa, npairs=args.train_pairs, epochs=args.train_epochs, batch=4,
+                        lr=2e-3, pair_tol=1e-8, seed=args.seed)
+        train = dict(rep, s=time.time() - t0, pairs=args.train_pairs, epochs=args.train_epochs)
+        log("learned: trained on %d pairs, %d epochs in %.1f s (pairs %.1f s): loss %.4f -> %.4f"
+            % (args.train_pairs, args.train_epochs, train["s"], rep["pairs_ms"] / 1e3,
+               rep["loss_first"], rep["loss_last"]))
+
+    def make_solver(method, use_graph=True):
+        if method in ("fgcr-learned", "fgcr"):
+            return ii.PropagatorSolver(ctx, kappa, tol=1e-10, batched=True, method="fgcr",
+                                       use_learned=(method == "fgcr-learned"),
+                                       use_graph=use_graph, nkrylov=args.nkrylov)
+        return ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=args.precision, batched=True,
+                                   method=method)
+
+    # ---- the timed method
+    solver = make_solver(args.method)
     for _ in range(args.warmup):
         solver.solve(src, sol)
-    import ctypes  # noqa: F401
     times, outs = [], None
     t_all = time.time()
     for _ in range(args.steps):
         t0 = time.time()
-        outs = solver.solve(src, sol)
+        outs = solver.solve(src, sol)  # returns after the device has finished
         times.append(time.time() - t0)
     t_all = time.time() - t_all
     ms_per_step = 1000.0 * t_all / args.steps
     max_res = max(o.true_residual for o in outs)
-    log("optimized 12-rhs solve: %.3f s/step (%s), iterations %s, reliable updates %s, "
-        "max true residual %.2e" % (ms_per_step / 1000, ["%.3f" % t for t in times],
-                                   [o.Iterations for o in outs], [o.restarts for o in outs],
-                                   max_res))
+    log("timed %s 12-rhs solve: %.3f s/step (%s), directions/iterations %s, cycles/updates %s, "
+        "max true residual %.2e" % (args.method, ms_per_step / 1000,
+                                   ["%.3f" % t for t in times], [o.Iterations for o in outs],
+                                   [o.restarts for o in outs], max_res))
+    err_fp64 = None
+    if fp64 is not None:
+        import numpy as np
+        err_fp64 = []
+        for k in range(12):
+            a, b = sol.download(k), x64.download(k)
+            err_fp64.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+        log("|x - x_fp64|/|x_fp64| per rhs: %s" % ["%.1e" % e for e in err_fp64])
 
-    # ---- the other optimized method (batched mixed CG on Dhat^dag Dhat or
-    # BiCGStab), for comparison
-    other = None
-    if not args.no_cg_mixed:
-        om = "cg" if args.method == "bicgstab" else "bicgstab"
-        s2 = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=args.precision, batched=True,
-                                 method=om)
-        s2.solve(src, sol)
-        t0 = time.time()
-        o2 = s2.solve(src, sol)
-        dt2 = time.time() - t0
-        other = {"method": om, "s": dt2, "iterations_per_rhs": [o.Iterations for o in o2],
-                 "reliable_updates": [o.restarts for o in o2],
-                 "max_true_residual": max(o.true_residual for o in o2)}
-        log("batched mixed %s: %.3f s, iterations %s" % (om, dt2, other["iterations_per_rhs"]))
-        outs = solver.solve(src, sol)  # the headline method's solution for the contraction
+    # ---- the other methods on the same problem (one warm-up, one timed solve)
+    others = {}
+    if not args.no_others:
+        for m in ["fgcr-learned", "fgcr", "bicgstab", "cg"]:
+            if m == args.method or (m == "fgcr-learned" and net is None):
+                continue
+            s2 = make_solver(m)
+            s2.solve(src, sol)
+            t0 = time.time()
+            o2 = s2.solve(src, sol)
+            dt2 = time.time() - t0
+            others[m] = {"s": dt2, "iterations_per_rhs": [o.Iterations for o in o2],
+                         "restarts": [o.restarts for o in o2],
+                         "max_true_residual": max(o.true_residual for o in o2)}
+            log("%s: %.3f s, iterations %s" % (m, dt2, others[m]["iterations_per_rhs"]))
+        outs = solver.solve(src, sol)  # the timed method's solution for the contraction
+
+    # ---- the network's stage kernels (HIP events, the rhs count of the solve)
+    net_stages = None
+    dominant = None
+    if net is not None:
+        st = net.bench_batched(12, 5)
+        names = ["k_bnet_first (odd, [L1|L2] from 8 transports of r)",
+                 "k_bnet_mid<0> (even, L2: 8 transported channels)",
+                 "k_bnet_mid<1> (odd, L3: 8 transported channels)",
+                 "k_bnet_last (even, 16->1 mixing + transport)"]
+        net_stages = []
+        for i, t in enumerate(st):
+            nb = NET_STAGE_BYTES[i]
+            alg = Vh * (12 * nb[0] + nb[1])
+            flops = Vh * 12 * NET_STAGE_MFMA_FLOPS[i]
+            net_stages.append({"kernel": names[i], "ms": t, "alg_bytes": alg,
+                               "alg_gbs": alg / (t * 1e-3) / 1e9,
+                               "mfma_tflops": flops / (t * 1e-3) / 1e12})
+            log("net stage %d: %.3f ms, %.0f GB/s algorithmic, %.0f TFLOP/s MFMA"
+                % (i, t, net_stages[-1]["alg_gbs"], net_stages[-1]["mfma_tflops"]))
+        dominant = max(net_stages, key=lambda r: r["ms"])
 
     # ---- meson correlators of the solved propagator (stays on the device)
     import numpy as np
@@ -364,39 +458,51 @@ def main():
     log("meson correlators (18 channels, time slices): %.3f ms, pion C(0..3) = %s"
         % (meson_ms, np.array2string(corr[0][:4], precision=4)))
 
-    # ---- learned preconditioner: trained on this gauge field, flexible GCR
-    learned = None
-    if not args.no_learned:
-        net = ii.LearnedPreconditioner(ctx)
-        t0 = time.time()
-        rep = net.train(kappa, npairs=args.train_pairs, epochs=args.train_epochs, batch=4,
-                        lr=2e-3, pair_tol=1e-8, seed=args.seed)
-        train_s = time.time() - t0
-        log("learned: trained on %d pairs, %d epochs in %.1f s (pairs %.1f s): loss %.4f -> %.4f"
-            % (args.train_pairs, args.train_epochs, train_s, rep["pairs_ms"] / 1e3,
-               rep["loss_first"], rep["loss_last"]))
-        ls = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision="double", batched=False,
-                                 use_learned=True)
-        t0 = time.time()
-        louts = ls.solve(src, sol)
-        l_s = time.time() - t0
-        learned = {"s": l_s, "directions_per_rhs": [o.Iterations for o in louts],
-                   "cg_fallback_iterations": [o.inner_iterations for o in louts],
-                   "max_true_residual": max(o.true_residual for o in louts),
-                   "train": rep, "train_s": train_s}
-        log("learned 12-rhs solve: %.3f s, directions %s, max true residual %.2e"
-            % (l_s, learned["directions_per_rhs"], learned["max_true_residual"]))
-
     cpu = None
     if not args.no_cpu and fp64 is not None:
         n_it = int(round(sum(fp64["iterations_per_rhs"]) / 12.0))
-        cpu = cpu_baseline(cfg, n_it)
+        cpu = cpu_baseline(ctx, cfg, n_it)
         if cpu:
             log("cpu baseline:", cpu["sample"])
 
+    fp64_hop_roofline = {
+        "bound": "hbm", "achieved": hop_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": hop_gbs / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+        "algorithmic_bytes": FP64_HOP_BYTES_PER_SITE * Vh,
+        "kernel": "k_hop FP64 even/odd Wilson hop, V/2 = %d sites/launch, 1536 B/site "
+                  "algorithmic" % Vh,
+        "avg_ms": hop_ms, "gflops": HOP_FLOPS_PER_SITE * Vh / (hop_ms * 1e-3) / 1e9,
+        "copy_kernel_gbs": copy_gbs}
+    if dominant is not None:
+        ntraffic, ntraffic_src = measured_traffic(dims, "k_bnet_dominant_32c64",
+                                                  "learned_net_batched_32c64.md")
+        roofline = {"bound": "hbm", "achieved": dominant["alg_gbs"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": dominant["alg_gbs"] / HBM_PEAK_GBS,
+                    "traffic": ntraffic, "traffic_source": ntraffic_src,
+                    "algorithmic_bytes": dominant["alg_bytes"],
+                    "kernel": dominant["kernel"] + ", the timed solve's dominant kernel, "
+                              "12 rhs x V/2 = %d sites per launch" % Vh,
+                    "avg_ms": dominant["ms"], "mfma_tflops": dominant["mfma_tflops"],
+                    "mfma_frac_of_bf16_peak": dominant["mfma_tflops"] / 2500.0}
+    else:
+        roofline = fp64_hop_roofline
+
+    method_desc = {
+        "fgcr-learned": "batched 12-rhs flexible GCR on Dhat with the trained gauge-covariant "
+                        "network as preconditioner (bf16 activations and links, MFMA channel "
+                        "mixing), FP32 Krylov vectors and FP32 Dhat, restart cycles of %d "
+                        "directions captured as HIP graphs, FP64 true-residual restarts"
+                        % args.nkrylov,
+        "fgcr": "batched 12-rhs flexible GCR on Dhat without preconditioner, FP32 Krylov "
+                "vectors, FP64 true-residual restarts",
+        "bicgstab": "batched 12-rhs BiCGStab on Dhat, %s storage, fp64 reliable updates"
+                    % args.precision,
+        "cg": "batched 12-rhs CG on Dhat^dag Dhat, %s storage, fp64 reliable updates"
+              % args.precision}[args.method]
+    value_s = ms_per_step / 1000.0
     result = {
         "metric": "solve time @1e-10 true resid: mixed+batched+learned vs FP64 CG; Dslash GB/s",
-        "value": ms_per_step / 1000.0,
+        "value": value_s,
         "unit": "s per 12-rhs propagator solve",
         "n_gpus": 1,
         "steps": args.steps,
@@ -405,44 +511,32 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "%s storage / fp32 compute / fp64 reliable updates" % args.precision,
+        "dtype": ("bf16 network / fp32 Krylov / fp64 restarts" if args.method.startswith("fgcr")
+                  else "%s storage / fp32 compute / fp64 reliable updates" % args.precision),
         "data": "synthetic: SU(3) heat-bath gauge field generated on the device, point sources",
         "config": {"workload": cfg["workload"], "lattice": list(dims), "beta": cfg["beta"],
-                   "kappa": kappa, "nrhs": 12, "tol": 1e-10, "method":
-                   ("batched 12-rhs BiCGStab on Dhat" if args.method == "bicgstab" else
-                    "batched 12-rhs CG on Dhat^dag Dhat")
-                   + ", %s storage, fp64 reliable updates" % args.precision,
+                   "kappa": kappa, "nrhs": 12, "tol": 1e-10, "method": method_desc,
                    "plaquette": plaq, "device": name},
         "iterations_per_rhs": [o.Iterations for o in outs],
+        "restarts_per_rhs": [o.restarts for o in outs],
         "max_true_residual": max_res,
-        "other_mixed_method": other,
+        "rel_error_vs_fp64_per_rhs": err_fp64,
         "fp64_cg": fp64,
-        "speedup_vs_fp64_cg": (fp64["s"] / (ms_per_step / 1000.0)) if fp64 else None,
-        "roofline": {"bound": "hbm", "achieved": hop_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hop_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": traffic_src,
-                     "algorithmic_bytes": FP64_HOP_BYTES_PER_SITE * Vh,
-                     "kernel": "k_hop FP64 even/odd Wilson hop, V/2 = %d sites/launch, "
-                               "1536 B/site algorithmic" % Vh,
-                     "avg_ms": hop_ms,
-                     "gflops": HOP_FLOPS_PER_SITE * Vh / (hop_ms * 1e-3) / 1e9,
-                     "copy_kernel_gbs": copy_gbs},
-        "roofline_batched_lowprec": {
-            "kernel": "k_hop_mrhs %s 12-rhs (the solve's dominant kernel), V/2 x 12 site-rhs "
-                      "per launch, 120 B/site-rhs algorithmic" % args.precision,
-            "bound": "vector-memory path (rocprof counters: texture addresser ~85% busy)",
-            "avg_ms": hop_lp_ms,
-            "achieved": lp_bytes / (hop_lp_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-            "frac": lp_bytes / (hop_lp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "unit": "GB/s", "bytes_per_site_per_rhs": 104 + 16,
-            "traffic": lp_traffic, "traffic_source": lp_traffic_src},
+        "speedup_vs_fp64_cg": (fp64["s"] / value_s) if fp64 else None,
+        "other_methods": others,
+        "speedup_vs_same_without_network": (others["fgcr"]["s"] / value_s
+                                            if args.method == "fgcr-learned" and "fgcr" in others
+                                            else None),
+        "learned_training": train,
+        "roofline": roofline,
+        "net_stages": net_stages,
+        "dslash_fp64_roofline": fp64_hop_roofline,
         "md_force": {"kernel": "k_md_force, Dirac_Wilson_EvenOdd::md_force over V = %d sites, "
                                "1920 B/site algorithmic" % ctx.V,
                      "avg_ms": force_ms, "achieved": force_gbs, "unit": "GB/s",
                      "frac": force_gbs / HBM_PEAK_GBS, "with_hops_ms": force_total_ms},
         "meson": meson,
         "cpu_baseline": cpu,
-        "learned": learned,
         "autotune": tune,
     }
     print(json.dumps(result), flush=True)

@@ -61,6 +61,8 @@ typedef struct {
   double true_residual;   /* |b - D x|/|b| on the full operator (propagator solves) */
   int restarts;           /* reliable updates / defect-correction cycles */
   int inner_iterations;   /* low-precision iterations (mixed solvers) */
+  int recursion_restarts; /* reliable updates whose recomputed residual was far above the
+                             recursive one, so the search direction was restarted */
 } ii_solver_output;
 
 /* ---------------------------------------------------------------- context */
@@ -310,12 +312,26 @@ typedef struct {
                             for BiCGStab in BF16).  Batched: rhs within a factor 4 of
                             their own trigger join the update of the rhs that trigger */
   int use_graph;         /* capture solver iterations in a HIP graph (II_METHOD_CG) */
-  int use_learned;       /* apply the trained learned preconditioner (flexible outer) */
+  int use_learned;       /* apply the trained learned preconditioner: with II_METHOD_FGCR
+                            inside the batched flexible GCR; with the other methods the
+                            rhs-by-rhs FP64 flexible GCR (ii_fgcr_solve) */
   int method;            /* II_METHOD_CG: CG on Dhat^dag Dhat (Solver_CG on Fopr_DdagD);
                             II_METHOD_BICGSTAB: BiCGStab on Dhat itself (no reference
-                            counterpart; sloppy precisions, batched) */
+                            counterpart; sloppy precisions, batched);
+                            II_METHOD_FGCR: flexible GCR on Dhat (no reference counterpart;
+                            batched only): every rhs in each launch, FP32 Krylov vectors
+                            and FP32 Dhat, the network (use_learned) on bfloat16
+                            activations, restarted every nkrylov directions from the
+                            FP64 true residual; use_graph captures a restart cycle as
+                            one HIP graph; batched mixed CG on the remaining defect if
+                            the flexible iteration stalls */
+  double restart_factor; /* a reliable update whose recomputed residual |r|^2 exceeds
+                            restart_factor times the recursive one restarts the search
+                            direction (0 = default 16) */
+  int nkrylov;           /* flexible GCR (use_learned): directions per restart cycle
+                            (0 = default) */
 } ii_prop_params;
-enum { II_METHOD_CG = 0, II_METHOD_BICGSTAB = 1 };
+enum { II_METHOD_CG = 0, II_METHOD_BICGSTAB = 1, II_METHOD_FGCR = 2 };
 
 /* Full solve D x = b for every rhs of a full field through the even/odd
  * Schur system (Inverter_WilsonLike::invert, eoUtils.cpp:43-80) with a
@@ -395,6 +411,13 @@ ii_status ii_net_loss_grad(ii_context* ctx, int npairs, const double* b_even,
                            const double* y_even, double* loss, float* grads);
 /* z = M r on even half fields (esec order, 24*V/2 doubles). */
 ii_status ii_net_apply(ii_context* ctx, const double* r_even, double* z_even);
+/* z = M r for nrhs even half fields back to back, all rhs in one launch per
+ * stage: the evaluation the batched flexible solver uses.  It regroups the
+ * layers by parity (L1 and L3 are needed on odd sites only, composite
+ * weights W2_0 W1_d + W2_d W1_0 etc. formed in double and rounded to
+ * bfloat16), stores activations in bfloat16 and transports with bfloat16
+ * links, so it equals ii_net_apply up to bfloat16 rounding. */
+ii_status ii_net_apply_batched(ii_context* ctx, int nrhs, const double* r_even, double* z_even);
 /* Produces npairs Gaussian sources bhat and their plain FP64 CG solutions
  * y = Dhat^-1 bhat on the device, then trains with Adam on the relative
  * loss.  Everything runs on the device. */
@@ -406,6 +429,10 @@ ii_status ii_net_load(ii_context* ctx, const char* path);
  * nrep launches, HIP events on the context stream; the last layer on the
  * even output sites only, as M applies it). */
 ii_status ii_net_bench(ii_context* ctx, int nrep, double* layer_ms);
+/* Times the four stage kernels of ii_net_apply_batched for nrhs rhs
+ * (stage_ms[4], average over nrep launches, HIP events on the context
+ * stream): odd [L1 | L2], even L2, odd L3, even output. */
+ii_status ii_bench_net_batched(ii_context* ctx, int nrhs, int nrep, double* stage_ms);
 /* Flexible GCR on Dhat x = b (even half fields), FP64 vectors and Dhat,
  * right preconditioner M (use_net != 0) or none, restarted every `nkrylov`
  * directions with an FP64 true residual; stops when |b - Dhat x| <= tol |b|.

@@ -64,6 +64,7 @@ class SolverOutput:
         self.true_residual = c.true_residual
         self.restarts = c.restarts
         self.inner_iterations = c.inner_iterations
+        self.recursion_restarts = c.recursion_restarts
         self.Msg = ""
 
     def __repr__(self):
@@ -917,12 +918,14 @@ class PropagatorSolver:
     Fopr_DdagD) or 'bicgstab' (BiCGStab on Dhat, sloppy precisions only)."""
 
     def __init__(self, ctx, kappa, tol=1e-10, precision="double", batched=False, maxiter=100000,
-                 delta=0.0, use_graph=False, use_learned=False, method="cg"):
+                 delta=0.0, use_graph=False, use_learned=False, method="cg",
+                 restart_factor=0.0, nkrylov=0):
         self.ctx = ctx
         self.kappa = float(kappa)
         self.params = _lib.PropParamsC(tol, maxiter, PRECISIONS[precision], int(batched),
                                        float(delta), int(use_graph), int(use_learned),
-                                       _lib.II_METHOD[method])
+                                       _lib.II_METHOD[method], float(restart_factor),
+                                       int(nkrylov))
 
     def solve(self, src, sol):
         outs = (_lib.SolverOutputC * src.nrhs)()
@@ -1015,6 +1018,22 @@ class LearnedPreconditioner:
         z = np.empty_like(r)
         self.ctx.check(self._L().ii_net_apply(self.ctx.handle, _lib.dptr(r), _lib.dptr(z)),
                        "ii_net_apply")
+        return z
+
+    def bench_batched(self, nrhs, nrep=5):
+        """Average ms of the four stage kernels of apply_batched."""
+        ms = np.zeros(4)
+        self.ctx.check(self._L().ii_bench_net_batched(self.ctx.handle, int(nrhs), int(nrep),
+                                                      _lib.dptr(ms)), "ii_bench_net_batched")
+        return ms
+
+    def apply_batched(self, r):
+        """z_k = M r_k for a stack of even half fields, all rhs per launch
+        (the evaluation of the batched flexible solver)."""
+        r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(-1, 24 * self.ctx.V // 2))
+        z = np.empty_like(r)
+        self.ctx.check(self._L().ii_net_apply_batched(self.ctx.handle, r.shape[0], _lib.dptr(r),
+                                                      _lib.dptr(z)), "ii_net_apply_batched")
         return z
 
     def train(self, kappa, npairs=64, epochs=20, batch=4, lr=1e-3, pair_tol=1e-8, seed=1,

@@ -32,8 +32,9 @@ EXPORTS = [
     "ii_multishift_mixed",
     "ii_qprop_multishift",
     "ii_net_init", "ii_net_set_weights", "ii_net_get_weights", "ii_net_layer_apply",
-    "ii_net_layer_wgrad", "ii_net_loss_grad", "ii_net_apply", "ii_net_train", "ii_net_save",
-    "ii_net_load", "ii_net_bench", "ii_fgcr_solve",
+    "ii_net_layer_wgrad", "ii_net_loss_grad", "ii_net_apply", "ii_net_apply_batched",
+    "ii_net_train", "ii_net_save",
+    "ii_net_load", "ii_net_bench", "ii_bench_net_batched", "ii_fgcr_solve",
     "ii_autotune", "ii_bench_hop", "ii_bench_copy",
     "ii_dirac_md_force", "ii_dirac_full_md_force", "ii_dirac_full_md_force_pm",
     "ii_traceless_antihermite", "ii_action_nf2_md_force", "ii_bench_md_force",
@@ -66,17 +67,19 @@ class NotConverged(IIError):
 class SolverOutputC(ctypes.Structure):
     _fields_ = [("diff", ctypes.c_double), ("iterations", ctypes.c_int),
                 ("timing_ms", ctypes.c_double), ("true_residual", ctypes.c_double),
-                ("restarts", ctypes.c_int), ("inner_iterations", ctypes.c_int)]
+                ("restarts", ctypes.c_int), ("inner_iterations", ctypes.c_int),
+                ("recursion_restarts", ctypes.c_int)]
 
 
 class PropParamsC(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("precision", ctypes.c_int),
                 ("batched", ctypes.c_int), ("delta", ctypes.c_double),
                 ("use_graph", ctypes.c_int), ("use_learned", ctypes.c_int),
-                ("method", ctypes.c_int)]
+                ("method", ctypes.c_int), ("restart_factor", ctypes.c_double),
+                ("nkrylov", ctypes.c_int)]
 
 
-II_METHOD = {"cg": 0, "bicgstab": 1}
+II_METHOD = {"cg": 0, "bicgstab": 1, "fgcr": 2}
 
 
 class TrainParamsC(ctypes.Structure):
@@ -170,10 +173,12 @@ def load():
         "ii_net_layer_wgrad": [vp, ci, dp, dp, fp],
         "ii_net_loss_grad": [vp, ci, dp, dp, dp, fp],
         "ii_net_apply": [vp, dp, dp],
+        "ii_net_apply_batched": [vp, ci, dp, dp],
         "ii_net_train": [vp, cd, ctypes.POINTER(TrainParamsC), ctypes.POINTER(TrainReportC)],
         "ii_net_save": [vp, ctypes.c_char_p],
         "ii_net_load": [vp, ctypes.c_char_p],
         "ii_net_bench": [vp, ci, dp],
+        "ii_bench_net_batched": [vp, ci, ci, dp],
         "ii_fgcr_solve": [vp, cd, dp, dp, cd, ci, ci, ci, so],
         "ii_autotune": [vp, ctypes.c_char_p, ctypes.c_size_t],
         "ii_bench_hop": [vp, ci, ci, ci, dp],

@@ -45,6 +45,19 @@ ii_status ensure_partials(ii_context* ctx, size_t n) {
   return II_OK;
 }
 
+ii_status ensure_host_pin(ii_context* ctx, size_t bytes, void** ptr) {
+  if (ctx->host_pin_len < bytes) {
+    II_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->host_pin) II_HIP(ctx, hipHostFree(ctx->host_pin));
+    ctx->host_pin = nullptr;
+    ctx->host_pin_len = 0;
+    II_HIP(ctx, hipHostMalloc(&ctx->host_pin, bytes));
+    ctx->host_pin_len = bytes;
+  }
+  *ptr = ctx->host_pin;
+  return II_OK;
+}
+
 ii_status ensure_scalars(ii_context* ctx, int n) {
   if (ctx->nscal < n) {
     II_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -150,6 +163,9 @@ ii_status ii_context_destroy(ii_context* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   learned_release(ctx);
+  bnet_release(ctx);
+  fgcr_release(ctx->fgcr);
+  ctx->fgcr = nullptr;
   for (ii_field* f : ctx->fields) {
     (void)hipFree(f->data);
     f->data = nullptr;
@@ -158,12 +174,13 @@ ii_status ii_context_destroy(ii_context* ctx) {
   ctx->fields.clear();
   for (auto& b : ctx->work)
     if (b.ptr) (void)hipFree(b.ptr);
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < ii_context::kGaugeCopies; ++i)
     if (ctx->gauge_lp[i]) (void)hipFree(ctx->gauge_lp[i]);
   if (ctx->gauge64) (void)hipFree(ctx->gauge64);
   if (ctx->partials) (void)hipFree(ctx->partials);
   if (ctx->scal) (void)hipFree(ctx->scal);
   if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
+  if (ctx->host_pin) (void)hipHostFree(ctx->host_pin);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -192,7 +209,7 @@ ii_status ii_gauge_upload(ii_context* ctx, const double* u) {
   double* d = nullptr;
   II_CHECK(stage_in(ctx, u, (size_t)18 * 4 * ctx->g.V, &d));
   II_CHECK(pack_gauge(ctx, d));
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < ii_context::kGaugeCopies; ++i)
     if (ctx->gauge_lp[i]) {
       II_HIP(ctx, hipStreamSynchronize(ctx->stream));
       (void)hipFree(ctx->gauge_lp[i]);

@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(kBlock) k_apbc(double2* __restrict__ u, Geom g
 }
 
 static void drop_lowprec(ii_context* ctx) {
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < ii_context::kGaugeCopies; ++i)
     if (ctx->gauge_lp[i]) {
       (void)hipStreamSynchronize(ctx->stream);
       (void)hipFree(ctx->gauge_lp[i]);

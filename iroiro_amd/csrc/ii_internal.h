@@ -84,7 +84,12 @@ struct ii_context {
   // gauge fields
   double2* gauge64 = nullptr;     // FP64 r18, [2][4][9][stride]
   bool gauge_valid = false;
-  void* gauge_lp[4] = {nullptr, nullptr, nullptr, nullptr};  // per II_PREC_* (1..3)
+  // derived copies of the gauge field, rebuilt lazily after it changes:
+  // [II_PREC_SINGLE..II_PREC_BF16] the reconstruct-12 links of the hops,
+  // [kGaugePairs] the bfloat16 transport matrices of the batched network
+  static constexpr int kGaugeCopies = 5;
+  static constexpr int kGaugePairs = 4;
+  void* gauge_lp[kGaugeCopies] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int tbc_sign = 1;               // sign of the t-links on the last time slice (r12)
   bool r12_ok = false;
   // workspaces, by slot index (see ii::workspace)
@@ -95,14 +100,19 @@ struct ii_context {
   ii::CGScalars* scal = nullptr;  // device solver scalars (array)
   ii::CGScalars* scal_host = nullptr;  // pinned mirror
   int nscal = 0;
+  void* host_pin = nullptr;       // pinned host block of the multishift scalars
+  size_t host_pin_len = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // autotuned launch shapes
   ii::HopOrder hop_order;
   ii::HopOrder mrhs_order;  // tile order of the multi-rhs hop (64-site tiles)
   bool hop_tuned = false;
   std::string tune_report;
-  // learned preconditioner state (owned by ii_learned.hip)
+  // learned preconditioner state (owned by ii_learned.hip) and the batched
+  // network and flexible solver workspaces (owned by ii_bnet.hip)
   void* learned = nullptr;
+  void* bnet = nullptr;
+  void* fgcr = nullptr;
   // live device fields of this context: destroying the context releases
   // their memory and detaches them, so a later ii_field_destroy is safe
   std::set<ii_field*> fields;
@@ -137,6 +147,9 @@ ii_status set_error(ii_context* ctx, ii_status st, const std::string& msg);
 ii_status workspace(ii_context* ctx, int slot, size_t bytes, void** ptr);
 ii_status ensure_partials(ii_context* ctx, size_t n);
 ii_status ensure_scalars(ii_context* ctx, int n);
+// Pinned host block of at least `bytes` owned by the context (grows, never
+// shrinks), so solves allocate no pinned memory per call.
+ii_status ensure_host_pin(ii_context* ctx, size_t bytes, void** ptr);
 ii_status require_gauge(ii_context* ctx);
 // Entry of every API call on a context: selects its device and clears a
 // stale HIP error left by an earlier, already reported failure.

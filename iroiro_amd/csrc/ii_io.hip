@@ -260,10 +260,10 @@ bool lime_write_record(FILE* f, const char* type, bool mb, bool me, const void* 
 
 using namespace ii;
 
-extern "C" {
+namespace {
 
-ii_status ii_gauge_read(ii_context* ctx, const char* path, int format, int check,
-                        ii_io_info* info) {
+ii_status gauge_read(ii_context* ctx, const char* path, int format, int check,
+                     ii_io_info* info) {
   if (!ctx || !path || format < II_IO_BINARY || format > II_IO_ILDG) return II_ERR_BAD_ARG;
   II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
@@ -297,7 +297,15 @@ ii_status ii_gauge_read(ii_context* ctx, const char* path, int format, int check
   } else if (format == II_IO_ILDG) {
     // walk the records to "ildg-binary-data" (ILDG_reader.cpp:44-54); an
     // "ildg-format" record announcing <precision>32</precision> selects
-    // 32-bit data (the reference assumes 64)
+    // 32-bit data (the reference assumes 64).  Record lengths come from the
+    // file, so each is checked against the bytes the file still holds; only
+    // the small ildg-format body is read, every other record is skipped.
+    if (fseeko(f, 0, SEEK_END) != 0)
+      return set_error(ctx, II_ERR_BAD_ARG, std::string("cannot seek in ") + path);
+    const uint64_t fsize = (uint64_t)ftello(f);
+    if (fseeko(f, 0, SEEK_SET) != 0)
+      return set_error(ctx, II_ERR_BAD_ARG, std::string("cannot seek in ") + path);
+    constexpr uint64_t kMaxFormatRecord = 1u << 20;
     for (;;) {
       unsigned char h[144];
       if (std::fread(h, 1, 144, f) != 144)
@@ -305,20 +313,32 @@ ii_status ii_gauge_read(ii_context* ctx, const char* path, int format, int check
       if ((uint32_t)get_be(h, 4) != kLimeMagic)
         return set_error(ctx, II_ERR_BAD_ARG, "ILDG: bad LIME magic number");
       const uint64_t len = get_be(h + 8, 8);
+      const uint64_t here = (uint64_t)ftello(f);
+      if (here > fsize || len > fsize - here)
+        return set_error(ctx, II_ERR_BAD_ARG, "ILDG: record length beyond the end of the file");
       char type[129];
       std::memcpy(type, h + 16, 128);
       type[128] = 0;
-      if (std::strncmp(type, "ildg-binary-data", 16) == 0) break;
-      std::vector<char> body(len);
-      if (len && std::fread(body.data(), 1, len, f) != len)
-        return set_error(ctx, II_ERR_BAD_ARG, "ILDG: truncated record");
-      if (std::strncmp(type, "ildg-format", 11) == 0) {
-        const std::string xml(body.begin(), body.end());
+      if (std::strncmp(type, "ildg-binary-data", 16) == 0) {
+        const uint64_t want = (uint64_t)g.V * 4 * 18 * (inf.bits / 8);
+        if (len != want)
+          return set_error(ctx, II_ERR_BAD_ARG,
+                           "ILDG: ildg-binary-data holds " + std::to_string(len) +
+                               " bytes, the lattice needs " + std::to_string(want));
+        break;
+      }
+      if (std::strncmp(type, "ildg-format", 11) == 0 && len <= kMaxFormatRecord) {
+        std::string xml(len, '\0');
+        if (len && std::fread(&xml[0], 1, len, f) != len)
+          return set_error(ctx, II_ERR_BAD_ARG, "ILDG: truncated record");
         const size_t p = xml.find("<precision>");
         if (p != std::string::npos && std::atoi(xml.c_str() + p + 11) == 32) inf.bits = 32;
+      } else if (fseeko(f, (off_t)len, SEEK_CUR) != 0) {
+        return set_error(ctx, II_ERR_BAD_ARG, "ILDG: truncated record");
       }
       const uint64_t pad = (8 - len % 8) % 8;
-      if (pad) std::fseek(f, (long)pad, SEEK_CUR);
+      if (pad && fseeko(f, (off_t)pad, SEEK_CUR) != 0)
+        return set_error(ctx, II_ERR_BAD_ARG, "ILDG: truncated record");
     }
   }
   const int nreal = inf.is3x2 ? 12 : 18;
@@ -336,7 +356,7 @@ ii_status ii_gauge_read(ii_context* ctx, const char* path, int format, int check
                      (const unsigned char*)raw, ctx->gauge64, g, bytes, nreal, big ? 1 : 0);
   II_HIP(ctx, hipGetLastError());
   // the field is new: drop the low-precision copies as ii_gauge_upload does
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < ii_context::kGaugeCopies; ++i)
     if (ctx->gauge_lp[i]) {
       II_HIP(ctx, hipStreamSynchronize(ctx->stream));
       (void)hipFree(ctx->gauge_lp[i]);
@@ -361,7 +381,7 @@ ii_status ii_gauge_read(ii_context* ctx, const char* path, int format, int check
   return II_OK;
 }
 
-ii_status ii_gauge_write(ii_context* ctx, const char* path, int format, ii_io_info* info) {
+ii_status gauge_write(ii_context* ctx, const char* path, int format, ii_io_info* info) {
   if (!path || format < II_IO_BINARY || format > II_IO_ILDG) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
   II_CHECK(enter(ctx));
@@ -411,6 +431,29 @@ ii_status ii_gauge_write(ii_context* ctx, const char* path, int format, ii_io_in
   if (!ok) return set_error(ctx, II_ERR_BAD_ARG, std::string("write to ") + path + " failed");
   if (info) *info = inf;
   return II_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// File contents decide allocation sizes and string parsing, so a C++
+// exception (bad_alloc, length_error) must not cross the C boundary.
+ii_status ii_gauge_read(ii_context* ctx, const char* path, int format, int check,
+                        ii_io_info* info) {
+  try {
+    return gauge_read(ctx, path, format, check, info);
+  } catch (const std::exception& e) {
+    return set_error(ctx, II_ERR_BAD_ARG, std::string("gauge read: ") + e.what());
+  }
+}
+
+ii_status ii_gauge_write(ii_context* ctx, const char* path, int format, ii_io_info* info) {
+  try {
+    return gauge_write(ctx, path, format, info);
+  } catch (const std::exception& e) {
+    return set_error(ctx, II_ERR_BAD_ARG, std::string("gauge write: ") + e.what());
+  }
 }
 
 }  // extern "C"

@@ -72,6 +72,7 @@ struct LayerParams {
 struct Net {
   LayerParams L[kNL];
   bool init = false;
+  int version = 0;             // bumped whenever the weights change
   int adam_step = 0;
   // activations / gradients, FP32 full fields [ch][2][12][stride]
   float2* act[kNL + 1] = {};   // act[0] 1 ch, act[1..3] 8 ch, act[4] 1 ch
@@ -712,6 +713,7 @@ ii_status run_wgrad(ii_context* ctx, Net* N, int l, const float2* in, const floa
 }
 
 ii_status build_frags(ii_context* ctx, Net* N) {
+  N->version++;
   for (int l = 0; l < kNL; ++l) {
     LayerParams& P = N->L[l];
     const int nf = frag_count(P.cin, P.cout), na = frag_count(P.cout, P.cin);
@@ -1028,6 +1030,22 @@ ii_status solve_even(ii_context* ctx, double kappa, double2* x, const double2* b
   o.true_residual = o.diff = std::sqrt(rr[0] / bb[0]);
   if (out) *out = o;
   if (!ok) return set_error(ctx, II_ERR_NOT_CONVERGED, "flexible GCR and CG fallback failed");
+  return II_OK;
+}
+
+ii_status net_host_weights(ii_context* ctx, int* version,
+                           std::vector<std::vector<float2>>* w) {
+  Net* N = net_of(ctx);
+  if (!N || !N->init) return set_error(ctx, II_ERR_STATE, "network not initialised");
+  *version = N->version;
+  if (!w) return II_OK;
+  w->assign(kNL, {});
+  for (int l = 0; l < kNL; ++l) {
+    (*w)[l].resize(N->L[l].nw);
+    II_HIP(ctx, hipMemcpyAsync((*w)[l].data(), N->L[l].w, N->L[l].nw * sizeof(float2),
+                               hipMemcpyDeviceToHost, ctx->stream));
+  }
+  II_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return II_OK;
 }
 

@@ -141,6 +141,22 @@ ii_status multishift_mixed_dev(ii_context* ctx, double kappa, double2* x, const 
 
 // learned preconditioner (ii_learned.hip)
 void learned_release(ii_context* ctx);
+// FP32 master weights of the 4 layers on the host, [9][cout][cin][4][4]
+// complex each, and their version (bumped whenever the weights change;
+// w = nullptr: the version only)
+ii_status net_host_weights(ii_context* ctx, int* version, std::vector<std::vector<float2>>* w);
+// batched network (ii_bnet.hip): z = M r for nrhs FP32 even half fields
+// [rhs][12][stride] on the device
+ii_status bnet_apply(ii_context* ctx, int nrhs, const float2* r, float2* z);
+// builds what bnet_apply needs (links, weights, buffers) so that its
+// launches can be captured in a graph
+ii_status bnet_prepare(ii_context* ctx, int nrhs);
+void bnet_release(ii_context* ctx);
+// batched flexible GCR with (use_learned) the network as preconditioner
+// (ii_fgcr.hip); *state holds its workspaces, released by fgcr_release
+ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_prop_params* prm,
+                       const ii_field* src, ii_field* sol, ii_solver_output* outs);
+void fgcr_release(void* state);
 ii_status learned_propagator(ii_context* ctx, double kappa, const ii_prop_params* p,
                              const ii_field* src, ii_field* sol, ii_solver_output* out);
 

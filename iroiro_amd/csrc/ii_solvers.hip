@@ -33,6 +33,7 @@ struct RUScal {
   double b2;       // |b|^2 of the full source
   double snorm;    // 1/|b|^2 (reference CG)
   double prec;     // reference CG threshold on rr*snorm
+  double restart;  // a recomputed |r|^2 above restart x the recursive one restarts p
   int done;        // iteration kernels skip when set
   int skip_ru;     // reliable-update kernels skip when set (= done || !update)
   int count;       // iterations done
@@ -359,7 +360,7 @@ __global__ void k_fin_ru(const double* __restrict__ partials, int nb, RUScal* __
     // corrupted (in the sloppy arithmetic or otherwise); the old search
     // direction is then useless and keeping it makes CG diverge, so the
     // caller restarts it from the new residual (p = r).
-    if (s > 16.0 * S.rr) {
+    if (s > S.restart * S.rr) {
       S.keep_p = 0;
       S.restarts += 1;
     }
@@ -658,7 +659,8 @@ ii_status join_updates(ii_context* ctx, int n) {
 }
 
 __global__ void k_scal_init(RUScal* sc, const double* __restrict__ rr, const double* __restrict__ bb,
-                            double prec, double tol, double delta, int mode) {
+                            double prec, double tol, double delta, int mode,
+                            double restart = 16.0) {
   RUScal& S = sc[blockIdx.x];
   if (threadIdx.x != 0) return;
   RUScal z{};
@@ -671,6 +673,7 @@ __global__ void k_scal_init(RUScal* sc, const double* __restrict__ rr, const dou
   z.tol2 = tol * tol * z.b2;
   z.target2 = z.tol2;
   z.delta2 = delta * delta;
+  z.restart = restart;
   z.iter = -1;
   z.skip_ru = 1;
   z.keep_p = 1;
@@ -1016,8 +1019,11 @@ ii_status multishift_reference(ii_context* ctx, double kappa, double2* x, const 
   MSScal* S = (MSScal*)wp;
   double* aux = (double*)((char*)wp + sizeof(MSScal));  // rr, bb, sigma[...]
   MSScal* Sh = nullptr;
-  II_HIP(ctx, hipHostMalloc(&Sh, sizeof(MSScal)));
-  struct Free { MSScal* p; ~Free() { if (p) (void)hipHostFree(p); } } fr_{Sh};
+  {
+    void* hp = nullptr;
+    II_CHECK(ensure_host_pin(ctx, sizeof(MSScal), &hp));
+    Sh = (MSScal*)hp;
+  }
   SF fr, fs, fq, ft, fp;
   II_CHECK(fieldP(ctx, 10, 0, nh, &fr));
   II_CHECK(fieldP(ctx, 12, 0, nh, &fs));
@@ -1213,8 +1219,11 @@ ii_status multishift_sloppy(ii_context* ctx, double kappa, float2* x32, const do
   MSScal* S = (MSScal*)wp;
   double* aux = (double*)((char*)wp + sizeof(MSScal));
   MSScal* Sh = nullptr;
-  II_HIP(ctx, hipHostMalloc(&Sh, sizeof(MSScal)));
-  struct Free { MSScal* p; ~Free() { if (p) (void)hipHostFree(p); } } fr_{Sh};
+  {
+    void* hp = nullptr;
+    II_CHECK(ensure_host_pin(ctx, sizeof(MSScal), &hp));
+    Sh = (MSScal*)hp;
+  }
   SF fr, fs, fq, ft, fp;
   II_CHECK(fieldP(ctx, 32, II_PREC_SINGLE, 1, &fr));
   II_CHECK(fieldP(ctx, 33, II_PREC_SINGLE, 1, &fs));
@@ -1522,7 +1531,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       II_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     hipLaunchKernelGGL(k_scal_init, dim3(nr), dim3(64), 0, ctx->stream, sc, aux + 2 * batch, aux,
-                       0.0, tol, delta, 1);
+                       0.0, tol, delta, 1, prm->restart_factor > 0 ? prm->restart_factor : 16.0);
     II_HIP(ctx, hipGetLastError());
     if (lo) {
       II_CHECK(convertP(ctx, 0, P, r64, rs, nr));
@@ -1715,6 +1724,7 @@ ii_status propagator(ii_context* ctx, double kappa, const ii_prop_params* prm,
       o.iterations = H[k].count;
       o.inner_iterations = lo ? H[k].count : 0;
       o.restarts = H[k].updates;
+      o.recursion_restarts = H[k].restarts;
       o.diff = std::sqrt(H[k].rho2 / H[k].b2);
       o.timing_ms = ms;
       o.true_residual = -1.0;
@@ -1826,6 +1836,10 @@ __global__ void k_bi_beta(const double* __restrict__ partials, int nb, size_t pl
     B.rho[1] = im;
     S.rr = rr;
     S.count += 1;
+    // a restart (keep_p = 0 from k_bi_ru) applies to the one update event
+    // that set it: the masked r -> r0, p copies of later events must not
+    // fire for this rhs again
+    S.keep_p = 1;
     if (!finite(rr) || !finite(B.beta[0]) || !finite(B.beta[1])) {
       S.nan = 1; S.done = 1; S.skip_ru = 1;
       return;
@@ -1845,7 +1859,7 @@ __global__ void k_bi_ru(const double* __restrict__ partials, int nb, size_t pl,
   const double rr = plane_sum(partials, pl, 2, nb);
   if (threadIdx.x == 0) {
     BiCoef& B = bc[blockIdx.x];
-    if (rr > 16.0 * S.rr) {
+    if (rr > S.restart * S.rr) {
       S.keep_p = 0;
       S.restarts += 1;
       B.rho[0] = rr;
@@ -2189,7 +2203,7 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
       II_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     hipLaunchKernelGGL(k_scal_init, dim3(nr), dim3(64), 0, ctx->stream, sc, aux + 2 * batch, aux,
-                       0.0, tol, delta, 1);
+                       0.0, tol, delta, 1, prm->restart_factor > 0 ? prm->restart_factor : 16.0);
     hipLaunchKernelGGL(k_bi_init, dim3(nr), dim3(64), 0, ctx->stream, sc, bc);
     II_HIP(ctx, hipGetLastError());
     // r = r0 = p = bhat (x0 = 0), ys = 0
@@ -2323,6 +2337,7 @@ ii_status propagator_bicgstab_t(ii_context* ctx, double kappa, const ii_prop_par
       o.iterations = H[k].count;
       o.inner_iterations = H[k].count;
       o.restarts = H[k].updates;
+      o.recursion_restarts = H[k].restarts;
       o.diff = std::sqrt(H[k].rho2 / H[k].b2);
       o.timing_ms = ms;
       o.true_residual = -1.0;
@@ -2630,7 +2645,11 @@ ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_param
   II_CHECK(require_gauge(ctx));
   II_CHECK(enter(ctx));
   ii_status st;
-  if (p->use_learned) {
+  if (p->method == II_METHOD_FGCR) {
+    if (!p->batched)
+      return set_error(ctx, II_ERR_BAD_ARG, "the flexible GCR method runs batched only");
+    st = fgcr_batched(ctx, &ctx->fgcr, kappa, p, src, sol, out);
+  } else if (p->use_learned) {
     st = learned_propagator(ctx, kappa, p, src, sol, out);
   } else if (p->method == II_METHOD_BICGSTAB) {
     st = propagator_bicgstab(ctx, kappa, p, src, sol, out);

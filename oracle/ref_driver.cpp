@@ -161,7 +161,8 @@ int main(int argc, char** argv) {
       "       iroiro_ref iowrite Lx Ly Lz Lt Binary|NERSC gauge file\n"
       "       iroiro_ref ioread  Lx Ly Lz Lt Binary file gauge_out\n"
       "       iroiro_ref meson   Lx Ly Lz Lt q1 q2 px py pz outdir\n"
-      "       iroiro_ref bench  Lx Ly Lz Lt mass nrep [cg_prec]\n");
+      "       iroiro_ref bench  Lx Ly Lz Lt mass nrep [cg_prec]\n"
+      "       iroiro_ref benchprop Lx Ly Lz Lt mass gauge nrep cg_prec\n");
     return 2;
   }
   std::string cmd = argv[1];
@@ -270,6 +271,39 @@ int main(int argc, char** argv) {
       write_vec(out + name, cp);
     }
     std::printf("IIREF_OK meson\n");
+    return 0;
+  }
+
+  if (cmd == "benchprop") {
+    // The reference CPU path on a given gauge field (with its boundary
+    // phase): one EO hop (Doe.mult), then the Schur solve of column 0 of
+    // Qprop_EvenOdd's point source at the origin exactly as
+    // Inverter_WilsonLike::invert calls it (eoUtils.cpp:51-60: Solver_CG on
+    // Fopr_DdagD for Dhat^dag bhat, bhat = b_e since b_o = 0 for a source on
+    // an even site), stopped at cg_prec (squared) so that a bounded sample
+    // of iterations is timed.
+    double mass = std::atof(argv[6]);
+    Field u = read_field(argv[7], gsz);
+    int nrep = std::atoi(argv[8]);
+    double prec = std::atof(argv[9]);
+    Dirac_Wilson_EvenOdd<Dirac_Wilson> D(mass, &u);
+    Field be(hsz);
+    be.set(0, 1.0);  // site 0 (even, esec index 0), spin 0, colour 0, real part
+    Field w = D.mult_oe(be);  // warm-up
+    double t0 = now_ms();
+    for (int r = 0; r < nrep; ++r) w = D.mult_oe(be);
+    double hop_ms = (now_ms() - t0) / nrep;
+    Fopr_DdagD op(&D);
+    double t2 = now_ms();
+    w = op.mult(be);
+    double ddagd_ms = now_ms() - t2;
+    Field rhs = D.mult_dag(be);
+    Solver_CG cg(prec, 100000, &op);
+    Field x(hsz);
+    SolverOutput so = cg.solve(x, rhs);
+    std::printf("IIREF_BENCH {\"hop_ms\": %.6f, \"nrep\": %d, \"Vh\": %d, "
+                "\"ddagd_ms\": %.6f, \"cg_ms\": %.6f, \"cg_iterations\": %d}\n",
+                hop_ms, nrep, Vh, ddagd_ms, (double)so.timing, so.Iterations);
     return 0;
   }
 

@@ -217,3 +217,113 @@ def test_training_reduces_loss_and_iterations(tmp_path):
         assert np.array_equal(net.apply(b), z1)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("dims,nrhs", [((4, 4, 4, 8), 5), ((6, 2, 2, 6), 3)])
+def test_batched_apply_vs_oracle(dims, nrhs):
+    """ii_net_apply_batched (all rhs per launch; layers regrouped by parity
+    with composite weights, bfloat16 activations and links) against the FP64
+    oracle network on every rhs, on a lattice whose half volume is a whole
+    number of 16-site tiles and on one that is not (6x2x2x6: 72 sites).
+    Bound 3e-2: bf16 rounding of the activations of four stages, of the
+    transport matrices and of the composite weights (8 significant bits
+    each).  A network with one transport path of one layer dropped is more
+    than three bounds away, so the bound separates the right network from a
+    wrong one."""
+    lat = orc.Lattice(dims)
+    u = lat.apbc(lat.gauge_expH(0.4, 77))
+    ctx = ii.Context(dims)
+    try:
+        ctx.upload_gauge(u)
+        net = ii.LearnedPreconditioner(ctx)
+        net.init_neumann(0.13, noise=0.3, seed=5)
+        ws = [net.get_weights(layer).astype(np.complex128) for layer in range(4)]
+        rng = np.random.default_rng(3)
+        b = rng.normal(size=(nrhs, lat.half_size))
+        got = net.apply_batched(b)
+        single = np.stack([net.apply(b[k]) for k in range(nrhs)])
+        ref = orc.Net(lat, u, ws)
+        wrong = [w.copy() for w in ws]
+        wrong[1][8] = 0.0
+        bad = orc.Net(lat, u, wrong)
+        for k in range(nrhs):
+            want = ref.apply(b[k])
+            e = rel(got[k], want)
+            assert e < 3e-2, (k, e)
+            assert rel(got[k], single[k]) < 3e-2
+            assert rel(bad.apply(b[k]), want) > 1e-1
+    finally:
+        ctx.close()
+
+
+def _fgcr_setup(dims, kappa, seed=5):
+    lat = orc.Lattice(dims)
+    u = lat.apbc(lat.gauge_expH(0.6, seed))
+    ctx = ii.Context(dims)
+    ctx.upload_gauge(u)
+    net = ii.LearnedPreconditioner(ctx)
+    net.init_neumann(kappa, noise=0.05, seed=1)
+    return lat, u, ctx, net
+
+
+@pytest.mark.parametrize("use_net,use_graph", [(True, False), (False, False), (True, True)])
+def test_fgcr_batched_point_sources(use_net, use_graph):
+    """The batched mixed-precision flexible GCR (every rhs per launch, FP32
+    Krylov vectors, the network on bfloat16 activations, FP64 restarts): the
+    12 point-source columns reach the full-operator true residual 1e-10
+    (oracle) and the oracle's FP64 solution to 1e-8, with and without the
+    network and with the restart cycle captured as a HIP graph."""
+    dims, kappa = (8, 8, 8, 8), 0.1355
+    lat, u, ctx, net = _fgcr_setup(dims, kappa)
+    try:
+        src = ii.DeviceField(ctx, "full", 12)
+        sol = ii.DeviceField(ctx, "full", 12)
+        src.point_sources((0, 0, 0, 0))
+        outs = ii.PropagatorSolver(ctx, kappa, tol=1e-10, batched=True, method="fgcr",
+                                   use_learned=use_net, use_graph=use_graph,
+                                   nkrylov=8).solve(src, sol)
+        print("fgcr net=%s graph=%s: directions %s, cycles %s, fallback %s"
+              % (use_net, use_graph, [o.Iterations for o in outs], [o.restarts for o in outs],
+                 [o.inner_iterations for o in outs]))
+        assert max(o.true_residual for o in outs) <= 1e-10
+        for rhs in (0, 4, 11):
+            b = src.download(rhs)
+            x = sol.download(rhs)
+            r = lat.true_residual_full(u, kappa, x, b)
+            assert r <= 1e-10, (rhs, r)
+            assert abs(outs[rhs].true_residual - r) < 1e-12
+            xo, _ = lat.eo_invert(u, kappa, b, 1e-28, 5000)
+            assert rel(x, xo) < 1e-8
+    finally:
+        ctx.close()
+
+
+def test_fgcr_graph_equals_eager_and_network_saves_directions():
+    """Graph replay of the restart cycle gives the eager solution bit for
+    bit; the Neumann-initialised network needs fewer directions than the
+    same flexible GCR without it; a ragged batch of 5 Gaussian sources."""
+    dims, kappa = (8, 8, 8, 8), 0.1355
+    lat, u, ctx, net = _fgcr_setup(dims, kappa, seed=8)
+    try:
+        src = ii.DeviceField(ctx, "full", 5)
+        sol = ii.DeviceField(ctx, "full", 5)
+        rng = np.random.default_rng(4)
+        bs = [rng.normal(size=lat.full_size) for _ in range(5)]
+        for k in range(5):
+            src.upload(k, bs[k])
+        runs = {}
+        for name, net_on, graph in [("eager", True, False), ("graph", True, True),
+                                    ("plain", False, False)]:
+            outs = ii.PropagatorSolver(ctx, kappa, tol=1e-10, batched=True, method="fgcr",
+                                       use_learned=net_on, use_graph=graph).solve(src, sol)
+            runs[name] = (outs, [sol.download(k) for k in range(5)])
+        for k in range(5):
+            assert np.array_equal(runs["eager"][1][k], runs["graph"][1][k])
+            assert lat.true_residual_full(u, kappa, runs["eager"][1][k], bs[k]) <= 1e-10
+            assert lat.true_residual_full(u, kappa, runs["plain"][1][k], bs[k]) <= 1e-10
+        d_net = max(o.Iterations for o in runs["eager"][0])
+        d_plain = max(o.Iterations for o in runs["plain"][0])
+        print("directions: network %d, none %d" % (d_net, d_plain))
+        assert d_net < d_plain
+    finally:
+        ctx.close()

@@ -268,15 +268,34 @@ def test_propagator_ragged_batches(nrhs, method):
     ctx.close()
 
 
-def test_lowprec_operator_error():
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_forced_recursion_restarts_converge(method):
+    """restart_factor = 1e-300 makes every reliable update take the branch that
+    restarts the search direction (a recomputed residual above restart_factor x
+    the recursive one).  The solve still reaches the true residual, every
+    update but the converging one restarted, and the solutions match the
+    oracle's FP64 solution: after a restart the recursion scalars (rho = |r|^2
+    for BiCGStab, r0 = p = r) stay consistent with the copied vectors."""
     dims = (8, 8, 8, 8)
-    lat, u = _prop_setup(dims, seed=6)
+    lat, u = _prop_setup(dims, seed=9)
+    kappa = 0.1355
     ctx = ii.Context(dims)
     ctx.upload_gauge(u)
-    t64 = ctx.bench_hop("double", 1, 2)
-    t32 = ctx.bench_hop("single", 1, 2)
-    t16 = ctx.bench_hop("half", 1, 2)
-    assert t64 > 0 and t32 > 0 and t16 > 0
+    src = ii.DeviceField(ctx, "full", 12)
+    sol = ii.DeviceField(ctx, "full", 12)
+    src.point_sources((0, 0, 0, 0))
+    outs = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision="half", batched=True,
+                               method=method, restart_factor=1e-300).solve(src, sol)
+    for k, o in enumerate(outs):
+        assert o.recursion_restarts >= 1, (k, o.recursion_restarts)
+        assert o.recursion_restarts == o.restarts - 1, (k, o.recursion_restarts, o.restarts)
+        assert o.true_residual <= 1e-10
+    for rhs in (0, 5, 11):
+        b = src.download(rhs)
+        x = sol.download(rhs)
+        assert lat.true_residual_full(u, kappa, x, b) <= 1e-10
+        xo, _ = lat.eo_invert(u, kappa, b, 1e-28, 5000)
+        assert rel(x, xo) < 1e-8
     ctx.close()
 
 
