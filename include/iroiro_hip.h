@@ -153,6 +153,17 @@ ii_status ii_dirac_mult_oe_dag(ii_context* ctx, double kappa, double* out, const
  * (dirac_wilson_EvenOdd.hpp:79-92), on even half fields. */
 ii_status ii_dirac_mult(ii_context* ctx, double kappa, double* out, const double* in);
 ii_status ii_dirac_mult_dag(ii_context* ctx, double kappa, double* out, const double* in);
+/* Fopr_DdagD::mult exactly as ii_fopr_ddagd_mult, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It
+ * allocates nothing: its staging and temporaries live in one static device
+ * array of 32 MiB per GPU (its address taken once per device when the first
+ * context on it is created), which bounds the lattice to a half volume of
+ * about 34,900 sites (II_ERR_BAD_ARG above that).  It does not run the hop
+ * autotuner.  All calls on one GPU share that array: calls that overlap on
+ * one GPU (from several threads) race. */
+ii_status ii_fopr_ddagd_mult_noalloc(ii_context* ctx, double kappa, double* out,
+                                     const double* in);
 /* Dirac_Wilson::gamma5 (dirac_wilson.cpp:120-126) on nsites sites. */
 ii_status ii_dirac_gamma5(ii_context* ctx, double* out, const double* in, long nsites);
 /* Fopr_DdagD::mult (lib/Fopr/fopr.h:146-153). */

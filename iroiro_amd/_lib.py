@@ -25,6 +25,7 @@ EXPORTS = [
     "ii_gauge_unit", "ii_gauge_heatbath", "ii_gauge_apply_apbc",
     "ii_dirac_mult_eo", "ii_dirac_mult_oe", "ii_dirac_mult_eo_dag", "ii_dirac_mult_oe_dag",
     "ii_dirac_mult", "ii_dirac_mult_dag", "ii_dirac_gamma5", "ii_fopr_ddagd_mult",
+    "ii_fopr_ddagd_mult_noalloc",
     "ii_dirac_full_mult", "ii_dirac_mult_eo_prec", "ii_solver_cg", "ii_multishift_cg", "ii_eo_invert",
     "ii_field_create", "ii_field_destroy", "ii_field_upload", "ii_field_download",
     "ii_field_point_sources", "ii_field_gaussian", "ii_propagator_solve",
@@ -139,6 +140,7 @@ def load():
         "ii_dirac_mult_dag": [vp, cd, dp, dp],
         "ii_dirac_gamma5": [vp, dp, dp, cl],
         "ii_fopr_ddagd_mult": [vp, cd, dp, dp],
+        "ii_fopr_ddagd_mult_noalloc": [vp, cd, dp, dp],
         "ii_dirac_full_mult": [vp, cd, dp, dp, ci],
         "ii_dirac_mult_eo_prec": [vp, cd, ci, ci, ci, ci, dp, dp],
         "ii_dirac_md_force": [vp, cd, dp, dp, dp],

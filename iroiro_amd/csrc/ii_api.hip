@@ -154,6 +154,7 @@ ii_status ii_context_create(ii_context** out, const int dims[4], int device) {
   size_t gbytes = (size_t)2 * 4 * 9 * g.stride * sizeof(double2);
   if (hipMalloc(&ctx->gauge64, gbytes) != hipSuccess) return II_ERR_DEVICE;
   if (hipMemset(ctx->gauge64, 0, gbytes) != hipSuccess) return II_ERR_DEVICE;
+  if (noalloc_setup(ctx.get()) != II_OK) return II_ERR_DEVICE;
   *out = ctx.release();
   return II_OK;
 }

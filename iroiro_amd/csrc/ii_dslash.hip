@@ -496,7 +496,7 @@ ii_status tune_hop(ii_context* ctx) {
 ii_status hop64(ii_context* ctx, const HopSpec& h, const double2* in, double2* out,
                 const double2* x, const double2* dv, int rs) {
   const Geom& g = ctx->g;
-  if (h.nrhs == 1 && !ctx->hop_tuned) II_CHECK(tune_hop(ctx));
+  if (h.nrhs == 1 && !ctx->hop_tuned && !h.no_tune) II_CHECK(tune_hop(ctx));
   SpF64 sin{in, g.stride, rs};
   GaugeF64 gf{ctx->gauge64, g.stride};
   StF64 so{out, g.stride};

@@ -42,6 +42,9 @@ struct HopSpec {
   // optional per-rhs skip flags: rhs r is skipped when skip[r * skip_stride] != 0
   const int* skip = nullptr;
   int skip_stride = 0;
+  // FP64 single-rhs hops: do not run the traversal autotuner (which
+  // allocates) before the first launch; the current traversal is used
+  bool no_tune = false;
 };
 
 // A 16-bit half field: components [rhs][12][stride] (half2 or bf16x2 as
@@ -138,6 +141,10 @@ ii_status multishift_reference(ii_context* ctx, double kappa, double2* x, const 
 ii_status multishift_mixed_dev(ii_context* ctx, double kappa, double2* x, const double2* b,
                                const double* shifts, int nshift, double tol, int precision,
                                int maxiter, ii_solver_output* out);
+
+// address of the no-allocation scratch array on the context's device, taken
+// once per device at context creation (ii_noalloc.hip)
+ii_status noalloc_setup(ii_context* ctx);
 
 // learned preconditioner (ii_learned.hip)
 void learned_release(ii_context* ctx);
