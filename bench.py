@@ -46,17 +46,19 @@ MESON_BYTES_PER_SITE = 12 * 24 * 8 + 2 * 18 * 8
 # algorithmic bytes of the network's four stage kernels per output site:
 # (per rhs, shared by all rhs).  Activations are bf16 complex (4 B), 48 B per
 # channel; r and z are FP32 half fields (96 B per site); each launch reads
-# the 8 transport matrices of its sites once (8 x 80 B) for every rhs.
+# the 8 link records of its sites once (8 x 36 B) for every rhs.
 #   first: r at the even neighbours 96 + [L1|L2](odd) written 768
-#   mid<0>: L1(odd) 384 + r 96 + L2(even) written 384
-#   mid<1>: L2(even) 384 + L2(odd) 384 + L3(odd) written 384
-#   last: (L3|L2)(odd) 768 + L2(even) 384 + z written 96
-NET_STAGE_BYTES = [(96 + 768, 640), (384 + 96 + 384, 640), (3 * 384, 640),
-                   (768 + 384 + 96, 640)]
+#   mid: L1(odd) 384 + r 96 + L2(even) written 384
+#   third: L2(even) 384 + L2(odd) 384 + Y(odd) written 384
+#   last: Y(odd) (one channel of each of 8 neighbours) 384 + L2(even) 384 +
+#         z written 96
+NET_STAGE_BYTES = [(96 + 768, 288), (384 + 96 + 384, 288), (3 * 384, 288),
+                   (384 + 384 + 96, 288)]
 # MFMA flops per output site and rhs (v_mfma_f32_16x16x32_bf16, 16384 flops,
-# 16 sites per instruction): first 3 x 8 x 2, mid 3 x 4 x 2 x 8 + identity
-# (3 x 4 x 1 or 3 x 4 x 2), last 3 x 4 x 8 + 3 x 2
-NET_STAGE_MFMA_FLOPS = [48 * 1024, (192 + 12) * 1024, (192 + 24) * 1024, (96 + 6) * 1024]
+# 16 sites per instruction, x 3 colours): first 8 x 2, mid 8 x 4 x 2 + 4,
+# third 8 x 4 x 2 + 4 x 2 + 4 x 4 (the Y mixes), last 2 (identity; the
+# transports run on v_dot2_f32_bf16)
+NET_STAGE_MFMA_FLOPS = [48 * 1024, 204 * 1024, 264 * 1024, 6 * 1024]
 
 CONFIGS = {
     "c5": dict(dims=(32, 32, 32, 64), beta=6.0, kappa=0.1355, sweeps=200,
@@ -372,6 +374,11 @@ def main():
         log("learned: trained on %d pairs, %d epochs in %.1f s (pairs %.1f s): loss %.4f -> %.4f"
             % (args.train_pairs, args.train_epochs, train["s"], rep["pairs_ms"] / 1e3,
                rep["loss_first"], rep["loss_last"]))
+        # the network's launch shapes for 12 rhs (otherwise chosen at the
+        # first application, inside the warm-up solve)
+        rg, rep_rg = net.tune(12)
+        tune = {"hop": tune, "network": rep_rg}
+        log("autotune:", rep_rg)
 
     def make_solver(method, use_graph=True):
         if method in ("fgcr-learned", "fgcr"):
@@ -430,9 +437,9 @@ def main():
     if net is not None:
         st = net.bench_batched(12, 5)
         names = ["k_bnet_first (odd, [L1|L2] from 8 transports of r)",
-                 "k_bnet_mid<0> (even, L2: 8 transported channels)",
-                 "k_bnet_mid<1> (odd, L3: 8 transported channels)",
-                 "k_bnet_last (even, 16->1 mixing + transport)"]
+                 "k_bnet_mid (even, L2: 8 transported channels)",
+                 "k_bnet_third (odd, L3: 8 transported channels, then the 8 Y mixes)",
+                 "k_bnet_last (even, z: one transport of Y per path)"]
         net_stages = []
         for i, t in enumerate(st):
             nb = NET_STAGE_BYTES[i]

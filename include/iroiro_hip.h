@@ -444,6 +444,15 @@ ii_status ii_net_bench(ii_context* ctx, int nrep, double* layer_ms);
  * (stage_ms[4], average over nrep launches, HIP events on the context
  * stream): odd [L1 | L2], even L2, odd L3, even output. */
 ii_status ii_bench_net_batched(ii_context* ctx, int nrhs, int nrep, double* stage_ms);
+/* Launch-shape autotuner of ii_net_apply_batched (no counterpart in the
+ * reference): times every admissible number of rhs processed side by side
+ * per tile (1, 2, 3, 4, 6, 12) for each of the four stage kernels on nrhs
+ * rhs and keeps the fastest (it also runs by itself before the first
+ * application to a new rhs count).  force_rg (4 values >= 1, or NULL)
+ * sets the choice instead of timing.  rg (4 values, or NULL) receives the
+ * choice; report (len bytes, or NULL) the timings as text. */
+ii_status ii_net_tune(ii_context* ctx, int nrhs, const int* force_rg, int* rg, char* report,
+                      size_t len);
 /* Flexible GCR on Dhat x = b (even half fields), FP64 vectors and Dhat,
  * right preconditioner M (use_net != 0) or none, restarted every `nkrylov`
  * directions with an FP64 true residual; stops when |b - Dhat x| <= tol |b|.

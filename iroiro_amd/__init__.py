@@ -1027,6 +1027,19 @@ class LearnedPreconditioner:
                                                       _lib.dptr(ms)), "ii_bench_net_batched")
         return ms
 
+    def tune(self, nrhs, force=None):
+        """Chooses (or, with force = 4 group sizes, sets) how many rhs each
+        stage kernel of apply_batched processes side by side per tile;
+        returns (group sizes, report)."""
+        f = None
+        if force is not None:
+            f = (ctypes.c_int * 4)(*[int(v) for v in force])
+        rg = (ctypes.c_int * 4)()
+        buf = ctypes.create_string_buffer(1024)
+        self.ctx.check(self._L().ii_net_tune(self.ctx.handle, int(nrhs), f, rg, buf, 1024),
+                       "ii_net_tune")
+        return list(rg), buf.value.decode()
+
     def apply_batched(self, r):
         """z_k = M r_k for a stack of even half fields, all rhs per launch
         (the evaluation of the batched flexible solver)."""

@@ -13,37 +13,49 @@
 //   L2(odd)  = sum_{d>0} (W2_0 W1_d + W2_d W1_0) T_d r,
 //   L2(even) = (W2_0 W1_0) r + sum_{d>0} W2_d T_d L1(odd),
 //   L3(odd)  = W3_0 L2(odd) + sum_{d>0} W3_d T_d L2(even),
-//   z = L4(even) = (W4_0 W3_0) L2(even)
-//                  + sum_{d>0} T_d [(W4_0 W3_d) L2(odd) + W4_d L3(odd)].
+//   z = L4(even) = (W4_0 W3_0) L2(even) + sum_{d>0} T_d Y_d(odd),
+//   Y_d(odd) = W4_d L3(odd) + (W4_0 W3_d) L2(odd)   (one channel per path).
 // L1(even) and L3(even) are never formed, and only two of the four stages
 // transport 8 channels, each on one parity: half the transports of the
 // layer-by-layer evaluation.  Four kernels, each over 16-site tiles of one
 // parity and every rhs:
 //   k_bnet_first  odd:  16 channels [L1(odd) | L2(odd)] from 8 transports of r
-//   k_bnet_mid<0> even: L2(even), 8 transported channels of L1(odd) + r
-//   k_bnet_mid<1> odd:  L3(odd) over L1(odd)'s slot, 8 transported channels
-//                       of L2(even) + L2(odd)
-//   k_bnet_last   even: z, mixing [L3(odd) | L2(odd)] at the neighbour first
-//                       (16 -> 1 channel), then one transport per path
+//   k_bnet_mid    even: L2(even), 8 transported channels of L1(odd) + r
+//   k_bnet_third  odd:  L3(odd) (8 transported channels of L2(even) plus
+//                       L2(odd)), then the 8 one-channel mixes Y_d of
+//                       [L3 | L2](odd), written over L1(odd)'s channels
+//   k_bnet_last   even: z, one transport of the one-channel Y_d per path
 // Channel/spin mixing runs on v_mfma_f32_16x16x32_bf16 (bf16 weights and
 // activations, FP32 accumulation) as in ii_learned.hip: B operand k = 8 i +
 // 2 t + re/im (input channel i, spin t), n = site; A the real embedding
-// [[Re X, -Im X], [Im X, Re X]] of the (composite) channel-spin matrix.
+// [[Re X, -Im X], [Im X, Re X]] of the (composite) channel-spin matrix.  The
+// Y mixes of k_bnet_third take L3 straight from the accumulators of the L3
+// products: the C rows a lane holds become its B operand, and the A columns
+// of the L3 part are ordered to match.
 // Transports run on v_dot2_f32_bf16: the link matrices are stored per site
-// and path as pairs P = (Re M, -Im M), Q = (Im M, Re M) of bf16, so
-// Re(M v) = sum_b dot2(P_ab, v_b) and Im(M v) = sum_b dot2(Q_ab, v_b) for
+// and path as 9 words P_ab = (Re M_ab, -Im M_ab) of bf16, so
+// Re(M v) = sum_b dot2(P_ab, v_b) and, with S_ab = (Im M_ab, Re M_ab) formed
+// from P_ab in two instructions, Im(M v) = sum_b dot2(S_ab, v_b), for
 // v_b = (Re, Im) bf16.
 //
 // Activations are bfloat16, one dword (re, im) per complex number, in
 // records [rhs][site][channel][colour][spin]: the 16 bytes of one channel
 // and colour are exactly one B operand (spins 0..3, re/im).  The input r and
 // the output z are FP32 half fields [rhs][12 = 3 spin + colour][stride].
+//
+// Order of work (ii_sweep.h): the waves of the blocks on one XCD share
+// that XCD's slab of tiles, taken as one front of consecutive (tile, rhs)
+// items; the rhs come in groups of rg (the autotuner's choice per stage)
+// whose waves share a tile's link records through L1.
 #include "ii_ops.h"
 #include "ii_stencil.h"
+#include "ii_sweep.h"
 
 #include <algorithm>
 #include <complex>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace ii {
@@ -55,10 +67,13 @@ typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kHop = 8;          // transported paths: 2 mu (forward), 2 mu + 1 (backward)
-constexpr int kLinkWords = 20;   // P_ab (9), Q_ab (9), 2 words of padding: 80 bytes
-constexpr int kBThreads = 256;   // 4 waves; a block runs its tiles for every rhs
-constexpr int kOddCh = 16;       // odd-site record: [L1 or L3 (0..7) | L2 (8..15)]
+constexpr int kLinkWords = 9;    // P_ab: 36 bytes per site and path
+constexpr int kOddCh = 16;       // odd-site record: [L1 or Y (0..7) | L2 (8..15)]
 constexpr int kEvenCh = 8;       // even-site record: L2(even)
+// threads per block: the third stage's A operands take 88 KB of LDS (one
+// block of 16 waves per CU), the others' at most 68 KB (two blocks of 8)
+constexpr int kThreadsFirst = 512, kThreadsMid = 512, kThreadsThird = 1024, kThreadsLast = 512;
+constexpr int kStageThreads[4] = {kThreadsFirst, kThreadsMid, kThreadsThird, kThreadsLast};
 
 constexpr int fr_mb(int cout) { return (8 * cout + 15) / 16; }
 constexpr int fr_kb(int cin) { return (8 * cin + 31) / 32; }
@@ -67,6 +82,12 @@ constexpr int fr_kb(int cin) { return (8 * cin + 31) / 32; }
 __device__ __forceinline__ U32x4 ld128(const void* base, uint32_t soff, uint32_t voff) {
   return __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), voff, soff, 0);
 }
+__device__ __forceinline__ uint32_t ld32(const void* base, uint32_t soff, uint32_t voff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(rsrc(base), voff, soff, 0);
+}
+// Activation records are written in 8-byte pieces spread over 48-byte
+// channel slots; they go through L2 (normal policy) so that the pieces of a
+// line merge there.  Streaming them (kNT) made the stages 1.2-3x slower.
 __device__ __forceinline__ void st64(void* base, uint32_t soff, uint32_t voff, U32x2 v) {
   __builtin_amdgcn_raw_buffer_store_b64(v, rsrc(base), voff, soff, 0);
 }
@@ -89,27 +110,46 @@ __device__ __forceinline__ uint32_t pack_b2(float re, float im) {
   __builtin_memcpy(&w, &r, 4);
   return w;
 }
-__device__ __forceinline__ float lo_f(uint32_t w) { return __uint_as_float(w << 16); }
-__device__ __forceinline__ float hi_f(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+// the 8 C values of two accumulator blocks as one B operand
+__device__ __forceinline__ bf16x8 pack8(f32x4 a, f32x4 b) {
+  const U32x4 w{pack_b2(a[0], a[1]), pack_b2(a[2], a[3]), pack_b2(b[0], b[1]),
+                pack_b2(b[2], b[3])};
+  return as_frag(w);
+}
+
+// dot2(a, b) + 0 as the three-operand v_dot2_f32_bf16 with an inline zero:
+// left to itself the compiler starts every chain with a v_mov of 0 and the
+// accumulating two-operand form (48 extra moves per path and lane)
+__device__ __forceinline__ float dot2_first(uint32_t a, uint32_t b) {
+  float r;
+  asm("v_dot2_f32_bf16 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float dot2(uint32_t a, uint32_t b, float c) {
+  return __builtin_amdgcn_fdot2_f32_bf16(as_b2(a), as_b2(b), c, false);
+}
+// S_ab = (Im M_ab, Re M_ab) from P_ab = (Re M_ab, -Im M_ab): the halves
+// swapped, with the sign of the high half (-Im) flipped on the way down
+__device__ __forceinline__ uint32_t s_of(uint32_t p) {
+  return __builtin_amdgcn_alignbit(p, p ^ 0x80000000u, 16);
+}
 
 // B operands of M v: v[b] holds spins 0..3 of colour b as (re, im) bf16
-// pairs; L the 5 x 16 bytes of the link record (P then Q).
-__device__ __forceinline__ void transport(const U32x4 L[5], const U32x4 v[3], bf16x8 w[3]) {
-  uint32_t W[20];
+// pairs; P the 9 words of the link record.
+__device__ __forceinline__ void transport(const uint32_t P[9], const U32x4 v[3], bf16x8 w[3]) {
+  uint32_t S[9];
 #pragma unroll
-  for (int k = 0; k < 5; ++k)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) W[4 * k + e] = L[k][e];
+  for (int k = 0; k < 9; ++k) S[k] = s_of(P[k]);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     float re[4], im[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      float r = 0.0f, i = 0.0f;
+      float r = dot2_first(P[3 * a], v[0][s]), i = dot2_first(S[3 * a], v[0][s]);
 #pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        r = __builtin_amdgcn_fdot2_f32_bf16(as_b2(W[3 * a + b]), as_b2(v[b][s]), r, false);
-        i = __builtin_amdgcn_fdot2_f32_bf16(as_b2(W[9 + 3 * a + b]), as_b2(v[b][s]), i, false);
+      for (int b = 1; b < 3; ++b) {
+        r = dot2(P[3 * a + b], v[b][s], r);
+        i = dot2(S[3 * a + b], v[b][s], i);
       }
       re[s] = r;
       im[s] = i;
@@ -120,6 +160,18 @@ __device__ __forceinline__ void transport(const U32x4 L[5], const U32x4 v[3], bf
       w[a][2 * s + 1] = (__bf16)im[s];
     }
   }
+}
+
+// the link record at byte offset soff + voff (36 bytes)
+__device__ __forceinline__ void load_link(const void* links, uint32_t soff, uint32_t voff,
+                                          uint32_t P[9]) {
+  const U32x4 x = ld128(links, soff, voff), y = ld128(links, soff, voff + 16u);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    P[e] = x[e];
+    P[4 + e] = y[e];
+  }
+  P[8] = ld32(links, soff, voff + 32u);
 }
 
 // r (FP32 half field of one rhs, [12][stride] float2) at site s as bf16
@@ -137,12 +189,13 @@ __device__ __forceinline__ void load_r_b2(const float2* __restrict__ r, int stri
 
 struct BArgs {
   Geom g;
-  int nrhs;
-  int tiles;            // 16-site tiles of the output parity
-  int tiles_per_block;
+  NetSweep sw;          // work order (ii_sweep.h)
+  const int* skip;      // rhs k is skipped if skip[k * skip_stride] != 0 (nullptr: none)
+  int skip_stride;
   const uint4* frag;    // hop-path A operands
   const uint4* frag_id; // identity-path A operands
-  const uint32_t* links;  // link records of the output parity: [8][stride][20]
+  const uint4* frag_y;  // the Y mixes (third stage)
+  const uint32_t* links;  // link records of the output parity: [8][stride][9]
   const void* in;       // hop input (the other parity)
   const void* in_id;    // identity input (the output parity)
   void* out;
@@ -183,75 +236,76 @@ k_bnet_links(Geom g, const double2* __restrict__ u, uint32_t* __restrict__ out) 
   for (int k = 0; k < 9; ++k) {
     const double2 m = M[k / 3][k % 3];
     o[k] = pack_b2((float)m.x, (float)-m.y);
-    o[9 + k] = pack_b2((float)m.y, (float)m.x);
   }
-  o[18] = 0u;
-  o[19] = 0u;
 }
 
 // --------------------------------------------------------------- stage 1
 // odd sites: [L1(odd) | L2(odd)] (16 channels) from the 8 transports of r.
 // The B operand's channel index is the path: lane group q of k-block kb
-// transports r along path 4 kb + q.
-__global__ void __launch_bounds__(kBThreads) k_bnet_first(BArgs a) {
-  constexpr int MB = fr_mb(kOddCh), KB = 2, NF = MB * KB;
+// transports r along path 4 kb + q.  The 8 output k-blocks are formed in
+// two halves (L1, then L2) from the same B operands.
+__global__ void __launch_bounds__(kThreadsFirst, 4) k_bnet_first(BArgs a) {
+  constexpr int MB = fr_mb(kOddCh), KB = 2, NF = MB * KB, MH = MB / 2;
+  constexpr int NW = kThreadsFirst / 64;
   __shared__ uint4 sA[NF * 64];
-  for (int i = threadIdx.x; i < NF * 64; i += kBThreads) sA[i] = a.frag[i];
+  for (int i = threadIdx.x; i < NF * 64; i += kThreadsFirst) sA[i] = a.frag[i];
   __syncthreads();
   const Geom& g = a.g;
-  const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
-  const int t0 = blockIdx.x * a.tiles_per_block, t1 = min(t0 + a.tiles_per_block, a.tiles);
-  for (int tile = t0; tile < t1; ++tile) {
-    const int site = tile * 16 + n;
+  const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
+  for (int it = S.first; it < S.nit; it += S.step) {
+    const NetItem w = net_item(a.sw, S, it);
+    if (w.rhs >= a.sw.nrhs || (a.skip && a.skip[w.rhs * a.skip_stride])) continue;
+    const int site = w.tile * 16 + n;
     const bool valid = site < g.Vh;
     const int s = valid ? site : g.Vh - 1;
     const SiteCoord sc = decode_site(g, s, 1);
-    int nbk[KB];
-    U32x4 L[KB][5];
+    const float2* r = (const float2*)a.in + (size_t)w.rhs * 12 * g.stride;
+    uint32_t L[KB][9];
+    U32x4 v[KB][3];
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) {
       const int dh = 4 * kb + q;
-      nbk[kb] = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
-#pragma unroll
-      for (int k = 0; k < 5; ++k)
-        L[kb][k] = ld128(a.links, 0u, link_off(g, dh) + (uint32_t)s * 80u + 16u * k);
+      const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
+      load_link(a.links, 0u, link_off(g, dh) + (uint32_t)s * (kLinkWords * 4), L[kb]);
+      load_r_b2(r, g.stride, nb, v[kb]);
     }
-    for (int rr = wave; rr < a.nrhs; rr += kBThreads / 64) {
-      const float2* r = (const float2*)a.in + (size_t)rr * 12 * g.stride;
-      bf16x8 B[KB][3];
+    bf16x8 B[KB][3];
 #pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        U32x4 v[3];
-        load_r_b2(r, g.stride, nbk[kb], v);
-        transport(L[kb], v, B[kb]);
-      }
-      f32x4 acc[3][MB];
+    for (int kb = 0; kb < KB; ++kb) transport(L[kb], v[kb], B[kb]);
+    // the matrix products start after both transports (their registers are free)
+    __builtin_amdgcn_sched_barrier(0);
+    char* o = (char*)a.out + (size_t)w.rhs * g.stride * (kOddCh * 48);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      f32x4 acc[3][MH];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int mb = 0; mb < MH; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb)
+      for (int mb = 0; mb < MH; ++mb)
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
-          const bf16x8 A = as_frag(*reinterpret_cast<const U32x4*>(&sA[(mb * KB + kb) * 64 + lane]));
+          const bf16x8 A =
+              as_frag(*reinterpret_cast<const U32x4*>(&sA[((h * MH + mb) * KB + kb) * 64 + lane]));
 #pragma unroll
           for (int c = 0; c < 3; ++c)
             acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[kb][c], acc[c][mb], 0, 0, 0);
         }
       if (valid) {
         // C row 4q + j of block mb: channel 2 mb + q/2, spin 2 (q & 1) + j/2, re/im j & 1
-        char* o = (char*)a.out + (size_t)rr * g.stride * (kOddCh * 48);
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) {
-          const int ch = 2 * mb + (q >> 1);
+        for (int mb = 0; mb < MH; ++mb) {
+          const int ch = 2 * (h * MH + mb) + (q >> 1);
           const uint32_t vo = ((uint32_t)s * kOddCh + ch) * 48u + 8u * (q & 1);
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            U32x2 w;
-            w[0] = pack_b2(acc[c][mb][0], acc[c][mb][1]);
-            w[1] = pack_b2(acc[c][mb][2], acc[c][mb][3]);
-            st64(o, 16u * c, vo, w);
+            U32x2 wv;
+            wv[0] = pack_b2(acc[c][mb][0], acc[c][mb][1]);
+            wv[1] = pack_b2(acc[c][mb][2], acc[c][mb][3]);
+            st64(o, 16u * c, vo, wv);
           }
         }
       }
@@ -259,212 +313,267 @@ __global__ void __launch_bounds__(kBThreads) k_bnet_first(BArgs a) {
   }
 }
 
-// --------------------------------------------------------------- stages 2, 3
-// ID = 0: even sites, L2(even) = (W2_0 W1_0) r + sum_{d>0} W2_d T_d L1(odd)
-//         (hop input: odd records, channels 0..7; identity input: r, FP32)
-// ID = 1: odd sites, L3(odd) = W3_0 L2(odd) + sum_{d>0} W3_d T_d L2(even)
-//         (hop input: even records; identity input and output: odd records,
-//         reading channels 8..15 and writing channels 0..7 of the same site)
-template <int ID>
-__global__ void __launch_bounds__(kBThreads) k_bnet_mid(BArgs a) {
+// ------------------------------------------------------- stages 2 and 3
+// The 8 transported paths of an 8-channel input record (INREC channels per
+// site, channels 0..7 read), added to the accumulators acc[colour][mb] of
+// the 8 output channels.  Per path, the link record and the first k-block
+// of the neighbour record are requested before the first transport, the
+// second k-block after it.
+template <int INREC>
+__device__ __forceinline__ void hop8(const BArgs& a, const SiteCoord& sc, int s, int lane, int q,
+                                     const char* in, const uint4* sA, f32x4 acc[3][4]) {
   constexpr int MB = 4, KB = 2;
-  constexpr int KBI = ID == 0 ? 1 : 2;
-  constexpr int NH = kHop * MB * KB, NI = MB * KBI;
-  constexpr int INREC = ID == 0 ? kOddCh : kEvenCh;   // hop input record
-  constexpr int OUTREC = ID == 0 ? kEvenCh : kOddCh;  // output record
-  constexpr int PAR = ID == 0 ? 0 : 1;                // output parity
-  __shared__ uint4 sA[(NH + NI) * 64];
-  for (int i = threadIdx.x; i < NH * 64; i += kBThreads) sA[i] = a.frag[i];
-  for (int i = threadIdx.x; i < NI * 64; i += kBThreads) sA[NH * 64 + i] = a.frag_id[i];
-  __syncthreads();
   const Geom& g = a.g;
-  const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
-  const int t0 = blockIdx.x * a.tiles_per_block, t1 = min(t0 + a.tiles_per_block, a.tiles);
-  for (int tile = t0; tile < t1; ++tile) {
-    const int site = tile * 16 + n;
-    const bool valid = site < g.Vh;
-    const int s = valid ? site : g.Vh - 1;
-    const SiteCoord sc = decode_site(g, s, PAR);
-    for (int rr = wave; rr < a.nrhs; rr += kBThreads / 64) {
-      f32x4 acc[3][MB];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
-      // identity path
-      {
-        bf16x8 B[KBI][3];
-        if constexpr (ID == 0) {
-          const float2* r = (const float2*)a.in_id + (size_t)rr * 12 * g.stride;
-          U32x4 v[3];
-          load_r_b2(r, g.stride, s, v);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            if (q != 0) v[c] = U32x4{0u, 0u, 0u, 0u};
-            B[0][c] = as_frag(v[c]);
-          }
-        } else {
-          const char* in = (const char*)a.in_id + (size_t)rr * g.stride * (kOddCh * 48);
-#pragma unroll
-          for (int kb = 0; kb < KBI; ++kb) {
-            const uint32_t vo = ((uint32_t)s * kOddCh + 8 + 4 * kb + q) * 48u;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) B[kb][c] = as_frag(ld128(in, 16u * c, vo));
-          }
-        }
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-          for (int kb = 0; kb < KBI; ++kb) {
-            const bf16x8 A =
-                as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + mb * KBI + kb) * 64 + lane]));
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[kb][c], acc[c][mb], 0, 0, 0);
-          }
-      }
-      // the 8 transported paths
-      const char* in = (const char*)a.in + (size_t)rr * g.stride * (INREC * 48);
 #pragma unroll 1
-      for (int dh = 0; dh < kHop; ++dh) {
-        const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
-        U32x4 L[5];
+  for (int dh = 0; dh < kHop; ++dh) {
+    const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
+    uint32_t L[9];
+    U32x4 v[3];
+    load_link(a.links, link_off(g, dh), (uint32_t)s * (kLinkWords * 4), L);
+    const uint32_t vo = ((uint32_t)nb * INREC + q) * 48u;
 #pragma unroll
-        for (int k = 0; k < 5; ++k)
-          L[k] = ld128(a.links, 0u, link_off(g, dh) + (uint32_t)s * 80u + 16u * k);
-        bf16x8 B[KB][3];
+    for (int c = 0; c < 3; ++c) v[c] = ld128(in, 16u * c, vo);
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-          const uint32_t vo = ((uint32_t)nb * INREC + 4 * kb + q) * 48u;
-          U32x4 v[3];
+    for (int kb = 0; kb < KB; ++kb) {
+      // one k-block at a time: its transport, the next k-block's loads,
+      // then its products
+      __builtin_amdgcn_sched_barrier(0);
+      bf16x8 B[3];
+      transport(L, v, B);
+      if (kb + 1 < KB) {
 #pragma unroll
-          for (int c = 0; c < 3; ++c) v[c] = ld128(in, 16u * c, vo);
-          transport(L, v, B[kb]);
-        }
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-          for (int kb = 0; kb < KB; ++kb) {
-            const bf16x8 A =
-                as_frag(*reinterpret_cast<const U32x4*>(&sA[((dh * MB + mb) * KB + kb) * 64 + lane]));
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[kb][c], acc[c][mb], 0, 0, 0);
-          }
+        for (int c = 0; c < 3; ++c) v[c] = ld128(in, 16u * c, vo + 4u * 48u * (kb + 1));
       }
-      if (valid) {
-        char* o = (char*)a.out + (size_t)rr * g.stride * (OUTREC * 48);
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) {
-          const int ch = 2 * mb + (q >> 1);
-          const uint32_t vo = ((uint32_t)s * OUTREC + ch) * 48u + 8u * (q & 1);
+      for (int mb = 0; mb < MB; ++mb) {
+        const bf16x8 A =
+            as_frag(*reinterpret_cast<const U32x4*>(&sA[((dh * MB + mb) * KB + kb) * 64 + lane]));
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            U32x2 w;
-            w[0] = pack_b2(acc[c][mb][0], acc[c][mb][1]);
-            w[1] = pack_b2(acc[c][mb][2], acc[c][mb][3]);
-            st64(o, 16u * c, vo, w);
-          }
-        }
+        for (int c = 0; c < 3; ++c)
+          acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], acc[c][mb], 0, 0, 0);
       }
     }
+  }
+}
+
+// stores the 8 channels of acc[colour][mb] (C layout) into channels 0..7
+// of the record of site s: C row 4q + j of block mb is channel 2 mb + q/2,
+// spin 2 (q & 1) + j/2, re/im j & 1
+template <int OUTREC>
+__device__ __forceinline__ void store8(char* o, int s, int q, const f32x4 acc[3][4]) {
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    const int ch = 2 * mb + (q >> 1);
+    const uint32_t vo = ((uint32_t)s * OUTREC + ch) * 48u + 8u * (q & 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      U32x2 wv;
+      wv[0] = pack_b2(acc[c][mb][0], acc[c][mb][1]);
+      wv[1] = pack_b2(acc[c][mb][2], acc[c][mb][3]);
+      st64(o, 16u * c, vo, wv);
+    }
+  }
+}
+
+// stage 2, even sites: L2(even) = (W2_0 W1_0) r + sum_{d>0} W2_d T_d L1(odd)
+// (hop input: odd records, channels 0..7; identity input: r, FP32, channel
+// 0 of the B operand)
+__global__ void __launch_bounds__(kThreadsMid, 4) k_bnet_mid(BArgs a) {
+  constexpr int MB = 4, NH = kHop * MB * 2, NI = MB;
+  constexpr int NW = kThreadsMid / 64;
+  __shared__ uint4 sA[(NH + NI) * 64];
+  for (int i = threadIdx.x; i < NH * 64; i += kThreadsMid) sA[i] = a.frag[i];
+  for (int i = threadIdx.x; i < NI * 64; i += kThreadsMid) sA[NH * 64 + i] = a.frag_id[i];
+  __syncthreads();
+  const Geom& g = a.g;
+  const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
+  for (int it = S.first; it < S.nit; it += S.step) {
+    const NetItem w = net_item(a.sw, S, it);
+    if (w.rhs >= a.sw.nrhs || (a.skip && a.skip[w.rhs * a.skip_stride])) continue;
+    const int site = w.tile * 16 + n;
+    const bool valid = site < g.Vh;
+    const int s = valid ? site : g.Vh - 1;
+    const SiteCoord sc = decode_site(g, s, 0);
+    f32x4 acc[3][MB];
+    {
+      const float2* r = (const float2*)a.in_id + (size_t)w.rhs * 12 * g.stride;
+      U32x4 v[3];
+      load_r_b2(r, g.stride, s, v);
+      bf16x8 B[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (q != 0) v[c] = U32x4{0u, 0u, 0u, 0u};
+        B[c] = as_frag(v[c]);
+      }
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const bf16x8 A = as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + mb) * 64 + lane]));
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], f32x4{0.f, 0.f, 0.f, 0.f},
+                                                               0, 0, 0);
+      }
+    }
+    const char* in = (const char*)a.in + (size_t)w.rhs * g.stride * (kOddCh * 48);
+    hop8<kOddCh>(a, sc, s, lane, q, in, sA, acc);
+    if (valid) store8<kEvenCh>((char*)a.out + (size_t)w.rhs * g.stride * (kEvenCh * 48), s, q, acc);
+  }
+}
+
+// stage 3, odd sites: L3(odd) = W3_0 L2(odd) + sum_{d>0} W3_d T_d L2(even)
+// (hop input: even records; identity input: channels 8..15 of the odd
+// record), then Y_d = [W4_d | W4_0 W3_d] [L3 | L2](odd) for the 8 paths d
+// (16 -> 1 channel each, two paths per accumulator block), stored as
+// channels 0..7 of the odd record.
+__global__ void __launch_bounds__(kThreadsThird, 4) k_bnet_third(BArgs a) {
+  constexpr int MB = 4, KBI = 2, KBY = 4;
+  constexpr int NH = kHop * MB * 2, NI = MB * KBI, NY = MB * KBY;
+  constexpr int NW = kThreadsThird / 64;
+  __shared__ uint4 sA[(NH + NI + NY) * 64];
+  for (int i = threadIdx.x; i < NH * 64; i += kThreadsThird) sA[i] = a.frag[i];
+  for (int i = threadIdx.x; i < NI * 64; i += kThreadsThird) sA[NH * 64 + i] = a.frag_id[i];
+  for (int i = threadIdx.x; i < NY * 64; i += kThreadsThird) sA[(NH + NI) * 64 + i] = a.frag_y[i];
+  __syncthreads();
+  const Geom& g = a.g;
+  const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
+  for (int it = S.first; it < S.nit; it += S.step) {
+    const NetItem w = net_item(a.sw, S, it);
+    if (w.rhs >= a.sw.nrhs || (a.skip && a.skip[w.rhs * a.skip_stride])) continue;
+    const int site = w.tile * 16 + n;
+    const bool valid = site < g.Vh;
+    const int s = valid ? site : g.Vh - 1;
+    const SiteCoord sc = decode_site(g, s, 1);
+    char* rec = (char*)a.out + (size_t)w.rhs * g.stride * (kOddCh * 48);
+    f32x4 acc[3][MB];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // identity: W3_0 L2(odd)
+#pragma unroll
+    for (int kb = 0; kb < KBI; ++kb) {
+      const uint32_t vo = ((uint32_t)s * kOddCh + 8 + 4 * kb + q) * 48u;
+      bf16x8 B[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) B[c] = as_frag(ld128(rec, 16u * c, vo));
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const bf16x8 A =
+            as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + mb * KBI + kb) * 64 + lane]));
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], acc[c][mb], 0, 0, 0);
+      }
+    }
+    const char* in = (const char*)a.in + (size_t)w.rhs * g.stride * (kEvenCh * 48);
+    hop8<kEvenCh>(a, sc, s, lane, q, in, sA, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    // Y mixes: k-blocks 0, 1 from the L3 accumulators (blocks 2 kb, 2 kb + 1),
+    // k-blocks 2, 3 from L2(odd)
+    bf16x8 B3[2][3];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) B3[kb][c] = pack8(acc[c][2 * kb], acc[c][2 * kb + 1]);
+    f32x4 y[3][MB];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) y[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < KBY; ++kb) {
+      bf16x8 B[3];
+      if (kb < 2) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) B[c] = B3[kb][c];
+      } else {
+        const uint32_t vo = ((uint32_t)s * kOddCh + 8 + 4 * (kb - 2) + q) * 48u;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) B[c] = as_frag(ld128(rec, 16u * c, vo));
+      }
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const bf16x8 A = as_frag(
+            *reinterpret_cast<const U32x4*>(&sA[(NH + NI + mb * KBY + kb) * 64 + lane]));
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          y[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], y[c][mb], 0, 0, 0);
+      }
+    }
+    if (valid) store8<kOddCh>(rec, s, q, y);
   }
 }
 
 // --------------------------------------------------------------- stage 4
-// even sites: z = (W4_0 W3_0) L2(even) + sum_{d>0} T_d [X_d (L3 | L2)(odd)],
-// X_d = [W4_d | W4_0 W3_d] (16 -> 1 channel).  The mixing runs at the
-// neighbour on the matrix cores (fresh accumulators per path), then lanes
-// of groups 0 and 1 (C rows: spins 2q, 2q + 1, re/im) transport their two
-// spins in FP32.  Output: FP32 half fields [rhs][12][stride].
-__global__ void __launch_bounds__(kBThreads) k_bnet_last(BArgs a) {
-  constexpr int KB = 4, KBI = 2;
-  constexpr int NH = kHop * KB, NI = KBI;
-  __shared__ uint4 sA[(NH + NI) * 64];
-  for (int i = threadIdx.x; i < NH * 64; i += kBThreads) sA[i] = a.frag[i];
-  for (int i = threadIdx.x; i < NI * 64; i += kBThreads) sA[NH * 64 + i] = a.frag_id[i];
+// even sites: z = (W4_0 W3_0) L2(even) + sum_{d>0} T_d Y_d(odd).  Lane
+// (site n, group q) holds spin q of z: the identity product's A rows are
+// ordered so that C rows 4q, 4q + 1 are (spin q, re / im); each path's
+// transport of Y_d's spin q (three colours, one word each) accumulates into
+// it on v_dot2_f32_bf16.  Output: FP32 half fields [rhs][12][stride].
+__global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
+  constexpr int KBI = 2;
+  constexpr int NW = kThreadsLast / 64;
+  __shared__ uint4 sA[KBI * 64];
+  for (int i = threadIdx.x; i < KBI * 64; i += kThreadsLast) sA[i] = a.frag_id[i];
   __syncthreads();
   const Geom& g = a.g;
-  const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
-  const int t0 = blockIdx.x * a.tiles_per_block, t1 = min(t0 + a.tiles_per_block, a.tiles);
-  for (int tile = t0; tile < t1; ++tile) {
-    const int site = tile * 16 + n;
+  const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
+  for (int it = S.first; it < S.nit; it += S.step) {
+    const NetItem w = net_item(a.sw, S, it);
+    if (w.rhs >= a.sw.nrhs || (a.skip && a.skip[w.rhs * a.skip_stride])) continue;
+    const int site = w.tile * 16 + n;
     const bool valid = site < g.Vh;
     const int s = valid ? site : g.Vh - 1;
     const SiteCoord sc = decode_site(g, s, 0);
-    for (int rr = wave; rr < a.nrhs; rr += kBThreads / 64) {
-      float2 z[2][3];
-      // identity path: (W4_0 W3_0) L2(even)(x)
-      {
-        const char* in = (const char*)a.in_id + (size_t)rr * g.stride * (kEvenCh * 48);
-        f32x4 C[3];
+    float zr[3], zi[3];
+    {
+      const char* ev = (const char*)a.in_id + (size_t)w.rhs * g.stride * (kEvenCh * 48);
+      f32x4 C[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) C[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int c = 0; c < 3; ++c) C[c] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kb = 0; kb < KBI; ++kb) {
-          const bf16x8 A = as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + kb) * 64 + lane]));
-          const uint32_t vo = ((uint32_t)s * kEvenCh + 4 * kb + q) * 48u;
+      for (int kb = 0; kb < KBI; ++kb) {
+        const bf16x8 A = as_frag(*reinterpret_cast<const U32x4*>(&sA[kb * 64 + lane]));
+        const uint32_t vo = ((uint32_t)s * kEvenCh + 4 * kb + q) * 48u;
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
-            C[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, as_frag(ld128(in, 16u * c, vo)), C[c],
-                                                           0, 0, 0);
-        }
-#pragma unroll
-        for (int sp = 0; sp < 2; ++sp)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) z[sp][c] = make_float2(C[c][2 * sp], C[c][2 * sp + 1]);
+        for (int c = 0; c < 3; ++c)
+          C[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, as_frag(ld128(ev, 16u * c, vo)), C[c],
+                                                         0, 0, 0);
       }
-      const char* in = (const char*)a.in + (size_t)rr * g.stride * (kOddCh * 48);
-#pragma unroll 1
-      for (int dh = 0; dh < kHop; ++dh) {
-        const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
-        f32x4 C[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) C[c] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-          const bf16x8 A = as_frag(*reinterpret_cast<const U32x4*>(&sA[(dh * KB + kb) * 64 + lane]));
-          const uint32_t vo = ((uint32_t)nb * kOddCh + 4 * kb + q) * 48u;
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            C[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, as_frag(ld128(in, 16u * c, vo)), C[c],
-                                                           0, 0, 0);
-        }
-        if (q < 2) {
-          U32x4 L[5];
-#pragma unroll
-          for (int k = 0; k < 5; ++k)
-            L[k] = ld128(a.links, 0u, link_off(g, dh) + (uint32_t)s * 80u + 16u * k);
-          uint32_t W[20];
-#pragma unroll
-          for (int k = 0; k < 5; ++k)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) W[4 * k + e] = L[k][e];
-#pragma unroll
-          for (int ca = 0; ca < 3; ++ca)
-#pragma unroll
-            for (int cb = 0; cb < 3; ++cb) {
-              // M_ab = (Re, Im) from P = (Re, -Im)
-              const float mr = lo_f(W[3 * ca + cb]), mi = -hi_f(W[3 * ca + cb]);
-#pragma unroll
-              for (int sp = 0; sp < 2; ++sp) {
-                const float vr = C[cb][2 * sp], vi = C[cb][2 * sp + 1];
-                z[sp][ca].x = fmaf(mr, vr, z[sp][ca].x);
-                z[sp][ca].x = fmaf(-mi, vi, z[sp][ca].x);
-                z[sp][ca].y = fmaf(mr, vi, z[sp][ca].y);
-                z[sp][ca].y = fmaf(mi, vr, z[sp][ca].y);
-              }
-            }
-        }
+      for (int c = 0; c < 3; ++c) {
+        zr[c] = C[c][0];
+        zi[c] = C[c][1];
       }
-      if (valid && q < 2) {
-        float2* o = (float2*)a.out + (size_t)rr * 12 * g.stride;
+    }
+    const char* od = (const char*)a.in + (size_t)w.rhs * g.stride * (kOddCh * 48);
+#pragma unroll 2
+    for (int dh = 0; dh < kHop; ++dh) {
+      const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
+      uint32_t P[9], y[3];
+      load_link(a.links, link_off(g, dh), (uint32_t)s * (kLinkWords * 4), P);
+      const uint32_t vo = ((uint32_t)nb * kOddCh + dh) * 48u + 4u * q;
 #pragma unroll
-        for (int sp = 0; sp < 2; ++sp)
+      for (int b = 0; b < 3; ++b) y[b] = ld32(od, 16u * b, vo);
 #pragma unroll
-          for (int c = 0; c < 3; ++c)
-            st_at(o, (uint32_t)((3 * (2 * q + sp) + c) * g.stride), (uint32_t)s, z[sp][c]);
-      }
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          zr[c] = dot2(P[3 * c + b], y[b], zr[c]);
+          zi[c] = dot2(s_of(P[3 * c + b]), y[b], zi[c]);
+        }
+    }
+    if (valid) {
+      float2* o = (float2*)a.out + (size_t)w.rhs * 12 * g.stride;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        st_at(o, (uint32_t)((3 * q + c) * g.stride), (uint32_t)s, make_float2(zr[c], zi[c]));
     }
   }
 }
@@ -480,6 +589,9 @@ struct Mix {
   Mix(int np_, int cout_, int cin_) : np(np_), cout(cout_), cin(cin_),
                                      x((size_t)np_ * cout_ * cin_ * 16) {}
   Cd& at(int p, int o, int i, int s, int t) {
+    return x[(((size_t)(p * cout + o) * cin + i) * 4 + s) * 4 + t];
+  }
+  Cd get(int p, int o, int i, int s, int t) const {
     return x[(((size_t)(p * cout + o) * cin + i) * 4 + s) * 4 + t];
   }
 };
@@ -524,10 +636,33 @@ uint16_t bf16_bits(float f) {
   return (uint16_t)(u >> 16);
 }
 
+// (channel or path, spin, re/im) of an A row or column; ok = 0 for padding
+struct Ent {
+  int c, s, r, ok;
+};
+// the standard order: row 8 o + 2 s + ro, column 8 i + 2 t + ri
+Ent std_ent(int idx, int n) { return Ent{idx >> 3, (idx & 7) >> 1, idx & 1, idx < 8 * n}; }
+// column k = 32 kb + 8 q + e of a B operand packed from the accumulator
+// blocks 2 kb and 2 kb + 1 (pack8): C row 4 q + (e & 3) of block
+// 2 kb + e / 4 is channel 4 kb + 2 (e / 4) + q / 2, spin 2 (q & 1) + (e & 3) / 2
+Ent acc_col(int k) {
+  const int kb = k >> 5, q = (k >> 3) & 3, e = k & 7;
+  return Ent{4 * kb + 2 * (e >> 2) + (q >> 1), 2 * (q & 1) + ((e & 3) >> 1), e & 1, 1};
+}
+// rows of the last stage's identity product: row 4 q + j is (spin q, re/im
+// j) for j < 2, padding for j >= 2
+Ent last_row(int row) { return Ent{0, row >> 2, row & 1, (row & 3) < 2}; }
+
+enum ColOrder { kColStd, kColAccL3 };
+enum RowOrder { kRowStd, kRowLast };
+
 // A operands of v_mfma_f32_16x16x32_bf16 for X: entry ((p MB + mb) KB + kb)
 // 64 + lane holds A[16 mb + lane % 16][32 kb + 8 (lane / 16) + e], e = 0..7,
-// A the real embedding: row 8 o + 2 s + ro, column 8 i + 2 t + ri.
-void append_frags(const Mix& X, std::vector<uint4>& out) {
+// A the real embedding [[Re X, -Im X], [Im X, Re X]].  With kColAccL3 the
+// columns of the first 8 input channels follow acc_col (the rest the
+// standard order shifted by 8 channels).
+void append_frags(const Mix& X, std::vector<uint4>& out, RowOrder ro_ = kRowStd,
+                  ColOrder co = kColStd) {
   const int MB = fr_mb(X.cout), KB = fr_kb(X.cin);
   for (int p = 0; p < X.np; ++p)
     for (int mb = 0; mb < MB; ++mb)
@@ -535,14 +670,14 @@ void append_frags(const Mix& X, std::vector<uint4>& out) {
         for (int lane = 0; lane < 64; ++lane) {
           uint16_t h[8];
           const int row = 16 * mb + (lane & 15);
+          const Ent R = ro_ == kRowStd ? std_ent(row, X.cout) : last_row(row);
           for (int e = 0; e < 8; ++e) {
             const int k = 32 * kb + 8 * (lane >> 4) + e;
+            const Ent K = (co == kColAccL3 && k < 64) ? acc_col(k) : std_ent(k, X.cin);
             double v = 0.0;
-            if (row < 8 * X.cout && k < 8 * X.cin) {
-              const int o = row >> 3, s = (row & 7) >> 1, ro = row & 1;
-              const int i = k >> 3, t = (k & 7) >> 1, ri = k & 1;
-              const Cd c = const_cast<Mix&>(X).at(p, o, i, s, t);
-              v = ro == 0 ? (ri == 0 ? c.real() : -c.imag()) : (ri == 0 ? c.imag() : c.real());
+            if (R.ok && K.ok) {
+              const Cd c = X.get(p, R.c, K.c, R.s, K.s);
+              v = R.r == 0 ? (K.r == 0 ? c.real() : -c.imag()) : (K.r == 0 ? c.imag() : c.real());
             }
             h[e] = bf16_bits((float)v);
           }
@@ -552,19 +687,31 @@ void append_frags(const Mix& X, std::vector<uint4>& out) {
         }
 }
 
+constexpr int kRgCands[] = {1, 2, 3, 4, 6, 12};
+
 struct BNet {
   int version = -1;            // weight version the fragments were built from
   uint4* frag = nullptr;
-  size_t off[7] = {};          // first, mid0, mid0_id, mid1, mid1_id, last, last_id
+  size_t off[7] = {};          // first, mid, mid_id, third, third_id, third_y, last_id
   uint32_t* odd = nullptr;     // [rhs][stride][16][12] bf16 pairs
   uint32_t* even = nullptr;    // [rhs][stride][8][12]
   int act_rhs = 0;
+  int ncu = 0;                 // compute units
+  int rg[4] = {0, 0, 0, 0};    // rhs per group of each stage (0: not chosen)
+  int tuned_nrhs = 0;          // the rhs count rg was chosen for
+  std::string report;
 };
 
 BNet* bnet_of(ii_context* ctx) { return static_cast<BNet*>(ctx->bnet); }
 
 ii_status bnet_get(ii_context* ctx, BNet** out) {
-  if (!ctx->bnet) ctx->bnet = new BNet();
+  if (!ctx->bnet) {
+    BNet* B = new BNet();
+    ctx->bnet = B;
+    int n = 0;
+    II_HIP(ctx, hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    B->ncu = std::max(1, n);
+  }
   *out = bnet_of(ctx);
   return II_OK;
 }
@@ -625,42 +772,45 @@ ii_status bnet_frags(ii_context* ctx, BNet* B) {
     B->off[2] = f.size();
     append_frags(I, f);
   }
-  // stage 3: W3_d on L2(even); identity W3_0 on L2(odd)
+  // stage 3: W3_d on L2(even); identity W3_0 on L2(odd); the Y mixes
+  // [W4_d | W4_0 W3_d] on [L3 | L2](odd), out "channel" = path dh
   {
-    Mix X(kHop, 8, 8), I(1, 8, 8);
+    Mix X(kHop, 8, 8), I(1, 8, 8), Y(1, kHop, kOddCh);
     for (int dh = 0; dh < kHop; ++dh)
       for (int o = 0; o < 8; ++o)
         for (int i = 0; i < 8; ++i) put(X, dh, o, i, W3(dh + 1, o, i));
     for (int o = 0; o < 8; ++o)
       for (int i = 0; i < 8; ++i) put(I, 0, o, i, W3(0, o, i));
+    for (int dh = 0; dh < kHop; ++dh)
+      for (int j = 0; j < 8; ++j) {
+        put(Y, 0, dh, j, W4(dh + 1, j));
+        Spin e{};
+        for (int k = 0; k < 8; ++k) add_to(e, mul(W4(0, k), W3(dh + 1, k, j)));
+        put(Y, 0, dh, 8 + j, e);
+      }
     B->off[3] = f.size();
     append_frags(X, f);
     B->off[4] = f.size();
     append_frags(I, f);
+    B->off[5] = f.size();
+    append_frags(Y, f, kRowStd, kColAccL3);
   }
-  // stage 4: [W4_d | W4_0 W3_d] on (L3 | L2)(odd) at the neighbour; identity
-  // W4_0 W3_0 on L2(even)
+  // stage 4: identity W4_0 W3_0 on L2(even), rows (spin q, re/im) at 4 q
   {
-    Mix X(kHop, 1, kOddCh), I(1, 1, 8);
-    for (int dh = 0; dh < kHop; ++dh)
-      for (int j = 0; j < 8; ++j) {
-        put(X, dh, 0, j, W4(dh + 1, j));
-        Spin e{};
-        for (int k = 0; k < 8; ++k) add_to(e, mul(W4(0, k), W3(dh + 1, k, j)));
-        put(X, dh, 0, 8 + j, e);
-      }
+    Mix I(1, 1, 8);
     for (int i = 0; i < 8; ++i) {
       Spin e{};
       for (int k = 0; k < 8; ++k) add_to(e, mul(W4(0, k), W3(0, k, i)));
       put(I, 0, 0, i, e);
     }
-    B->off[5] = f.size();
-    append_frags(X, f);
     B->off[6] = f.size();
-    append_frags(I, f);
+    append_frags(I, f, kRowLast, kColStd);
   }
   if (!B->frag) II_HIP(ctx, hipMalloc(&B->frag, f.size() * sizeof(uint4)));
-  II_HIP(ctx, hipMemcpy(B->frag, f.data(), f.size() * sizeof(uint4), hipMemcpyHostToDevice));
+  // on the context stream (after any launch still reading the old operands)
+  II_HIP(ctx, hipMemcpyAsync(B->frag, f.data(), f.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                             ctx->stream));
+  II_HIP(ctx, hipStreamSynchronize(ctx->stream));
   B->version = ver;
   return II_OK;
 }
@@ -679,47 +829,26 @@ ii_status bnet_acts(ii_context* ctx, BNet* B, int nrhs) {
   return II_OK;
 }
 
-// blocks of a stage: 2 per CU (the A operands of stages 2 and 3 take ~70 KB
-// of LDS), each a contiguous run of tiles, so that the blocks a time slice
-// apart (8 blocks, the same XCD under round-robin dispatch) read the same
-// planes at about the same time
-void bnet_grid(const Geom& g, BArgs& a, int* nblk) {
-  a.tiles = (g.Vh + 15) / 16;
-  int nb = std::min(512, a.tiles);
-  a.tiles_per_block = (a.tiles + nb - 1) / nb;
-  *nblk = (a.tiles + a.tiles_per_block - 1) / a.tiles_per_block;
-}
-
-}  // namespace
-
-// links, weights and activation buffers for nrhs rhs: everything
-// bnet_apply needs to launch without allocating (so that it can be captured)
-ii_status bnet_prepare(ii_context* ctx, int nrhs) {
-  BNet* B = nullptr;
-  II_CHECK(bnet_get(ctx, &B));
-  II_CHECK(bnet_links(ctx));
-  II_CHECK(bnet_frags(ctx, B));
-  return bnet_acts(ctx, B, nrhs);
-}
-
-// launches stage `stage` (0..3) of z = M r
-ii_status bnet_stage(ii_context* ctx, int stage, int nrhs, const float2* r, float2* z) {
+// launches stage `stage` (0..3) of z = M r with rg rhs per group
+ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, const float2* r, float2* z,
+                      const int* skip = nullptr, int skip_stride = 0) {
   BNet* B = bnet_of(ctx);
   const Geom& g = ctx->g;
   const uint32_t* links = (const uint32_t*)ctx->gauge_lp[ii_context::kGaugePairs];
   const size_t lpar = (size_t)kHop * g.stride * kLinkWords;
   BArgs a{};
   a.g = g;
-  a.nrhs = nrhs;
+  a.skip = skip;
+  a.skip_stride = skip_stride;
   int nblk = 0;
-  bnet_grid(g, a, &nblk);
+  net_grid(g, B->ncu, kStageThreads[stage], nrhs, rg, a.sw, &nblk);
   switch (stage) {
     case 0:  // odd: [L1 | L2](odd) from the transports of r
       a.frag = B->frag + B->off[0];
       a.links = links + lpar;
       a.in = r;
       a.out = B->odd;
-      hipLaunchKernelGGL(k_bnet_first, dim3(nblk), dim3(kBThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL(k_bnet_first, dim3(nblk), dim3(kThreadsFirst), 0, ctx->stream, a);
       break;
     case 1:  // even: L2(even)
       a.frag = B->frag + B->off[1];
@@ -728,35 +857,111 @@ ii_status bnet_stage(ii_context* ctx, int stage, int nrhs, const float2* r, floa
       a.in = B->odd;
       a.in_id = r;
       a.out = B->even;
-      hipLaunchKernelGGL(k_bnet_mid<0>, dim3(nblk), dim3(kBThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL(k_bnet_mid, dim3(nblk), dim3(kThreadsMid), 0, ctx->stream, a);
       break;
-    case 2:  // odd: L3 over L1's channels of the odd records
+    case 2:  // odd: L3, then Y over L1's channels of the odd records
       a.frag = B->frag + B->off[3];
       a.frag_id = B->frag + B->off[4];
+      a.frag_y = B->frag + B->off[5];
       a.links = links + lpar;
       a.in = B->even;
-      a.in_id = B->odd;
       a.out = B->odd;
-      hipLaunchKernelGGL(k_bnet_mid<1>, dim3(nblk), dim3(kBThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL(k_bnet_third, dim3(nblk), dim3(kThreadsThird), 0, ctx->stream, a);
       break;
     default:  // even: z
-      a.frag = B->frag + B->off[5];
       a.frag_id = B->frag + B->off[6];
       a.links = links;
       a.in = B->odd;
       a.in_id = B->even;
       a.out = z;
-      hipLaunchKernelGGL(k_bnet_last, dim3(nblk), dim3(kBThreads), 0, ctx->stream, a);
+      hipLaunchKernelGGL(k_bnet_last, dim3(nblk), dim3(kThreadsLast), 0, ctx->stream, a);
       break;
   }
   II_HIP(ctx, hipGetLastError());
   return II_OK;
 }
 
+// Autotuner of the rhs grouping: times every candidate group size of each
+// stage (events around 3 launches after one warm-up, on zero fields) and
+// keeps the fastest.  Runs once per context and rhs count, before the
+// first application, never inside a stream capture.
+ii_status bnet_tune(ii_context* ctx, BNet* B, int nrhs) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  II_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) return II_OK;  // keep the current choice
+  const size_t he = half_elems(ctx->g);
+  void *pr = nullptr, *pz = nullptr;
+  II_CHECK(workspace(ctx, 86, nrhs * he * sizeof(float2), &pr));
+  II_CHECK(workspace(ctx, 87, nrhs * he * sizeof(float2), &pz));
+  II_HIP(ctx, hipMemsetAsync(pr, 0, nrhs * he * sizeof(float2), ctx->stream));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  II_HIP(ctx, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return set_error(ctx, II_ERR_DEVICE, "network autotuner: event creation failed");
+  }
+  ii_status st = II_OK;
+  std::string rep = "network rhs groups (ms):";
+  for (int stage = 0; stage < 4 && st == II_OK; ++stage) {
+    float best = 1e30f;
+    int best_rg = 1;
+    rep += " stage" + std::to_string(stage) + "[";
+    for (int rg : kRgCands) {
+      if (rg > nrhs) continue;
+      st = bnet_launch(ctx, stage, nrhs, rg, (const float2*)pr, (float2*)pz);
+      if (st != II_OK) break;
+      if (hipEventRecord(e0, ctx->stream) != hipSuccess) { st = II_ERR_DEVICE; break; }
+      for (int i = 0; i < 3 && st == II_OK; ++i)
+        st = bnet_launch(ctx, stage, nrhs, rg, (const float2*)pr, (float2*)pz);
+      if (st != II_OK) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        st = set_error(ctx, II_ERR_DEVICE, "network autotuner: timing failed");
+        break;
+      }
+      ms /= 3.f;
+      char buf[48];
+      std::snprintf(buf, sizeof(buf), " %d:%.3f", rg, ms);
+      rep += buf;
+      if (ms < best) {
+        best = ms;
+        best_rg = rg;
+      }
+    }
+    rep += " -> " + std::to_string(best_rg) + "]";
+    B->rg[stage] = best_rg;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != II_OK) return st;
+  B->tuned_nrhs = nrhs;
+  B->report = rep;
+  return II_OK;
+}
+
+}  // namespace
+
+// links, weights and activation buffers for nrhs rhs, and the tuned launch
+// shapes: everything bnet_apply needs to launch without allocating (so that
+// it can be captured)
+ii_status bnet_prepare(ii_context* ctx, int nrhs) {
+  BNet* B = nullptr;
+  II_CHECK(bnet_get(ctx, &B));
+  II_CHECK(bnet_links(ctx));
+  II_CHECK(bnet_frags(ctx, B));
+  II_CHECK(bnet_acts(ctx, B, nrhs));
+  if (B->tuned_nrhs != nrhs) II_CHECK(bnet_tune(ctx, B, nrhs));
+  return II_OK;
+}
+
 // z = M r for nrhs FP32 even half fields [rhs][12][stride] on the device
-ii_status bnet_apply(ii_context* ctx, int nrhs, const float2* r, float2* z) {
+ii_status bnet_apply(ii_context* ctx, int nrhs, const float2* r, float2* z, const int* skip,
+                     int skip_stride) {
   II_CHECK(bnet_prepare(ctx, nrhs));
-  for (int st = 0; st < 4; ++st) II_CHECK(bnet_stage(ctx, st, nrhs, r, z));
+  BNet* B = bnet_of(ctx);
+  for (int st = 0; st < 4; ++st)
+    II_CHECK(bnet_launch(ctx, st, nrhs, B->rg[st] > 0 ? B->rg[st] : 1, r, z, skip, skip_stride));
   return II_OK;
 }
 
@@ -804,6 +1009,35 @@ ii_status ii_net_apply_batched(ii_context* ctx, int nrhs, const double* r_even, 
   return II_OK;
 }
 
+ii_status ii_net_tune(ii_context* ctx, int nrhs, const int* force_rg, int* rg, char* report,
+                      size_t len) {
+  if (!ctx || nrhs < 1) return II_ERR_BAD_ARG;
+  II_CHECK(require_gauge(ctx));
+  II_CHECK(enter(ctx));
+  BNet* B = nullptr;
+  II_CHECK(bnet_get(ctx, &B));
+  II_CHECK(bnet_links(ctx));
+  II_CHECK(bnet_frags(ctx, B));
+  II_CHECK(bnet_acts(ctx, B, nrhs));
+  if (force_rg) {
+    for (int st = 0; st < 4; ++st)
+      if (force_rg[st] < 1) return set_error(ctx, II_ERR_BAD_ARG, "rhs group below 1");
+    for (int st = 0; st < 4; ++st) B->rg[st] = force_rg[st];
+    B->tuned_nrhs = nrhs;
+    B->report = "network rhs groups forced by the caller";
+  } else {
+    B->tuned_nrhs = 0;
+    II_CHECK(bnet_tune(ctx, B, nrhs));
+  }
+  if (rg)
+    for (int st = 0; st < 4; ++st) rg[st] = B->rg[st];
+  if (report && len) {
+    std::strncpy(report, B->report.c_str(), len - 1);
+    report[len - 1] = '\0';
+  }
+  return II_OK;
+}
+
 ii_status ii_bench_net_batched(ii_context* ctx, int nrhs, int nrep, double* stage_ms) {
   if (!ctx || !stage_ms || nrhs < 1 || nrep < 1) return II_ERR_BAD_ARG;
   II_CHECK(require_gauge(ctx));
@@ -815,10 +1049,11 @@ ii_status ii_bench_net_batched(ii_context* ctx, int nrhs, int nrep, double* stag
   // a defined input: zeros (the stage times do not depend on the values)
   II_HIP(ctx, hipMemsetAsync(pr, 0, nrhs * he * sizeof(float2), ctx->stream));
   II_CHECK(bnet_apply(ctx, nrhs, (const float2*)pr, (float2*)pz));
+  BNet* B = bnet_of(ctx);
   for (int st = 0; st < 4; ++st) {
     II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     for (int i = 0; i < nrep; ++i)
-      II_CHECK(bnet_stage(ctx, st, nrhs, (const float2*)pr, (float2*)pz));
+      II_CHECK(bnet_launch(ctx, st, nrhs, B->rg[st], (const float2*)pr, (float2*)pz));
     II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     II_HIP(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.f;

@@ -62,7 +62,8 @@ struct GScal {
   int cycles;             // restart cycles (FP64 true residuals)
   int stall;              // consecutive cycles without halving |rho|
   int nan;
-  int pad[2];
+  int taken;              // directions taken in the current cycle
+  int pad;
 };
 static_assert(sizeof(GScal) % sizeof(double) == 0, "GScal is read as a double array");
 constexpr int kGInts = sizeof(GScal) / sizeof(int);
@@ -223,6 +224,7 @@ __global__ void k_g_rr(const double* __restrict__ part, int nb, GScal* __restric
   if (threadIdx.x == 0) {
     S.rr = rr;
     S.count += 1;
+    S.taken += 1;
     if (!gfin(rr)) {
       S.nan = 1;
       S.idle = 1;
@@ -245,6 +247,7 @@ __global__ void k_g_cycle(GScal* __restrict__ gs) {
   if (threadIdx.x == 0) {
     S.idle = S.done;
     S.cyc_start = S.rho2;
+    S.taken = 0;
   }
 }
 
@@ -278,15 +281,18 @@ __global__ void k_g_coef(int m, GScal* __restrict__ gs) {
   }
 }
 
-// y += sum_i gamma_i z_i (FP64 solution, FP32 directions)
+// y += sum_i gamma_i z_i (FP64 solution, FP32 directions) over the
+// directions this rhs took in the cycle (the others' z slots were not
+// written for it)
 __global__ void __launch_bounds__(kGThreads)
-k_g_fold(Geom g, const float2* __restrict__ Z, size_t slot, int m, double2* __restrict__ y,
+k_g_fold(Geom g, const float2* __restrict__ Z, size_t slot, int mm, double2* __restrict__ y,
          const GScal* __restrict__ gs) {
   const int rhs = blockIdx.y;
   const GScal& S = gs[rhs];
   if (S.done) return;
   const int hs = blockIdx.x * blockDim.x + threadIdx.x;
   if (hs >= g.Vh) return;
+  const int m = min(mm, S.taken);
   const size_t he = (size_t)12 * g.stride;
   double gr[kMaxK], gi[kMaxK];
 #pragma unroll
@@ -498,7 +504,7 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
       float2* zk = Z + k * slot;
       float2* wk = W + k * slot;
       if (use_net) {
-        II_CHECK(bnet_apply(ctx, nr, R, zk));
+        II_CHECK(bnet_apply(ctx, nr, R, zk, idle, kGInts));
       } else {
         hipLaunchKernelGGL(k_g_copy32, sgrid, dim3(kGThreads), 0, ctx->stream, g, R, zk, gs);
       }

@@ -153,8 +153,10 @@ void learned_release(ii_context* ctx);
 // w = nullptr: the version only)
 ii_status net_host_weights(ii_context* ctx, int* version, std::vector<std::vector<float2>>* w);
 // batched network (ii_bnet.hip): z = M r for nrhs FP32 even half fields
-// [rhs][12][stride] on the device
-ii_status bnet_apply(ii_context* ctx, int nrhs, const float2* r, float2* z);
+// [rhs][12][stride] on the device; rhs k with skip[k * skip_stride] != 0 is
+// left untouched (skip = nullptr: none)
+ii_status bnet_apply(ii_context* ctx, int nrhs, const float2* r, float2* z,
+                     const int* skip = nullptr, int skip_stride = 0);
 // builds what bnet_apply needs (links, weights, buffers) so that its
 // launches can be captured in a graph
 ii_status bnet_prepare(ii_context* ctx, int nrhs);

@@ -8,11 +8,12 @@
 //   * every site traversal the hop autotuner may pick (contiguous with 256
 //     or 128 threads per block, the XCD time-strip orders) and the 64-site
 //     tiles of the multi-rhs hop cover [0, V/2) exactly once;
-//   * the 16-site tiles of the batched network's stage kernels cover every
-//     tile once.
+//   * the batched network's stage kernels take every (16-site tile, rhs)
+//     item once (ii_sweep.h: per-XCD slabs, t-fastest fronts, rhs groups).
 // Built with the host sanitizers (address, undefined behaviour) by
 // tests/test_sanitizers.py; exits non-zero on the first violation.
 #include "../../iroiro_amd/csrc/ii_stencil.h"
+#include "../../iroiro_amd/csrc/ii_sweep.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -63,7 +64,7 @@ static void check_cover(const Geom& g, const HopOrder& o, int bs, const char* wh
 int main() {
   const int lats[][4] = {{2, 2, 2, 2},   {4, 4, 4, 4},   {6, 2, 2, 6},     {6, 4, 4, 6},
                          {4, 6, 2, 4},   {8, 8, 8, 16},  {16, 16, 16, 32}, {24, 24, 24, 48},
-                         {32, 32, 32, 64}, {12, 10, 6, 8}};
+                         {32, 32, 32, 64}, {12, 10, 6, 8},  {8, 8, 32, 32}};
   for (const auto& L : lats) {
     const Geom g = make_geom(L);
     // neighbours
@@ -98,16 +99,35 @@ int main() {
       for (int W : {1, 2, 4, 8, 16})
         if ((g.L[3] / 8) % W == 0) check_cover(g, HopOrder{1, W, P / kBlock}, kBlock, "strip");
     check_cover(g, HopOrder(), 64, "multi-rhs tiles");
-    // batched network: 16-site tiles, contiguous runs per block
-    const int tiles = (g.Vh + 15) / 16;
-    const int nb = std::min(512, tiles);
-    const int tpb = (tiles + nb - 1) / nb;
-    const int nblk = (tiles + tpb - 1) / tpb;
-    std::vector<int> th(tiles, 0);
-    for (int b = 0; b < nblk; ++b)
-      for (int t = b * tpb; t < std::min(b * tpb + tpb, tiles); ++t) th[t]++;
-    for (int t = 0; t < tiles; ++t) CHECK(th[t] == 1, "network tile %d covered %d times", t, th[t]);
-    CHECK(tiles * 16 >= g.Vh && (tiles - 1) * 16 < g.Vh, "network tile count");
+    // batched network: every (tile, rhs) item exactly once for each stage's
+    // block size, rhs count and grouping, on a 256-CU and a 38-CU device
+    for (int ncu : {256, 38})
+      for (int threads : {512, 1024})
+        for (int nrhs : {1, 3, 5, 12})
+          for (int rg : {1, 2, 3, 4, 6, 12}) {
+            NetSweep s{};
+            int nblk = 0;
+            net_grid(g, ncu, threads, nrhs, rg, s, &nblk);
+            CHECK(s.tiles * 16 >= g.Vh && (s.tiles - 1) * 16 < g.Vh, "network tile count");
+            CHECK(nblk >= 1, "network grid");
+            std::vector<int> th((size_t)s.tiles * nrhs, 0);
+            const int nw = threads / 64;
+            for (int b = 0; b < nblk; ++b)
+              for (int w = 0; w < nw; ++w) {
+                const NetRange R = net_range(s, b, nblk, nw, w);
+                for (int it = R.first; it < R.nit; it += R.step) {
+                  const NetItem m = net_item(s, R, it);
+                  if (m.rhs >= nrhs) continue;
+                  CHECK(m.tile >= 0 && m.tile < s.tiles && m.rhs >= 0, "network item %d -> %d",
+                        it, m.tile);
+                  if (m.tile >= 0 && m.tile < s.tiles) th[(size_t)m.tile * nrhs + m.rhs]++;
+                }
+              }
+            for (size_t i = 0; i < th.size(); ++i)
+              CHECK(th[i] == 1, "L=%dx%dx%dx%d network item (tile %d, rhs %d) covered %d times "
+                    "(ncu %d, %d threads, rg %d, nt %d)", L[0], L[1], L[2], L[3],
+                    (int)(i / nrhs), (int)(i % nrhs), th[i], ncu, threads, rg, s.nt);
+          }
   }
   if (fails) return 1;
   std::printf("index_check: ok\n");

@@ -9,8 +9,12 @@ full Wilson operator; lat.ddagd for the shifted systems):
       one fused multi-rhs CG (FP32 inner, FP64 reliable updates), and the
       multishift on Dhat^dag Dhat with 12 shifts log-spaced in [1e-4, 1]
       against the FP64 multishift
+  C4  16^3 x 32, beta = 5.8, kappa = 0.1355: the learned preconditioner
+      trained on 64 plain-CG pairs, 16 fresh Gaussian sources solved by the
+      batched flexible GCR with it, against the same solver without it
   C5  32^3 x 64, beta = 6.0, kappa = 0.1355: every column of the batched
-      mixed-precision propagator solves, error against the plain FP64 CG
+      mixed-precision propagator solves (the flexible GCR with the learned
+      preconditioner, CG and BiCGStab), error against the plain FP64 CG
       solution (one rhs after the other, as Qprop_EvenOdd does)
 
 The caller chain they stand for is Qprop_EvenOdd::calc
@@ -164,21 +168,87 @@ def c5():
     ctx.close()
 
 
-@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+@pytest.mark.parametrize("method", ["fgcr-learned", "cg", "bicgstab"])
 def test_c5_every_column_vs_oracle(c5, method):
     ctx, lat, u, kappa = c5["ctx"], c5["lat"], c5["u"], c5["kappa"]
-    outs = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision="half", batched=True,
-                               method=method).solve(c5["src"], c5["sol"])
+    if method == "fgcr-learned":
+        # the network trained on this gauge field (64 plain-CG pairs)
+        net = ii.LearnedPreconditioner(ctx)
+        rep = net.train(kappa, npairs=64, epochs=8, batch=4, lr=2e-3, pair_tol=1e-8, seed=SEED)
+        print("C5 network trained: loss %.4f -> %.4f" % (rep["loss_first"], rep["loss_last"]),
+              flush=True)
+        solver = ii.PropagatorSolver(ctx, kappa, tol=1e-10, batched=True, method="fgcr",
+                                     use_learned=True, use_graph=True)
+    else:
+        solver = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision="half", batched=True,
+                                     method=method)
+    outs = solver.solve(c5["src"], c5["sol"])
     errs, pairs = [], []
     for k in range(12):
         x = c5["sol"].download(k)
         errs.append(rel(x, c5["x64"].download(k)))
         pairs.append((x, c5["src"].download(k)))
     res = oracle_residuals(lat, u, kappa, pairs)
-    print("C5 batched %s (FP16 inner): iterations %s, oracle residuals max %.2e, "
+    print("C5 batched %s: iterations %s, oracle residuals max %.2e, "
           "|x - x_fp64|/|x_fp64| per rhs %s" % (method, [o.Iterations for o in outs], max(res),
                                                 ["%.1e" % e for e in errs]))
     assert max(res) <= 1e-10
     for o, r in zip(outs, res):
         assert abs(o.true_residual - r) < 1e-12
     assert max(errs) < 1e-7
+
+
+# ------------------------------------------------------------------ C4
+def test_c4_learned_16_fresh_sources():
+    """C4: 16^3 x 32, heat-bath beta = 5.8, kappa = 0.1355.  The network is
+    trained on 64 (Gaussian source, plain-CG solution) pairs of this gauge
+    field, then 16 fresh Gaussian sources are solved in one batch by the
+    flexible GCR with the network as preconditioner.  Every solution meets
+    the full-operator true residual 1e-10 by the oracle and agrees with the
+    plain FP64 CG solution; the network takes fewer directions than the
+    same flexible GCR without it (times of both, and of the plain FP64 CG,
+    are printed)."""
+    import time
+    dims, beta, kappa = (16, 16, 16, 32), 5.8, 0.1355
+    print("C4 setup %s beta=%.1f kappa=%.4f" % (dims, beta, kappa), flush=True)
+    ctx, plaq = heatbath_ctx(dims, beta)
+    try:
+        lat = oracle.Lattice(dims)
+        u = ctx.download_gauge()
+        net = ii.LearnedPreconditioner(ctx)
+        rep = net.train(kappa, npairs=64, epochs=8, batch=4, lr=2e-3, pair_tol=1e-8, seed=SEED)
+        print("C4 trained on 64 pairs: loss %.4f -> %.4f" % (rep["loss_first"], rep["loss_last"]),
+              flush=True)
+        assert rep["loss_last"] < rep["loss_first"]
+        n = 16
+        bs = [oracle.gauss_field(lat.full_size, 1000 + k) for k in range(n)]
+        src = ii.DeviceField(ctx, "full", n)
+        sol = ii.DeviceField(ctx, "full", n)
+        for k in range(n):
+            src.upload(k, bs[k])
+        runs = {}
+        for name, kw in [("learned", dict(method="fgcr", use_learned=True, use_graph=True)),
+                         ("none", dict(method="fgcr", use_learned=False, use_graph=True)),
+                         ("fp64_cg", dict(method="cg", precision="double"))]:
+            s = ii.PropagatorSolver(ctx, kappa, tol=1e-10, batched=(name != "fp64_cg"), **kw)
+            s.solve(src, sol)                      # warm-up
+            t0 = time.time()
+            outs = s.solve(src, sol)
+            dt = time.time() - t0
+            runs[name] = (dt, outs, [sol.download(k) for k in range(n)])
+            print("C4 %s: %.3f s for %d sources, directions/iterations %s"
+                  % (name, dt, n, [o.Iterations for o in outs]), flush=True)
+        xs = runs["learned"][2]
+        res = oracle_residuals(lat, u, kappa, list(zip(xs, bs)))
+        errs = [rel(x, y) for x, y in zip(xs, runs["fp64_cg"][2])]
+        print("C4 learned: oracle residuals max %.2e, |x - x_fp64|/|x_fp64| max %.1e"
+              % (max(res), max(errs)))
+        assert max(res) <= 1e-10
+        for o, r in zip(runs["learned"][1], res):
+            assert abs(o.true_residual - r) < 1e-12
+        assert max(errs) < 1e-7
+        d_net = max(o.Iterations for o in runs["learned"][1])
+        d_none = max(o.Iterations for o in runs["none"][1])
+        assert d_net < d_none, (d_net, d_none)
+    finally:
+        ctx.close()

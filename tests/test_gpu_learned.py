@@ -219,12 +219,15 @@ def test_training_reduces_loss_and_iterations(tmp_path):
         ctx.close()
 
 
-@pytest.mark.parametrize("dims,nrhs", [((4, 4, 4, 8), 5), ((6, 2, 2, 6), 3)])
+@pytest.mark.parametrize("dims,nrhs", [((4, 4, 4, 8), 5), ((6, 2, 2, 6), 3), ((8, 8, 32, 32), 3)])
 def test_batched_apply_vs_oracle(dims, nrhs):
     """ii_net_apply_batched (all rhs per launch; layers regrouped by parity
     with composite weights, bfloat16 activations and links) against the FP64
     oracle network on every rhs, on a lattice whose half volume is a whole
-    number of 16-site tiles and on one that is not (6x2x2x6: 72 sites).
+    number of 16-site tiles and on one that is not (6x2x2x6: 72 sites), and
+    on 8x8x32x32, where on a 256-CU device the blocks own several z-planes
+    each (taken z fastest, regions grouped per XCD), with the autotuned rhs
+    grouping and three forced ones (bit-identical).
     Bound 3e-2: bf16 rounding of the activations of four stages, of the
     transport matrices and of the composite weights (8 significant bits
     each).  A network with one transport path of one layer dropped is more
@@ -240,18 +243,26 @@ def test_batched_apply_vs_oracle(dims, nrhs):
         ws = [net.get_weights(layer).astype(np.complex128) for layer in range(4)]
         rng = np.random.default_rng(3)
         b = rng.normal(size=(nrhs, lat.half_size))
+        rg, report = net.tune(nrhs)
+        print("autotuned rhs groups %s: %s" % (rg, report))
+        assert all(1 <= v <= 12 for v in rg)
         got = net.apply_batched(b)
         single = np.stack([net.apply(b[k]) for k in range(nrhs)])
         ref = orc.Net(lat, u, ws)
         wrong = [w.copy() for w in ws]
         wrong[1][8] = 0.0
         bad = orc.Net(lat, u, wrong)
+        want = [ref.apply(b[k]) for k in range(nrhs)]
         for k in range(nrhs):
-            want = ref.apply(b[k])
-            e = rel(got[k], want)
+            e = rel(got[k], want[k])
             assert e < 3e-2, (k, e)
             assert rel(got[k], single[k]) < 3e-2
-            assert rel(bad.apply(b[k]), want) > 1e-1
+            assert rel(bad.apply(b[k]), want[k]) > 1e-1
+        # every grouping of the rhs gives the same result (the work order
+        # changes, not the arithmetic of a site)
+        for force in [(1, 1, 1, 1), (2, 3, 4, 12), (12, 6, 1, 2)]:
+            net.tune(nrhs, force)
+            assert np.array_equal(net.apply_batched(b), got), force
     finally:
         ctx.close()
 
