@@ -23,7 +23,9 @@
 //   k_bnet_mid    even: L2(even), 8 transported channels of L1(odd) + r
 //   k_bnet_third  odd:  L3(odd) (8 transported channels of L2(even) plus
 //                       L2(odd)), then the 8 one-channel mixes Y_d of
-//                       [L3 | L2](odd), written over L1(odd)'s channels
+//                       [L3 | L2](odd), written path-major
+//                       ([rhs][path][site]: the last stage's reads along a
+//                       path are contiguous)
 //   k_bnet_last   even: z, one transport of the one-channel Y_d per path
 // Channel/spin mixing runs on v_mfma_f32_16x16x32_bf16 (bf16 weights and
 // activations, FP32 accumulation) as in ii_learned.hip: B operand k = 8 i +
@@ -68,12 +70,17 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kHop = 8;          // transported paths: 2 mu (forward), 2 mu + 1 (backward)
 constexpr int kLinkWords = 9;    // P_ab: 36 bytes per site and path
-constexpr int kOddCh = 16;       // odd-site record: [L1 or Y (0..7) | L2 (8..15)]
+constexpr int kOddCh = 16;       // odd-site record: [L1 (0..7) | L2 (8..15)]
 constexpr int kEvenCh = 8;       // even-site record: L2(even)
-// threads per block: the third stage's A operands take 88 KB of LDS (one
-// block of 16 waves per CU), the others' at most 68 KB (two blocks of 8)
-constexpr int kThreadsFirst = 512, kThreadsMid = 512, kThreadsThird = 1024, kThreadsLast = 512;
+// threads per block: 8 waves.  The first three stages keep the next path's
+// or item's input in flight in registers of their own (2 waves per SIMD,
+// one block per CU; 68 and 88 KB of A operands in the middle stages), the
+// last runs 8 waves per SIMD (four blocks per CU).
+constexpr int kThreadsFirst = 512, kThreadsMid = 512, kThreadsThird = 512, kThreadsLast = 512;
 constexpr int kStageThreads[4] = {kThreadsFirst, kThreadsMid, kThreadsThird, kThreadsLast};
+// blocks resident per CU: registers (waves per SIMD x 4 SIMDs / waves per
+// block) and LDS (A operands) allow this many
+constexpr int kStageBlocksPerCU[4] = {1, 1, 1, 4};
 
 constexpr int fr_mb(int cout) { return (8 * cout + 15) / 16; }
 constexpr int fr_kb(int cin) { return (8 * cin + 31) / 32; }
@@ -199,6 +206,7 @@ struct BArgs {
   const void* in;       // hop input (the other parity)
   const void* in_id;    // identity input (the output parity)
   void* out;
+  void* yout;           // the third stage's Y_d, path-major [rhs][8][stride][12]
 };
 
 __device__ __forceinline__ uint32_t link_off(const Geom& g, int dh) {
@@ -239,119 +247,75 @@ k_bnet_links(Geom g, const double2* __restrict__ u, uint32_t* __restrict__ out) 
   }
 }
 
-// --------------------------------------------------------------- stage 1
-// odd sites: [L1(odd) | L2(odd)] (16 channels) from the 8 transports of r.
-// The B operand's channel index is the path: lane group q of k-block kb
-// transports r along path 4 kb + q.  The 8 output k-blocks are formed in
-// two halves (L1, then L2) from the same B operands.
-__global__ void __launch_bounds__(kThreadsFirst, 4) k_bnet_first(BArgs a) {
-  constexpr int MB = fr_mb(kOddCh), KB = 2, NF = MB * KB, MH = MB / 2;
-  constexpr int NW = kThreadsFirst / 64;
-  __shared__ uint4 sA[NF * 64];
-  for (int i = threadIdx.x; i < NF * 64; i += kThreadsFirst) sA[i] = a.frag[i];
-  __syncthreads();
+// ------------------------------------------------------- stages 2 and 3
+// Both stages transport the 8 channels of the neighbour records along the
+// 8 paths.  Each path's link record and neighbour record (both k-blocks)
+// are requested two steps before they are used (item9: three register sets,
+// 2 waves per SIMD with 256 registers each), the identity input likewise,
+// and the next item's first two paths before the item's results are stored.  (On
+// this architecture one counter covers loads and stores in issue order, so
+// loads issued after the stores would also wait for the stores.)
+struct PathBuf {
+  uint32_t L[9];
+  U32x4 v[2][3];  // k-blocks 0 (channels q) and 1 (4 + q) of the neighbour record
+};
+struct ItemPos {
+  int rhs, s;
+  bool valid;
+  SiteCoord c;
+};
+__device__ __forceinline__ ItemPos item_pos(const Geom& g, const NetItem& w, int n, int par) {
+  ItemPos p;
+  p.rhs = w.rhs;
+  const int site = w.tile * 16 + n;
+  p.valid = site < g.Vh;
+  p.s = p.valid ? site : g.Vh - 1;
+  p.c = decode_site(g, p.s, par);
+  return p;
+}
+// The skip flags of the first 64 rhs, read once per wave: lane k holds rhs
+// k's flag, tested with readlane (a flag read from memory inside the item
+// loop would wait for every load in flight).
+__device__ __forceinline__ int skip_lanes(const BArgs& a, int lane) {
+  return (a.skip && lane < a.sw.nrhs) ? a.skip[lane * a.skip_stride] : 0;
+}
+__device__ __forceinline__ bool rhs_live(const BArgs& a, int rhs, int flags) {
+  if (rhs >= a.sw.nrhs) return false;
+  if (!a.skip) return true;
+  return rhs < 64 ? __builtin_amdgcn_readlane(flags, rhs) == 0 : a.skip[rhs * a.skip_stride] == 0;
+}
+// the first item at or after it that is a live rhs (or >= nit)
+__device__ __forceinline__ int next_live(const BArgs& a, const NetRange& S, int it, int flags) {
+  for (; it < S.nit; it += S.step)
+    if (rhs_live(a, net_item(a.sw, S, it).rhs, flags)) break;
+  return it;
+}
+// path dh's link record and both k-blocks (channels q, 4 + q) of the
+// neighbour record (INREC channels per site)
+template <int INREC>
+__device__ __forceinline__ void load_path(const BArgs& a, const ItemPos& P, const char* in, int dh,
+                                          int q, PathBuf& b) {
   const Geom& g = a.g;
-  const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
-  for (int it = S.first; it < S.nit; it += S.step) {
-    const NetItem w = net_item(a.sw, S, it);
-    if (w.rhs >= a.sw.nrhs || (a.skip && a.skip[w.rhs * a.skip_stride])) continue;
-    const int site = w.tile * 16 + n;
-    const bool valid = site < g.Vh;
-    const int s = valid ? site : g.Vh - 1;
-    const SiteCoord sc = decode_site(g, s, 1);
-    const float2* r = (const float2*)a.in + (size_t)w.rhs * 12 * g.stride;
-    uint32_t L[KB][9];
-    U32x4 v[KB][3];
+  load_link(a.links, link_off(g, dh), (uint32_t)P.s * (kLinkWords * 4), b.L);
+  const uint32_t vo = ((uint32_t)nb_of(g, P.c, P.s, dh) * INREC + q) * 48u;
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      const int dh = 4 * kb + q;
-      const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
-      load_link(a.links, 0u, link_off(g, dh) + (uint32_t)s * (kLinkWords * 4), L[kb]);
-      load_r_b2(r, g.stride, nb, v[kb]);
-    }
-    bf16x8 B[KB][3];
+  for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) transport(L[kb], v[kb], B[kb]);
-    // the matrix products start after both transports (their registers are free)
-    __builtin_amdgcn_sched_barrier(0);
-    char* o = (char*)a.out + (size_t)w.rhs * g.stride * (kOddCh * 48);
+    for (int c = 0; c < 3; ++c) b.v[kb][c] = ld128(in, 16u * c, vo + 4u * 48u * kb);
+}
+__device__ __forceinline__ void compute_path(int dh, const PathBuf& b, const uint4* sA, int lane,
+                                             f32x4 acc[3][4]) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      f32x4 acc[3][MH];
+  for (int kb = 0; kb < 2; ++kb) {
+    bf16x8 B[3];
+    transport(b.L, b.v[kb], B);
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const bf16x8 A =
+          as_frag(*reinterpret_cast<const U32x4*>(&sA[((dh * 4 + mb) * 2 + kb) * 64 + lane]));
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int mb = 0; mb < MH; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int mb = 0; mb < MH; ++mb)
-#pragma unroll
-        for (int kb = 0; kb < KB; ++kb) {
-          const bf16x8 A =
-              as_frag(*reinterpret_cast<const U32x4*>(&sA[((h * MH + mb) * KB + kb) * 64 + lane]));
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[kb][c], acc[c][mb], 0, 0, 0);
-        }
-      if (valid) {
-        // C row 4q + j of block mb: channel 2 mb + q/2, spin 2 (q & 1) + j/2, re/im j & 1
-#pragma unroll
-        for (int mb = 0; mb < MH; ++mb) {
-          const int ch = 2 * (h * MH + mb) + (q >> 1);
-          const uint32_t vo = ((uint32_t)s * kOddCh + ch) * 48u + 8u * (q & 1);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            U32x2 wv;
-            wv[0] = pack_b2(acc[c][mb][0], acc[c][mb][1]);
-            wv[1] = pack_b2(acc[c][mb][2], acc[c][mb][3]);
-            st64(o, 16u * c, vo, wv);
-          }
-        }
-      }
-    }
-  }
-}
-
-// ------------------------------------------------------- stages 2 and 3
-// The 8 transported paths of an 8-channel input record (INREC channels per
-// site, channels 0..7 read), added to the accumulators acc[colour][mb] of
-// the 8 output channels.  Per path, the link record and the first k-block
-// of the neighbour record are requested before the first transport, the
-// second k-block after it.
-template <int INREC>
-__device__ __forceinline__ void hop8(const BArgs& a, const SiteCoord& sc, int s, int lane, int q,
-                                     const char* in, const uint4* sA, f32x4 acc[3][4]) {
-  constexpr int MB = 4, KB = 2;
-  const Geom& g = a.g;
-#pragma unroll 1
-  for (int dh = 0; dh < kHop; ++dh) {
-    const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
-    uint32_t L[9];
-    U32x4 v[3];
-    load_link(a.links, link_off(g, dh), (uint32_t)s * (kLinkWords * 4), L);
-    const uint32_t vo = ((uint32_t)nb * INREC + q) * 48u;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] = ld128(in, 16u * c, vo);
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      // one k-block at a time: its transport, the next k-block's loads,
-      // then its products
-      __builtin_amdgcn_sched_barrier(0);
-      bf16x8 B[3];
-      transport(L, v, B);
-      if (kb + 1 < KB) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) v[c] = ld128(in, 16u * c, vo + 4u * 48u * (kb + 1));
-      }
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        const bf16x8 A =
-            as_frag(*reinterpret_cast<const U32x4*>(&sA[((dh * MB + mb) * KB + kb) * 64 + lane]));
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], acc[c][mb], 0, 0, 0);
-      }
+        acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], acc[c][mb], 0, 0, 0);
     }
   }
 }
@@ -375,10 +339,144 @@ __device__ __forceinline__ void store8(char* o, int s, int q, const f32x4 acc[3]
   }
 }
 
+// One item as 9 steps (paths 0..7, then the identity) over three register
+// sets X0, X1, X2, step k in X[k mod 3]; on entry X0 and X1 hold paths 0
+// and 1.  Before step k computes, step k + 2 is requested: a path of this
+// item, the identity input (ident(X2), during step 6), or paths 0 and 1 of
+// the next item (next(X, path), during steps 7 and 8; nothing when there is
+// no next item), so that two steps of loads are in flight while one
+// computes.  The scheduling barriers keep each request ahead of the
+// computation it overlaps (left alone, the scheduler sinks the loads to
+// their first use).
+template <int INREC, typename Ident, typename Next, typename IdStep>
+__device__ __forceinline__ void item9(const BArgs& a, const ItemPos& P, const char* in, int q,
+                                      int lane, const uint4* sA, PathBuf& X0, PathBuf& X1,
+                                      PathBuf& X2, f32x4 acc[3][4], Ident ident, Next next,
+                                      IdStep idstep) {
+#define II_STEP(K, CUR, REQUEST)                  \
+  REQUEST;                                        \
+  __builtin_amdgcn_sched_barrier(0);              \
+  compute_path(K, CUR, sA, lane, acc);            \
+  __builtin_amdgcn_sched_barrier(0);
+  II_STEP(0, X0, load_path<INREC>(a, P, in, 2, q, X2))
+  II_STEP(1, X1, load_path<INREC>(a, P, in, 3, q, X0))
+  II_STEP(2, X2, load_path<INREC>(a, P, in, 4, q, X1))
+  II_STEP(3, X0, load_path<INREC>(a, P, in, 5, q, X2))
+  II_STEP(4, X1, load_path<INREC>(a, P, in, 6, q, X0))
+  II_STEP(5, X2, load_path<INREC>(a, P, in, 7, q, X1))
+  II_STEP(6, X0, ident(X2))
+  II_STEP(7, X1, next(X0, 0))
+#undef II_STEP
+  next(X1, 1);
+  __builtin_amdgcn_sched_barrier(0);
+  idstep(X2);
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// --------------------------------------------------------------- stage 1
+// odd sites: [L1(odd) | L2(odd)] (16 channels) from the 8 transports of r.
+// The B operand's channel index is the path: lane group q of k-block kb
+// transports r along path 4 kb + q.  The 8 output k-blocks are formed in
+// two halves (L1, then L2) from the same B operands.  The next item's links
+// and r are requested once this item's are transported, before its matrix
+// products and stores.
+struct RawFirst {
+  uint32_t L[2][9];
+  float2 r[2][12];
+};
+__device__ __forceinline__ void load_first(const BArgs& a, const ItemPos& P, int q,
+                                           RawFirst& x) {
+  const Geom& g = a.g;
+  const float2* r = (const float2*)a.in + (size_t)P.rhs * 12 * g.stride;
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    const int dh = 4 * kb + q;
+    const int nb = nb_of(g, P.c, P.s, dh);
+    load_link(a.links, 0u, link_off(g, dh) + (uint32_t)P.s * (kLinkWords * 4), x.L[kb]);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) x.r[kb][k] = ld_at(r, (uint32_t)(k * g.stride), (uint32_t)nb);
+  }
+}
+__global__ void __launch_bounds__(kThreadsFirst, 2) k_bnet_first(BArgs a) {
+  constexpr int MB = fr_mb(kOddCh), KB = 2, NF = MB * KB, MH = MB / 2;
+  constexpr int NW = kThreadsFirst / 64;
+  __shared__ uint4 sA[NF * 64];
+  for (int i = threadIdx.x; i < NF * 64; i += kThreadsFirst) sA[i] = a.frag[i];
+  __syncthreads();
+  const Geom& g = a.g;
+  const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
+  const int flags = skip_lanes(a, lane);
+  int it = next_live(a, S, S.first, flags);
+  if (it >= S.nit) return;
+  ItemPos P = item_pos(g, net_item(a.sw, S, it), n, 1);
+  RawFirst X;
+  load_first(a, P, q, X);
+  for (;;) {
+    bf16x8 B[KB][3];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      U32x4 v[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp)
+          v[c][sp] = pack_b2(X.r[kb][3 * sp + c].x, X.r[kb][3 * sp + c].y);
+      transport(X.L[kb], v, B[kb]);
+    }
+    const int it2 = next_live(a, S, it + S.step, flags);
+    ItemPos P2 = P;
+    if (it2 < S.nit) {
+      P2 = item_pos(g, net_item(a.sw, S, it2), n, 1);
+      load_first(a, P2, q, X);
+    }
+    // the matrix products and stores after the next item's loads are issued
+    __builtin_amdgcn_sched_barrier(0);
+    char* o = (char*)a.out + (size_t)P.rhs * g.stride * (kOddCh * 48);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      f32x4 acc[3][MH];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int mb = 0; mb < MH; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int mb = 0; mb < MH; ++mb)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+          const bf16x8 A =
+              as_frag(*reinterpret_cast<const U32x4*>(&sA[((h * MH + mb) * KB + kb) * 64 + lane]));
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[kb][c], acc[c][mb], 0, 0, 0);
+        }
+      if (P.valid) {
+        // C row 4q + j of block mb: channel 2 mb + q/2, spin 2 (q & 1) + j/2, re/im j & 1
+#pragma unroll
+        for (int mb = 0; mb < MH; ++mb) {
+          const int ch = 2 * (h * MH + mb) + (q >> 1);
+          const uint32_t vo = ((uint32_t)P.s * kOddCh + ch) * 48u + 8u * (q & 1);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            U32x2 wv;
+            wv[0] = pack_b2(acc[c][mb][0], acc[c][mb][1]);
+            wv[1] = pack_b2(acc[c][mb][2], acc[c][mb][3]);
+            st64(o, 16u * c, vo, wv);
+          }
+        }
+      }
+    }
+    if (it2 >= S.nit) break;
+    it = it2;
+    P = P2;
+  }
+}
+
 // stage 2, even sites: L2(even) = (W2_0 W1_0) r + sum_{d>0} W2_d T_d L1(odd)
 // (hop input: odd records, channels 0..7; identity input: r, FP32, channel
 // 0 of the B operand)
-__global__ void __launch_bounds__(kThreadsMid, 4) k_bnet_mid(BArgs a) {
+__global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
   constexpr int MB = 4, NH = kHop * MB * 2, NI = MB;
   constexpr int NW = kThreadsMid / 64;
   __shared__ uint4 sA[(NH + NI) * 64];
@@ -389,45 +487,77 @@ __global__ void __launch_bounds__(kThreadsMid, 4) k_bnet_mid(BArgs a) {
   const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
-  for (int it = S.first; it < S.nit; it += S.step) {
-    const NetItem w = net_item(a.sw, S, it);
-    if (w.rhs >= a.sw.nrhs || (a.skip && a.skip[w.rhs * a.skip_stride])) continue;
-    const int site = w.tile * 16 + n;
-    const bool valid = site < g.Vh;
-    const int s = valid ? site : g.Vh - 1;
-    const SiteCoord sc = decode_site(g, s, 0);
+  const int flags = skip_lanes(a, lane);
+  int it = next_live(a, S, S.first, flags);
+  if (it >= S.nit) return;
+  ItemPos P = item_pos(g, net_item(a.sw, S, it), n, 0);
+  auto inp = [&](const ItemPos& p) {
+    return (const char*)a.in + (size_t)p.rhs * g.stride * (kOddCh * 48);
+  };
+  PathBuf X0, X1, X2;
+  load_path<kOddCh>(a, P, inp(P), 0, q, X0);
+  load_path<kOddCh>(a, P, inp(P), 1, q, X1);
+  for (;;) {
+    const int it2 = next_live(a, S, it + S.step, flags);
+    const ItemPos P2 = it2 < S.nit ? item_pos(g, net_item(a.sw, S, it2), n, 0) : P;
     f32x4 acc[3][MB];
-    {
-      const float2* r = (const float2*)a.in_id + (size_t)w.rhs * 12 * g.stride;
-      U32x4 v[3];
-      load_r_b2(r, g.stride, s, v);
-      bf16x8 B[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (q != 0) v[c] = U32x4{0u, 0u, 0u, 0u};
-        B[c] = as_frag(v[c]);
-      }
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        const bf16x8 A = as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + mb) * 64 + lane]));
+      for (int mb = 0; mb < MB; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const float2* r = (const float2*)a.in_id + (size_t)P.rhs * 12 * g.stride;
+    item9<kOddCh>(
+        a, P, inp(P), q, lane, sA, X0, X1, X2, acc,
+        // the identity input r (12 complex numbers) in the 24 words of X2.v
+        [&](PathBuf& b) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], f32x4{0.f, 0.f, 0.f, 0.f},
-                                                               0, 0, 0);
-      }
-    }
-    const char* in = (const char*)a.in + (size_t)w.rhs * g.stride * (kOddCh * 48);
-    hop8<kOddCh>(a, sc, s, lane, q, in, sA, acc);
-    if (valid) store8<kEvenCh>((char*)a.out + (size_t)w.rhs * g.stride * (kEvenCh * 48), s, q, acc);
+          for (int k = 0; k < 12; ++k) {
+            const float2 x = ld_at(r, (uint32_t)(k * g.stride), (uint32_t)P.s);
+            b.v[(2 * k) / 12][((2 * k) % 12) / 4][(2 * k) % 4] = __float_as_uint(x.x);
+            b.v[(2 * k + 1) / 12][((2 * k + 1) % 12) / 4][(2 * k + 1) % 4] = __float_as_uint(x.y);
+          }
+        },
+        [&](PathBuf& b, int path) {
+          if (it2 < S.nit) load_path<kOddCh>(a, P2, inp(P2), path, q, b);
+        },
+        // identity: (W2_0 W1_0) r, r in channel 0 (lane group 0) of the B operand
+        [&](const PathBuf& b) {
+          bf16x8 Bi[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            U32x4 v;
+#pragma unroll
+            for (int sp = 0; sp < 4; ++sp) {
+              const int k = 3 * sp + c;
+              const float re = __uint_as_float(b.v[(2 * k) / 12][((2 * k) % 12) / 4][(2 * k) % 4]);
+              const float im =
+                  __uint_as_float(b.v[(2 * k + 1) / 12][((2 * k + 1) % 12) / 4][(2 * k + 1) % 4]);
+              v[sp] = q == 0 ? pack_b2(re, im) : 0u;
+            }
+            Bi[c] = as_frag(v);
+          }
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) {
+            const bf16x8 Af = as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + mb) * 64 + lane]));
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Af, Bi[c], acc[c][mb], 0, 0, 0);
+          }
+        });
+    if (P.valid)
+      store8<kEvenCh>((char*)a.out + (size_t)P.rhs * g.stride * (kEvenCh * 48), P.s, q, acc);
+    if (it2 >= S.nit) break;
+    it = it2;
+    P = P2;
   }
 }
 
 // stage 3, odd sites: L3(odd) = W3_0 L2(odd) + sum_{d>0} W3_d T_d L2(even)
 // (hop input: even records; identity input: channels 8..15 of the odd
 // record), then Y_d = [W4_d | W4_0 W3_d] [L3 | L2](odd) for the 8 paths d
-// (16 -> 1 channel each, two paths per accumulator block), stored as
-// channels 0..7 of the odd record.
-__global__ void __launch_bounds__(kThreadsThird, 4) k_bnet_third(BArgs a) {
+// (16 -> 1 channel each, two paths per accumulator block), stored
+// path-major.
+__global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
   constexpr int MB = 4, KBI = 2, KBY = 4;
   constexpr int NH = kHop * MB * 2, NI = MB * KBI, NY = MB * KBY;
   constexpr int NW = kThreadsThird / 64;
@@ -440,71 +570,95 @@ __global__ void __launch_bounds__(kThreadsThird, 4) k_bnet_third(BArgs a) {
   const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
-  for (int it = S.first; it < S.nit; it += S.step) {
-    const NetItem w = net_item(a.sw, S, it);
-    if (w.rhs >= a.sw.nrhs || (a.skip && a.skip[w.rhs * a.skip_stride])) continue;
-    const int site = w.tile * 16 + n;
-    const bool valid = site < g.Vh;
-    const int s = valid ? site : g.Vh - 1;
-    const SiteCoord sc = decode_site(g, s, 1);
-    char* rec = (char*)a.out + (size_t)w.rhs * g.stride * (kOddCh * 48);
+  const int flags = skip_lanes(a, lane);
+  int it = next_live(a, S, S.first, flags);
+  if (it >= S.nit) return;
+  ItemPos P = item_pos(g, net_item(a.sw, S, it), n, 1);
+  auto inp = [&](const ItemPos& p) {
+    return (const char*)a.in + (size_t)p.rhs * g.stride * (kEvenCh * 48);
+  };
+  PathBuf X0, X1, X2;
+  load_path<kEvenCh>(a, P, inp(P), 0, q, X0);
+  load_path<kEvenCh>(a, P, inp(P), 1, q, X1);
+  for (;;) {
+    const int it2 = next_live(a, S, it + S.step, flags);
+    const ItemPos P2 = it2 < S.nit ? item_pos(g, net_item(a.sw, S, it2), n, 1) : P;
+    char* rec = (char*)a.out + (size_t)P.rhs * g.stride * (kOddCh * 48);
     f32x4 acc[3][MB];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // identity: W3_0 L2(odd)
-#pragma unroll
-    for (int kb = 0; kb < KBI; ++kb) {
-      const uint32_t vo = ((uint32_t)s * kOddCh + 8 + 4 * kb + q) * 48u;
-      bf16x8 B[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) B[c] = as_frag(ld128(rec, 16u * c, vo));
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        const bf16x8 A =
-            as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + mb * KBI + kb) * 64 + lane]));
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], acc[c][mb], 0, 0, 0);
-      }
-    }
-    const char* in = (const char*)a.in + (size_t)w.rhs * g.stride * (kEvenCh * 48);
-    hop8<kEvenCh>(a, sc, s, lane, q, in, sA, acc);
-    __builtin_amdgcn_sched_barrier(0);
-    // Y mixes: k-blocks 0, 1 from the L3 accumulators (blocks 2 kb, 2 kb + 1),
-    // k-blocks 2, 3 from L2(odd)
-    bf16x8 B3[2][3];
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) B3[kb][c] = pack8(acc[c][2 * kb], acc[c][2 * kb + 1]);
     f32x4 y[3][MB];
+    item9<kEvenCh>(
+        a, P, inp(P), q, lane, sA, X0, X1, X2, acc,
+        // the identity input: channels 8..15 (L2(odd)) of the own odd record
+        [&](PathBuf& b) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+          for (int kb = 0; kb < KBI; ++kb) {
+            const uint32_t vo = ((uint32_t)P.s * kOddCh + 8 + 4 * kb + q) * 48u;
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) y[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < 3; ++c) b.v[kb][c] = ld128(rec, 16u * c, vo);
+          }
+        },
+        [&](PathBuf& b, int path) {
+          if (it2 < S.nit) load_path<kEvenCh>(a, P2, inp(P2), path, q, b);
+        },
+        // identity W3_0 L2(odd), then the Y mixes: k-blocks 0, 1 from the L3
+        // accumulators (blocks 2 kb, 2 kb + 1), k-blocks 2, 3 from L2(odd)
+        [&](const PathBuf& b) {
 #pragma unroll
-    for (int kb = 0; kb < KBY; ++kb) {
-      bf16x8 B[3];
-      if (kb < 2) {
+          for (int kb = 0; kb < KBI; ++kb)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) B[c] = B3[kb][c];
-      } else {
-        const uint32_t vo = ((uint32_t)s * kOddCh + 8 + 4 * (kb - 2) + q) * 48u;
+            for (int mb = 0; mb < MB; ++mb) {
+              const bf16x8 Af =
+                  as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + mb * KBI + kb) * 64 + lane]));
 #pragma unroll
-        for (int c = 0; c < 3; ++c) B[c] = as_frag(ld128(rec, 16u * c, vo));
-      }
+              for (int c = 0; c < 3; ++c)
+                acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Af, as_frag(b.v[kb][c]),
+                                                                     acc[c][mb], 0, 0, 0);
+            }
+          bf16x8 B3[2][3];
+#pragma unroll
+          for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) B3[kb][c] = pack8(acc[c][2 * kb], acc[c][2 * kb + 1]);
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) y[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kb = 0; kb < KBY; ++kb)
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+              const bf16x8 Af = as_frag(
+                  *reinterpret_cast<const U32x4*>(&sA[(NH + NI + mb * KBY + kb) * 64 + lane]));
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                y[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    Af, kb < 2 ? B3[kb][c] : as_frag(b.v[kb - 2][c]), y[c][mb], 0, 0, 0);
+            }
+        });
+    if (P.valid) {
+      // Y_d path-major: C row 4q + j of block mb is path 2 mb + q/2, spin
+      // 2 (q & 1) + j/2, re/im j & 1
+      char* yo = (char*)a.yout + (size_t)P.rhs * kHop * g.stride * 48;
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
-        const bf16x8 A = as_frag(
-            *reinterpret_cast<const U32x4*>(&sA[(NH + NI + mb * KBY + kb) * 64 + lane]));
+        const int path = 2 * mb + (q >> 1);
+        const uint32_t vo = ((uint32_t)path * (uint32_t)g.stride + (uint32_t)P.s) * 48u + 8u * (q & 1);
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          y[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], y[c][mb], 0, 0, 0);
+        for (int c = 0; c < 3; ++c) {
+          U32x2 wv;
+          wv[0] = pack_b2(y[c][mb][0], y[c][mb][1]);
+          wv[1] = pack_b2(y[c][mb][2], y[c][mb][3]);
+          st64(yo, 16u * c, vo, wv);
+        }
       }
     }
-    if (valid) store8<kOddCh>(rec, s, q, y);
+    if (it2 >= S.nit) break;
+    it = it2;
+    P = P2;
   }
 }
 
@@ -524,9 +678,10 @@ __global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
   const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
+  const int flags = skip_lanes(a, lane);
   for (int it = S.first; it < S.nit; it += S.step) {
     const NetItem w = net_item(a.sw, S, it);
-    if (w.rhs >= a.sw.nrhs || (a.skip && a.skip[w.rhs * a.skip_stride])) continue;
+    if (!rhs_live(a, w.rhs, flags)) continue;
     const int site = w.tile * 16 + n;
     const bool valid = site < g.Vh;
     const int s = valid ? site : g.Vh - 1;
@@ -552,15 +707,17 @@ __global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
         zi[c] = C[c][1];
       }
     }
-    const char* od = (const char*)a.in + (size_t)w.rhs * g.stride * (kOddCh * 48);
+    const char* yb = (const char*)a.in + (size_t)w.rhs * kHop * g.stride * 48;
 #pragma unroll 2
     for (int dh = 0; dh < kHop; ++dh) {
       const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
       uint32_t P[9], y[3];
       load_link(a.links, link_off(g, dh), (uint32_t)s * (kLinkWords * 4), P);
-      const uint32_t vo = ((uint32_t)nb * kOddCh + dh) * 48u + 4u * q;
+      // Y_d at the neighbour: path-major, so the 16 sites of a tile read one
+      // contiguous run along a path
+      const uint32_t so = (uint32_t)dh * (uint32_t)g.stride * 48u;
 #pragma unroll
-      for (int b = 0; b < 3; ++b) y[b] = ld32(od, 16u * b, vo);
+      for (int b = 0; b < 3; ++b) y[b] = ld32(yb, so + 16u * b, (uint32_t)nb * 48u + 4u * q);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -687,7 +844,10 @@ void append_frags(const Mix& X, std::vector<uint4>& out, RowOrder ro_ = kRowStd,
         }
 }
 
-constexpr int kRgCands[] = {1, 2, 3, 4, 6, 12};
+// autotuner candidates: rhs per group, (y, z) block of the sweep ((1, 0):
+// one block column per y, the whole z extent)
+constexpr int kRgCands[] = {1, 2, 4, 12};
+constexpr int kShapeCands[][2] = {{1, 0}, {8, 4}, {4, 8}, {2, 16}};
 
 struct BNet {
   int version = -1;            // weight version the fragments were built from
@@ -695,9 +855,12 @@ struct BNet {
   size_t off[7] = {};          // first, mid, mid_id, third, third_id, third_y, last_id
   uint32_t* odd = nullptr;     // [rhs][stride][16][12] bf16 pairs
   uint32_t* even = nullptr;    // [rhs][stride][8][12]
+  uint32_t* ybuf = nullptr;    // [rhs][8 paths][stride][12]
   int act_rhs = 0;
   int ncu = 0;                 // compute units
   int rg[4] = {0, 0, 0, 0};    // rhs per group of each stage (0: not chosen)
+  int by[4] = {1, 1, 1, 1};    // (y, z) block of each stage's sweep (bz 0: L_z)
+  int bz[4] = {0, 0, 0, 0};
   int tuned_nrhs = 0;          // the rhs count rg was chosen for
   std::string report;
 };
@@ -820,18 +983,21 @@ ii_status bnet_acts(ii_context* ctx, BNet* B, int nrhs) {
   II_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (B->odd) (void)hipFree(B->odd);
   if (B->even) (void)hipFree(B->even);
-  B->odd = B->even = nullptr;
+  if (B->ybuf) (void)hipFree(B->ybuf);
+  B->odd = B->even = B->ybuf = nullptr;
   B->act_rhs = 0;
   const size_t st = ctx->g.stride;
   II_HIP(ctx, hipMalloc(&B->odd, (size_t)nrhs * st * kOddCh * 48));
   II_HIP(ctx, hipMalloc(&B->even, (size_t)nrhs * st * kEvenCh * 48));
+  II_HIP(ctx, hipMalloc(&B->ybuf, (size_t)nrhs * kHop * st * 48));
   B->act_rhs = nrhs;
   return II_OK;
 }
 
 // launches stage `stage` (0..3) of z = M r with rg rhs per group
-ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, const float2* r, float2* z,
-                      const int* skip = nullptr, int skip_stride = 0) {
+ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, int by, int bz,
+                      const float2* r, float2* z, const int* skip = nullptr,
+                      int skip_stride = 0) {
   BNet* B = bnet_of(ctx);
   const Geom& g = ctx->g;
   const uint32_t* links = (const uint32_t*)ctx->gauge_lp[ii_context::kGaugePairs];
@@ -841,7 +1007,8 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, const float2
   a.skip = skip;
   a.skip_stride = skip_stride;
   int nblk = 0;
-  net_grid(g, B->ncu, kStageThreads[stage], nrhs, rg, a.sw, &nblk);
+  net_grid(g, B->ncu, kStageBlocksPerCU[stage], kStageThreads[stage], nrhs, rg, by,
+           bz > 0 ? bz : g.L[2], a.sw, &nblk);
   switch (stage) {
     case 0:  // odd: [L1 | L2](odd) from the transports of r
       a.frag = B->frag + B->off[0];
@@ -866,12 +1033,13 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, const float2
       a.links = links + lpar;
       a.in = B->even;
       a.out = B->odd;
+      a.yout = B->ybuf;
       hipLaunchKernelGGL(k_bnet_third, dim3(nblk), dim3(kThreadsThird), 0, ctx->stream, a);
       break;
     default:  // even: z
       a.frag_id = B->frag + B->off[6];
       a.links = links;
-      a.in = B->odd;
+      a.in = B->ybuf;
       a.in_id = B->even;
       a.out = z;
       hipLaunchKernelGGL(k_bnet_last, dim3(nblk), dim3(kThreadsLast), 0, ctx->stream, a);
@@ -881,15 +1049,17 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, const float2
   return II_OK;
 }
 
-// Autotuner of the rhs grouping: times every candidate group size of each
-// stage (events around 3 launches after one warm-up, on zero fields) and
-// keeps the fastest.  Runs once per context and rhs count, before the
-// first application, never inside a stream capture.
+// Autotuner of the launch shapes: times every candidate (rhs per group,
+// (y, z) block of the sweep) of each stage (events around 3 launches after
+// one warm-up, on zero fields) and keeps the fastest.  Runs once per
+// context and rhs count, before the first application, never inside a
+// stream capture.
 ii_status bnet_tune(ii_context* ctx, BNet* B, int nrhs) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   II_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
   if (cs != hipStreamCaptureStatusNone) return II_OK;  // keep the current choice
-  const size_t he = half_elems(ctx->g);
+  const Geom& g = ctx->g;
+  const size_t he = half_elems(g);
   void *pr = nullptr, *pz = nullptr;
   II_CHECK(workspace(ctx, 86, nrhs * he * sizeof(float2), &pr));
   II_CHECK(workspace(ctx, 87, nrhs * he * sizeof(float2), &pz));
@@ -901,36 +1071,55 @@ ii_status bnet_tune(ii_context* ctx, BNet* B, int nrhs) {
     return set_error(ctx, II_ERR_DEVICE, "network autotuner: event creation failed");
   }
   ii_status st = II_OK;
-  std::string rep = "network rhs groups (ms):";
+  std::string rep = "network (rhs per group, y x z block) ms:";
   for (int stage = 0; stage < 4 && st == II_OK; ++stage) {
     float best = 1e30f;
-    int best_rg = 1;
+    int brg = 1, bby = 1, bbz = 0;
     rep += " stage" + std::to_string(stage) + "[";
     for (int rg : kRgCands) {
-      if (rg > nrhs) continue;
-      st = bnet_launch(ctx, stage, nrhs, rg, (const float2*)pr, (float2*)pz);
-      if (st != II_OK) break;
-      if (hipEventRecord(e0, ctx->stream) != hipSuccess) { st = II_ERR_DEVICE; break; }
-      for (int i = 0; i < 3 && st == II_OK; ++i)
-        st = bnet_launch(ctx, stage, nrhs, rg, (const float2*)pr, (float2*)pz);
-      if (st != II_OK) break;
-      float ms = 0.f;
-      if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        st = set_error(ctx, II_ERR_DEVICE, "network autotuner: timing failed");
-        break;
+      if (rg > nrhs && rg != kRgCands[0]) continue;
+      for (const auto& sh : kShapeCands) {
+        // shapes the lattice does not admit fall back to (1, L_z): skip them
+        NetSweep probe{};
+        int nb = 0;
+        net_grid(g, B->ncu, kStageBlocksPerCU[stage], kStageThreads[stage], nrhs, rg, sh[0],
+                 sh[1] > 0 ? sh[1] : g.L[2], probe, &nb);
+        if (sh[1] > 0 && (probe.by != sh[0] || probe.bz != sh[1])) continue;
+        st = bnet_launch(ctx, stage, nrhs, rg, sh[0], sh[1], (const float2*)pr, (float2*)pz);
+        if (st != II_OK) break;
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) {
+          st = set_error(ctx, II_ERR_DEVICE, "network autotuner: timing failed");
+          break;
+        }
+        for (int i = 0; i < 3 && st == II_OK; ++i)
+          st = bnet_launch(ctx, stage, nrhs, rg, sh[0], sh[1], (const float2*)pr, (float2*)pz);
+        if (st != II_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, ctx->stream) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          st = set_error(ctx, II_ERR_DEVICE, "network autotuner: timing failed");
+          break;
+        }
+        ms /= 3.f;
+        char buf[64];
+        std::snprintf(buf, sizeof(buf), " %d,%dx%d:%.3f", rg, probe.by, probe.bz, ms);
+        rep += buf;
+        if (ms < best) {
+          best = ms;
+          brg = rg;
+          bby = sh[0];
+          bbz = sh[1];
+        }
       }
-      ms /= 3.f;
-      char buf[48];
-      std::snprintf(buf, sizeof(buf), " %d:%.3f", rg, ms);
-      rep += buf;
-      if (ms < best) {
-        best = ms;
-        best_rg = rg;
-      }
+      if (st != II_OK) break;
     }
-    rep += " -> " + std::to_string(best_rg) + "]";
-    B->rg[stage] = best_rg;
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), " -> %d,%dx%d]", brg, bby, bbz > 0 ? bbz : g.L[2]);
+    rep += buf;
+    B->rg[stage] = brg;
+    B->by[stage] = bby;
+    B->bz[stage] = bbz;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -961,7 +1150,8 @@ ii_status bnet_apply(ii_context* ctx, int nrhs, const float2* r, float2* z, cons
   II_CHECK(bnet_prepare(ctx, nrhs));
   BNet* B = bnet_of(ctx);
   for (int st = 0; st < 4; ++st)
-    II_CHECK(bnet_launch(ctx, st, nrhs, B->rg[st] > 0 ? B->rg[st] : 1, r, z, skip, skip_stride));
+    II_CHECK(bnet_launch(ctx, st, nrhs, B->rg[st] > 0 ? B->rg[st] : 1, B->by[st], B->bz[st], r, z,
+                         skip, skip_stride));
   return II_OK;
 }
 
@@ -971,6 +1161,7 @@ void bnet_release(ii_context* ctx) {
   if (B->frag) (void)hipFree(B->frag);
   if (B->odd) (void)hipFree(B->odd);
   if (B->even) (void)hipFree(B->even);
+  if (B->ybuf) (void)hipFree(B->ybuf);
   delete B;
   ctx->bnet = nullptr;
 }
@@ -1022,9 +1213,13 @@ ii_status ii_net_tune(ii_context* ctx, int nrhs, const int* force_rg, int* rg, c
   if (force_rg) {
     for (int st = 0; st < 4; ++st)
       if (force_rg[st] < 1) return set_error(ctx, II_ERR_BAD_ARG, "rhs group below 1");
-    for (int st = 0; st < 4; ++st) B->rg[st] = force_rg[st];
+    for (int st = 0; st < 4; ++st) {
+      B->rg[st] = force_rg[st];
+      B->by[st] = 1;
+      B->bz[st] = 0;
+    }
     B->tuned_nrhs = nrhs;
-    B->report = "network rhs groups forced by the caller";
+    B->report = "network rhs groups forced by the caller (sweep blocks (1, L_z))";
   } else {
     B->tuned_nrhs = 0;
     II_CHECK(bnet_tune(ctx, B, nrhs));
@@ -1053,7 +1248,8 @@ ii_status ii_bench_net_batched(ii_context* ctx, int nrhs, int nrep, double* stag
   for (int st = 0; st < 4; ++st) {
     II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     for (int i = 0; i < nrep; ++i)
-      II_CHECK(bnet_launch(ctx, st, nrhs, B->rg[st], (const float2*)pr, (float2*)pz));
+      II_CHECK(bnet_launch(ctx, st, nrhs, B->rg[st], B->by[st], B->bz[st], (const float2*)pr,
+                           (float2*)pz));
     II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     II_HIP(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.f;

@@ -8,9 +8,10 @@
 // in turn (wave j nw + w of the XCD takes items j nw + w, then + the XCD's
 // wave count, ...): at any time the XCD works on one front of consecutive
 // items.  The list: the group of rg rhs, then the tile, then the rhs in the
-// group; tiles of the slab t fastest, then z, then y, so that a tile's
-// t-neighbours are the next items, its z-neighbours nt tiles away and its
-// y-neighbours nt Lz tiles away, inside the front.
+// group; the slab's tiles in (y, z) blocks of by x bz rows, blocks y-major,
+// inside a block t fastest, then y, then z: a front of nt by bz tiles reads
+// neighbour rows of about (nt + 2)(by + 2)(bz + 2) tiles, so the blocks are
+// chosen (by the autotuner) so that a front and its halo stay in L2.
 #ifndef II_SWEEP_H
 #define II_SWEEP_H
 
@@ -26,6 +27,7 @@ struct NetSweep {
   int ytiles;   // tiles per z-plane
   int ztiles;   // tiles per time slice
   int lz;       // L_z
+  int by, bz;   // (y, z) blocks of the slab's tiles (t fastest inside a block)
   int nrhs;
   int rg;       // rhs per group
 };
@@ -36,17 +38,15 @@ struct NetRange {
   int t0, ntl, nit, first, step;  // the XCD's tiles, its item count, this wave's items
 };
 
-// Blocks of a stage: as many as the CUs hold at once (1024 / threads per
-// block per CU: the LDS of the A operands and the 128-register budget of 4
-// waves per SIMD), fewer for small problems, a multiple of 8 where there
-// are at least 8.
-inline void net_grid(const Geom& g, int ncu, int threads, int nrhs, int rg, NetSweep& s,
-                     int* nblk) {
+// Blocks of a stage: as many as the CUs hold at once (per_cu blocks each),
+// fewer for small problems, a multiple of 8 where there are at least 8.
+inline void net_grid(const Geom& g, int ncu, int per_cu, int threads, int nrhs, int rg, int by,
+                     int bz, NetSweep& s, int* nblk) {
   s.tiles = (g.Vh + 15) / 16;
   s.nrhs = nrhs;
   s.rg = rg < 1 ? 1 : (rg > nrhs ? nrhs : rg);
   s.lz = g.L[2];
-  const int per_cu = threads >= 1024 ? 1 : 1024 / threads, nw = threads / 64;
+  const int nw = threads / 64;
   const long need = ((long)s.tiles * nrhs + nw - 1) / nw;
   long nb = (long)ncu * per_cu;
   if (need < nb) nb = need < 1 ? 1 : need;
@@ -62,6 +62,13 @@ inline void net_grid(const Geom& g, int ncu, int threads, int nrhs, int rg, NetS
     s.ztiles = s.ytiles * g.L[2];
     if ((long)s.ztiles * g.L[3] == s.tiles && s.tiles_x == (g.L[3] / 8) * s.ztiles)
       s.nt = g.L[3] / 8;
+  }
+  // the (y, z) blocks: whole numbers of rows, else one block column per y
+  s.by = 1;
+  s.bz = g.L[2];
+  if (s.nt > 0 && by >= 1 && bz >= 1 && s.ytiles % by == 0 && g.L[2] % bz == 0) {
+    s.by = by;
+    s.bz = bz;
   }
 }
 
@@ -83,11 +90,37 @@ __host__ __device__ __forceinline__ NetItem net_item(const NetSweep& s, const Ne
   const int o = rem / s.rg;
   const int rhs = grp * s.rg + (rem - o * s.rg);
   if (s.nt > 0) {
-    const int tt = o % s.nt, rest = o / s.nt;
-    const int zz = rest % s.lz, yi = rest / s.lz;
+    const int tt = o % s.nt;
+    int rest = o / s.nt;
+    const int yb = rest % s.by;
+    rest /= s.by;
+    const int zb = rest % s.bz;
+    rest /= s.bz;
+    const int nby = s.ytiles / s.by;
+    const int yi = (rest % nby) * s.by + yb, zz = (rest / nby) * s.bz + zb;
     return NetItem{R.t0 + tt * s.ztiles + zz * s.ytiles + yi, rhs};
   }
   return NetItem{R.t0 + o, rhs};
+}
+
+// Half index of the neighbour of site s (coordinates c, either parity) along
+// path dh (mu = dh / 2, forward for even dh): neighbour() of ii_internal.h
+// written as s plus a step and a wrap correction that each select between 0
+// and one wave-uniform value, so that no uniform value needs a vector
+// register of its own.
+__host__ __device__ __forceinline__ int sel0(bool c, int v) { return c ? v : 0; }
+__host__ __device__ __forceinline__ int nb_of(const Geom& g, const SiteCoord& c, int s, int dh) {
+  const bool odd = (c.x & 1) != 0;
+  switch (dh) {
+    case 0: return s + sel0(odd, 1) - sel0(odd && c.xh == g.Lxh - 1, g.Lxh);
+    case 1: return s - sel0(!odd, 1) + sel0(!odd && c.xh == 0, g.Lxh);
+    case 2: return s + g.sY - sel0(c.y == g.L[1] - 1, g.L[1] * g.sY);
+    case 3: return s - g.sY + sel0(c.y == 0, g.L[1] * g.sY);
+    case 4: return s + g.sZ - sel0(c.z == g.L[2] - 1, g.L[2] * g.sZ);
+    case 5: return s - g.sZ + sel0(c.z == 0, g.L[2] * g.sZ);
+    case 6: return s + g.sT - sel0(c.t == g.L[3] - 1, g.L[3] * g.sT);
+    default: return s - g.sT + sel0(c.t == 0, g.L[3] * g.sT);
+  }
 }
 
 }  // namespace ii

@@ -4,7 +4,8 @@
 // half volume is and is not a multiple of the tile sizes:
 //   * neighbour() returns a site of the opposite parity inside [0, V/2),
 //     one lattice step away in direction mu, and the backward neighbour of
-//     the forward neighbour is the site itself;
+//     the forward neighbour is the site itself; the batched network's
+//     branch-free nb_of() gives the same site;
 //   * every site traversal the hop autotuner may pick (contiguous with 256
 //     or 128 threads per block, the XCD time-strip orders) and the 64-site
 //     tiles of the multi-rhs hop cover [0, V/2) exactly once;
@@ -86,6 +87,10 @@ int main() {
                     L[0], L[1], L[2], L[3], hs, mu, fw, k, no[k], want);
             }
             CHECK(neighbour(g, n, nb, mu, fw == 0) == hs, "neighbour not involutive at %d", hs);
+            CHECK(nb_of(g, c, hs, 2 * mu + (fw ? 0 : 1)) == nb,
+                  "L=%dx%dx%dx%d site %d parity %d mu %d fw %d: network neighbour %d, not %d",
+                  L[0], L[1], L[2], L[3], hs, par, mu, fw, nb_of(g, c, hs, 2 * mu + (fw ? 0 : 1)),
+                  nb);
           }
       }
     // hop traversals
@@ -102,12 +107,15 @@ int main() {
     // batched network: every (tile, rhs) item exactly once for each stage's
     // block size, rhs count and grouping, on a 256-CU and a 38-CU device
     for (int ncu : {256, 38})
-      for (int threads : {512, 1024})
-        for (int nrhs : {1, 3, 5, 12})
-          for (int rg : {1, 2, 3, 4, 6, 12}) {
+      for (int per_cu : {1, 4})
+        for (int nrhs : {1, 5, 12})
+          for (int rg : {1, 4, 12})
+           for (int shape = 0; shape < 4; ++shape) {
+            const int threads = 512;
+            const int bys[4] = {1, 8, 4, 2}, bzs[4] = {g.L[2], 4, 8, 2};
             NetSweep s{};
             int nblk = 0;
-            net_grid(g, ncu, threads, nrhs, rg, s, &nblk);
+            net_grid(g, ncu, per_cu, threads, nrhs, rg, bys[shape], bzs[shape], s, &nblk);
             CHECK(s.tiles * 16 >= g.Vh && (s.tiles - 1) * 16 < g.Vh, "network tile count");
             CHECK(nblk >= 1, "network grid");
             std::vector<int> th((size_t)s.tiles * nrhs, 0);
@@ -125,8 +133,9 @@ int main() {
               }
             for (size_t i = 0; i < th.size(); ++i)
               CHECK(th[i] == 1, "L=%dx%dx%dx%d network item (tile %d, rhs %d) covered %d times "
-                    "(ncu %d, %d threads, rg %d, nt %d)", L[0], L[1], L[2], L[3],
-                    (int)(i / nrhs), (int)(i % nrhs), th[i], ncu, threads, rg, s.nt);
+                    "(ncu %d, %d blocks per CU, rg %d, nt %d, by %d, bz %d)", L[0], L[1], L[2],
+                    L[3], (int)(i / nrhs), (int)(i % nrhs), th[i], ncu, per_cu, rg, s.nt, s.by,
+                    s.bz);
           }
   }
   if (fails) return 1;
