@@ -230,6 +230,8 @@ def main():
                     help="only time the MD force (Dirac_Wilson_EvenOdd::md_force)")
     ap.add_argument("--rhmc-only", action="store_true",
                     help="only time RationalSolver_CG and the RHMC force (Action_Nf)")
+    ap.add_argument("--kappa", type=float, default=0.0,
+                    help="hopping parameter instead of the configuration's (0: keep it)")
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--train-pairs", type=int, default=64)
     args = ap.parse_args()
@@ -240,7 +242,10 @@ def main():
 
     import iroiro_amd as ii
 
-    cfg = CONFIGS[args.config]
+    cfg = dict(CONFIGS[args.config])
+    if args.kappa > 0:
+        cfg["kappa"] = args.kappa
+        cfg["workload"] = re.sub(r"_k[0-9.]+_", "_k%.4f_" % args.kappa, cfg["workload"])
     dims = cfg["dims"]
     kappa = cfg["kappa"]
     ctx = ii.Context(dims)
