@@ -23,9 +23,9 @@
 //   k_bnet_mid    even: L2(even), 8 transported channels of L1(odd) + r
 //   k_bnet_third  odd:  L3(odd) (8 transported channels of L2(even) plus
 //                       L2(odd)), then the 8 one-channel mixes Y_d of
-//                       [L3 | L2](odd), written path-major
-//                       ([rhs][path][site]: the last stage's reads along a
-//                       path are contiguous)
+//                       [L3 | L2](odd), written in colour planes
+//                       ([rhs][path][colour][site]: the last stage's reads
+//                       along a path are contiguous)
 //   k_bnet_last   even: z, one transport of the one-channel Y_d per path
 // Channel/spin mixing runs on v_mfma_f32_16x16x32_bf16 (bf16 weights and
 // activations, FP32 accumulation) as in ii_learned.hip: B operand k = 8 i +
@@ -41,8 +41,9 @@
 // v_b = (Re, Im) bf16.
 //
 // Activations are bfloat16, one dword (re, im) per complex number, in
-// records [rhs][site][channel][colour][spin]: the 16 bytes of one channel
-// and colour are exactly one B operand (spins 0..3, re/im).  The input r and
+// records [rhs][site][colour][channel][spin]: the 16 bytes of one channel
+// and colour are exactly one B operand (spins 0..3, re/im), and the four
+// channels one load instruction of a lane group reads are contiguous.  The input r and
 // the output z are FP32 half fields [rhs][12 = 3 spin + colour][stride].
 //
 // Order of work (ii_sweep.h): the waves of the blocks on one XCD share
@@ -86,17 +87,49 @@ constexpr int fr_mb(int cout) { return (8 * cout + 15) / 16; }
 constexpr int fr_kb(int cin) { return (8 * cin + 31) / 32; }
 
 // ------------------------------------------------------------ device helpers
+// Cache policies (0 = default): streaming (non-temporal) loads of the links
+// or of the identity inputs, or streaming stores, each made the stages
+// 5-30% slower at 32^3x64 (the links are re-read once per rhs).
+constexpr int kLinkCP = 0, kIdCP = 0, kStCP = 0;
+template <int CP = 0>
 __device__ __forceinline__ U32x4 ld128(const void* base, uint32_t soff, uint32_t voff) {
-  return __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), voff, soff, 0);
+  return __builtin_amdgcn_raw_buffer_load_b128(rsrc(base), voff, soff, CP);
 }
+template <int CP = 0>
 __device__ __forceinline__ uint32_t ld32(const void* base, uint32_t soff, uint32_t voff) {
-  return __builtin_amdgcn_raw_buffer_load_b32(rsrc(base), voff, soff, 0);
+  return __builtin_amdgcn_raw_buffer_load_b32(rsrc(base), voff, soff, CP);
 }
-// Activation records are written in 8-byte pieces spread over 48-byte
-// channel slots; they go through L2 (normal policy) so that the pieces of a
-// line merge there.  Streaming them (kNT) made the stages 1.2-3x slower.
+// Activation records are written in 8-byte pieces (a lane group writes 32
+// contiguous bytes); they go through L2 (normal policy) so that the pieces
+// of a line merge there.  Streaming them (kNT) made the stages 1.2-3x slower.
 __device__ __forceinline__ void st64(void* base, uint32_t soff, uint32_t voff, U32x2 v) {
-  __builtin_amdgcn_raw_buffer_store_b64(v, rsrc(base), voff, soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(v, rsrc(base), voff, soff, kStCP);
+}
+__device__ __forceinline__ void st128(void* base, uint32_t soff, uint32_t voff, U32x4 v) {
+  __builtin_amdgcn_raw_buffer_store_b128(v, rsrc(base), voff, soff, kStCP);
+}
+
+// Activation records: [rhs][site][colour][channel] with 16 bytes per
+// channel (spins 0..3 as bf16 (re, im) pairs), REC channels per site, so
+// the 4 channels a lane group loads for one colour are 64 contiguous bytes.
+// rec_off: byte offset of channel ch of site s (vector part); rec_colour:
+// the colour's byte offset (scalar part).
+template <int REC>
+__device__ __forceinline__ uint32_t rec_off(uint32_t s, int ch) {
+  return s * (uint32_t)(REC * 48) + 16u * (uint32_t)ch;
+}
+template <int REC>
+__device__ __forceinline__ uint32_t rec_colour(int c) {
+  return (uint32_t)(REC * 16 * c);
+}
+// The third stage's Y_d: [rhs][path][colour][site] with 16 bytes per site
+// (spins 0..3), so a tile's 16 sites of one path and colour are one
+// contiguous run.
+__device__ __forceinline__ uint32_t y_off(uint32_t stride, int path, uint32_t s) {
+  return ((uint32_t)path * 3u * stride + s) * 16u;
+}
+__device__ __forceinline__ uint32_t y_colour(uint32_t stride, int c) {
+  return (uint32_t)c * stride * 16u;
 }
 
 __device__ __forceinline__ bf16x8 as_frag(U32x4 v) {
@@ -109,6 +142,13 @@ __device__ __forceinline__ bf16x2v as_b2(uint32_t v) {
   __builtin_memcpy(&r, &v, 4);
   return r;
 }
+#ifdef II_BNET_EMU8
+__device__ __forceinline__ float emu8(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7ffffu + ((u >> 20) & 1u);
+  return __uint_as_float(u & 0xfff00000u);
+}
+#endif
 __device__ __forceinline__ uint32_t pack_b2(float re, float im) {
   bf16x2v r;
   r[0] = (__bf16)re;
@@ -116,6 +156,14 @@ __device__ __forceinline__ uint32_t pack_b2(float re, float im) {
   uint32_t w;
   __builtin_memcpy(&w, &r, 4);
   return w;
+}
+// a stored activation (records and Y)
+__device__ __forceinline__ uint32_t pack_st(float re, float im) {
+#ifdef II_BNET_EMU8
+  re = emu8(re);
+  im = emu8(im);
+#endif
+  return pack_b2(re, im);
 }
 // the 8 C values of two accumulator blocks as one B operand
 __device__ __forceinline__ bf16x8 pack8(f32x4 a, f32x4 b) {
@@ -172,26 +220,13 @@ __device__ __forceinline__ void transport(const uint32_t P[9], const U32x4 v[3],
 // the link record at byte offset soff + voff (36 bytes)
 __device__ __forceinline__ void load_link(const void* links, uint32_t soff, uint32_t voff,
                                           uint32_t P[9]) {
-  const U32x4 x = ld128(links, soff, voff), y = ld128(links, soff, voff + 16u);
+  const U32x4 x = ld128<kLinkCP>(links, soff, voff), y = ld128<kLinkCP>(links, soff, voff + 16u);
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     P[e] = x[e];
     P[4 + e] = y[e];
   }
-  P[8] = ld32(links, soff, voff + 32u);
-}
-
-// r (FP32 half field of one rhs, [12][stride] float2) at site s as bf16
-// pairs per colour
-__device__ __forceinline__ void load_r_b2(const float2* __restrict__ r, int stride, int s,
-                                          U32x4 v[3]) {
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int sp = 0; sp < 4; ++sp) {
-      const float2 x = ld_at(r, (uint32_t)((3 * sp + c) * stride), (uint32_t)s);
-      v[c][sp] = pack_b2(x.x, x.y);
-    }
+  P[8] = ld32<kLinkCP>(links, soff, voff + 32u);
 }
 
 struct BArgs {
@@ -206,7 +241,7 @@ struct BArgs {
   const void* in;       // hop input (the other parity)
   const void* in_id;    // identity input (the output parity)
   void* out;
-  void* yout;           // the third stage's Y_d, path-major [rhs][8][stride][12]
+  void* yout;           // the third stage's Y_d, [rhs][8 paths][3 colours][stride][4 spins]
 };
 
 __device__ __forceinline__ uint32_t link_off(const Geom& g, int dh) {
@@ -297,11 +332,11 @@ __device__ __forceinline__ void load_path(const BArgs& a, const ItemPos& P, cons
                                           int q, PathBuf& b) {
   const Geom& g = a.g;
   load_link(a.links, link_off(g, dh), (uint32_t)P.s * (kLinkWords * 4), b.L);
-  const uint32_t vo = ((uint32_t)nb_of(g, P.c, P.s, dh) * INREC + q) * 48u;
+  const uint32_t vo = rec_off<INREC>((uint32_t)nb_of(g, P.c, P.s, dh), q);
 #pragma unroll
   for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) b.v[kb][c] = ld128(in, 16u * c, vo + 4u * 48u * kb);
+    for (int c = 0; c < 3; ++c) b.v[kb][c] = ld128(in, rec_colour<INREC>(c), vo + 64u * kb);
 }
 __device__ __forceinline__ void compute_path(int dh, const PathBuf& b, const uint4* sA, int lane,
                                              f32x4 acc[3][4]) {
@@ -320,21 +355,24 @@ __device__ __forceinline__ void compute_path(int dh, const PathBuf& b, const uin
   }
 }
 
-// stores the 8 channels of acc[colour][mb] (C layout) into channels 0..7
-// of the record of site s: C row 4q + j of block mb is channel 2 mb + q/2,
-// spin 2 (q & 1) + j/2, re/im j & 1
+// stores 8 channels (ch0 .. ch0 + 7) of the record of site s from
+// acc[colour][mb] (C rows in store_row order: lane group q holds channels
+// ch0 + q in blocks 0, 1 and ch0 + 4 + q in blocks 2, 3), one 16-byte
+// channel per store, so that each store of a lane group writes 64
+// contiguous bytes
 template <int OUTREC>
-__device__ __forceinline__ void store8(char* o, int s, int q, const f32x4 acc[3][4]) {
+__device__ __forceinline__ void store_rec(char* o, int s, int ch0, int q, const f32x4* acc0,
+                                          const f32x4* acc1, const f32x4* acc2) {
+  const f32x4* acc[3] = {acc0, acc1, acc2};
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb) {
-    const int ch = 2 * mb + (q >> 1);
-    const uint32_t vo = ((uint32_t)s * OUTREC + ch) * 48u + 8u * (q & 1);
+  for (int half = 0; half < 2; ++half) {
+    const uint32_t vo = rec_off<OUTREC>((uint32_t)s, ch0 + 4 * half + q);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      U32x2 wv;
-      wv[0] = pack_b2(acc[c][mb][0], acc[c][mb][1]);
-      wv[1] = pack_b2(acc[c][mb][2], acc[c][mb][3]);
-      st64(o, 16u * c, vo, wv);
+      const f32x4 a = acc[c][2 * half], b = acc[c][2 * half + 1];
+      const U32x4 w{pack_st(a[0], a[1]), pack_st(a[2], a[3]), pack_st(b[0], b[1]),
+                    pack_st(b[2], b[3])};
+      st128(o, rec_colour<OUTREC>(c), vo, w);
     }
   }
 }
@@ -451,21 +489,7 @@ __global__ void __launch_bounds__(kThreadsFirst, 2) k_bnet_first(BArgs a) {
           for (int c = 0; c < 3; ++c)
             acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[kb][c], acc[c][mb], 0, 0, 0);
         }
-      if (P.valid) {
-        // C row 4q + j of block mb: channel 2 mb + q/2, spin 2 (q & 1) + j/2, re/im j & 1
-#pragma unroll
-        for (int mb = 0; mb < MH; ++mb) {
-          const int ch = 2 * (h * MH + mb) + (q >> 1);
-          const uint32_t vo = ((uint32_t)P.s * kOddCh + ch) * 48u + 8u * (q & 1);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            U32x2 wv;
-            wv[0] = pack_b2(acc[c][mb][0], acc[c][mb][1]);
-            wv[1] = pack_b2(acc[c][mb][2], acc[c][mb][3]);
-            st64(o, 16u * c, vo, wv);
-          }
-        }
-      }
+      if (P.valid) store_rec<kOddCh>(o, P.s, 8 * h, q, acc[0], acc[1], acc[2]);
     }
     if (it2 >= S.nit) break;
     it = it2;
@@ -512,7 +536,7 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
         [&](PathBuf& b) {
 #pragma unroll
           for (int k = 0; k < 12; ++k) {
-            const float2 x = ld_at(r, (uint32_t)(k * g.stride), (uint32_t)P.s);
+            const float2 x = ld_at<float2, kIdCP>(r, (uint32_t)(k * g.stride), (uint32_t)P.s);
             b.v[(2 * k) / 12][((2 * k) % 12) / 4][(2 * k) % 4] = __float_as_uint(x.x);
             b.v[(2 * k + 1) / 12][((2 * k + 1) % 12) / 4][(2 * k + 1) % 4] = __float_as_uint(x.y);
           }
@@ -545,7 +569,8 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
           }
         });
     if (P.valid)
-      store8<kEvenCh>((char*)a.out + (size_t)P.rhs * g.stride * (kEvenCh * 48), P.s, q, acc);
+      store_rec<kEvenCh>((char*)a.out + (size_t)P.rhs * g.stride * (kEvenCh * 48), P.s, 0, q,
+                         acc[0], acc[1], acc[2]);
     if (it2 >= S.nit) break;
     it = it2;
     P = P2;
@@ -556,7 +581,7 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
 // (hop input: even records; identity input: channels 8..15 of the odd
 // record), then Y_d = [W4_d | W4_0 W3_d] [L3 | L2](odd) for the 8 paths d
 // (16 -> 1 channel each, two paths per accumulator block), stored
-// path-major.
+// in colour planes.
 __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
   constexpr int MB = 4, KBI = 2, KBY = 4;
   constexpr int NH = kHop * MB * 2, NI = MB * KBI, NY = MB * KBY;
@@ -596,9 +621,9 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
         [&](PathBuf& b) {
 #pragma unroll
           for (int kb = 0; kb < KBI; ++kb) {
-            const uint32_t vo = ((uint32_t)P.s * kOddCh + 8 + 4 * kb + q) * 48u;
+            const uint32_t vo = rec_off<kOddCh>((uint32_t)P.s, 8 + 4 * kb + q);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) b.v[kb][c] = ld128(rec, 16u * c, vo);
+            for (int c = 0; c < 3; ++c) b.v[kb][c] = ld128<kIdCP>(rec, rec_colour<kOddCh>(c), vo);
           }
         },
         [&](PathBuf& b, int path) {
@@ -640,19 +665,19 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
             }
         });
     if (P.valid) {
-      // Y_d path-major: C row 4q + j of block mb is path 2 mb + q/2, spin
-      // 2 (q & 1) + j/2, re/im j & 1
+      // Y_d in colour planes [path][colour][site]: C row 4q + j of block mb
+      // is path 2 mb + q/2, spin 2 (q & 1) + j/2, re/im j & 1
       char* yo = (char*)a.yout + (size_t)P.rhs * kHop * g.stride * 48;
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
         const int path = 2 * mb + (q >> 1);
-        const uint32_t vo = ((uint32_t)path * (uint32_t)g.stride + (uint32_t)P.s) * 48u + 8u * (q & 1);
+        const uint32_t vo = y_off((uint32_t)g.stride, path, (uint32_t)P.s) + 8u * (q & 1);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           U32x2 wv;
-          wv[0] = pack_b2(y[c][mb][0], y[c][mb][1]);
-          wv[1] = pack_b2(y[c][mb][2], y[c][mb][3]);
-          st64(yo, 16u * c, vo, wv);
+          wv[0] = pack_st(y[c][mb][0], y[c][mb][1]);
+          wv[1] = pack_st(y[c][mb][2], y[c][mb][3]);
+          st64(yo, y_colour((uint32_t)g.stride, c), vo, wv);
         }
       }
     }
@@ -695,11 +720,11 @@ __global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
 #pragma unroll
       for (int kb = 0; kb < KBI; ++kb) {
         const bf16x8 A = as_frag(*reinterpret_cast<const U32x4*>(&sA[kb * 64 + lane]));
-        const uint32_t vo = ((uint32_t)s * kEvenCh + 4 * kb + q) * 48u;
+        const uint32_t vo = rec_off<kEvenCh>((uint32_t)s, 4 * kb + q);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          C[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, as_frag(ld128(ev, 16u * c, vo)), C[c],
-                                                         0, 0, 0);
+          C[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              A, as_frag(ld128<kIdCP>(ev, rec_colour<kEvenCh>(c), vo)), C[c], 0, 0, 0);
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -713,11 +738,12 @@ __global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
       const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
       uint32_t P[9], y[3];
       load_link(a.links, link_off(g, dh), (uint32_t)s * (kLinkWords * 4), P);
-      // Y_d at the neighbour: path-major, so the 16 sites of a tile read one
-      // contiguous run along a path
-      const uint32_t so = (uint32_t)dh * (uint32_t)g.stride * 48u;
+      // Y_d at the neighbour, colour plane b: the 16 sites of a tile read
+      // one contiguous run of a path's plane
 #pragma unroll
-      for (int b = 0; b < 3; ++b) y[b] = ld32(yb, so + 16u * b, (uint32_t)nb * 48u + 4u * q);
+      for (int b = 0; b < 3; ++b)
+        y[b] = ld32(yb, y_off((uint32_t)g.stride, dh, 0u) + y_colour((uint32_t)g.stride, b),
+                    (uint32_t)nb * 16u + 4u * q);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -809,9 +835,18 @@ Ent acc_col(int k) {
 // rows of the last stage's identity product: row 4 q + j is (spin q, re/im
 // j) for j < 2, padding for j >= 2
 Ent last_row(int row) { return Ent{0, row >> 2, row & 1, (row & 3) < 2}; }
+// rows of a product whose result is stored (store_rec): row 16 mb + 4 q + j,
+// mb = 4 h + m, is channel 8 h + 4 (m / 2) + q, spin 2 (m & 1) + j / 2,
+// re/im j & 1, so that lane group q holds channels 8 h + q and 8 h + 4 + q
+// whole (blocks 4 h, 4 h + 1 and 4 h + 2, 4 h + 3)
+Ent store_row(int row, int n) {
+  const int mb = row >> 4, q = (row >> 2) & 3, j = row & 3, h = mb >> 2, m = mb & 3;
+  const int ch = 8 * h + 4 * (m >> 1) + q;
+  return Ent{ch, 2 * (m & 1) + (j >> 1), j & 1, ch < n};
+}
 
 enum ColOrder { kColStd, kColAccL3 };
-enum RowOrder { kRowStd, kRowLast };
+enum RowOrder { kRowStd, kRowLast, kRowStore };
 
 // A operands of v_mfma_f32_16x16x32_bf16 for X: entry ((p MB + mb) KB + kb)
 // 64 + lane holds A[16 mb + lane % 16][32 kb + 8 (lane / 16) + e], e = 0..7,
@@ -827,7 +862,9 @@ void append_frags(const Mix& X, std::vector<uint4>& out, RowOrder ro_ = kRowStd,
         for (int lane = 0; lane < 64; ++lane) {
           uint16_t h[8];
           const int row = 16 * mb + (lane & 15);
-          const Ent R = ro_ == kRowStd ? std_ent(row, X.cout) : last_row(row);
+          const Ent R = ro_ == kRowStd     ? std_ent(row, X.cout)
+                        : ro_ == kRowStore ? store_row(row, X.cout)
+                                           : last_row(row);
           for (int e = 0; e < 8; ++e) {
             const int k = 32 * kb + 8 * (lane >> 4) + e;
             const Ent K = (co == kColAccL3 && k < 64) ? acc_col(k) : std_ent(k, X.cin);
@@ -853,9 +890,9 @@ struct BNet {
   int version = -1;            // weight version the fragments were built from
   uint4* frag = nullptr;
   size_t off[7] = {};          // first, mid, mid_id, third, third_id, third_y, last_id
-  uint32_t* odd = nullptr;     // [rhs][stride][16][12] bf16 pairs
-  uint32_t* even = nullptr;    // [rhs][stride][8][12]
-  uint32_t* ybuf = nullptr;    // [rhs][8 paths][stride][12]
+  uint32_t* odd = nullptr;     // [rhs][stride][3 colours][16 channels][4 spins] bf16 pairs
+  uint32_t* even = nullptr;    // [rhs][stride][3][8][4]
+  uint32_t* ybuf = nullptr;    // [rhs][8 paths][3 colours][stride][4 spins]
   int act_rhs = 0;
   int ncu = 0;                 // compute units
   int rg[4] = {0, 0, 0, 0};    // rhs per group of each stage (0: not chosen)
@@ -917,7 +954,7 @@ ii_status bnet_frags(ii_context* ctx, BNet* B) {
         put(X, 0, 8 + o, dh, e);
       }
     B->off[0] = f.size();
-    append_frags(X, f);
+    append_frags(X, f, kRowStore);
   }
   // stage 2: W2_d on L1(odd); identity W2_0 W1_0 on r
   {
@@ -931,9 +968,9 @@ ii_status bnet_frags(ii_context* ctx, BNet* B) {
       put(I, 0, o, 0, e);
     }
     B->off[1] = f.size();
-    append_frags(X, f);
+    append_frags(X, f, kRowStore);
     B->off[2] = f.size();
-    append_frags(I, f);
+    append_frags(I, f, kRowStore);
   }
   // stage 3: W3_d on L2(even); identity W3_0 on L2(odd); the Y mixes
   // [W4_d | W4_0 W3_d] on [L3 | L2](odd), out "channel" = path dh
