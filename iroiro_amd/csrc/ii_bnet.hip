@@ -108,19 +108,22 @@ __device__ __forceinline__ void st64(void* base, uint32_t soff, uint32_t voff, U
 __device__ __forceinline__ void st128(void* base, uint32_t soff, uint32_t voff, U32x4 v) {
   __builtin_amdgcn_raw_buffer_store_b128(v, rsrc(base), voff, soff, kStCP);
 }
-
-// Activation records: [rhs][site][colour][channel] with 16 bytes per
-// channel (spins 0..3 as bf16 (re, im) pairs), REC channels per site, so
-// the 4 channels a lane group loads for one colour are 64 contiguous bytes.
-// rec_off: byte offset of channel ch of site s (vector part); rec_colour:
-// the colour's byte offset (scalar part).
-template <int REC>
-__device__ __forceinline__ uint32_t rec_off(uint32_t s, int ch) {
-  return s * (uint32_t)(REC * 48) + 16u * (uint32_t)ch;
+__device__ __forceinline__ void st32(void* base, uint32_t soff, uint32_t voff, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, rsrc(base), voff, soff, kStCP);
 }
-template <int REC>
-__device__ __forceinline__ uint32_t rec_colour(int c) {
-  return (uint32_t)(REC * 16 * c);
+
+// Activation records (the outputs of the first two stages) are OCP e4m3
+// (fp8) bytes with one power-of-two scale per site, rhs and 8-channel half:
+// a half is 192 bytes [lane group q][colour][k-block kb][spin][re/im] for
+// channel 4 kb + q, so the 48 bytes a lane group reads (channels q and
+// 4 + q, three colours) are contiguous and one 16-byte load holds both
+// channels of one colour.  Odd records hold two halves ([L1 | L2], 384
+// bytes per site), even records one (L2(even), 192 bytes).  Scales: FP32
+// [rhs][site][half]; a value is its byte's e4m3 value times the scale.
+constexpr uint32_t kHalf8 = 192;
+template <int H>
+__device__ __forceinline__ uint32_t r8_off(uint32_t s, int h, int q) {
+  return s * (kHalf8 * H) + kHalf8 * (uint32_t)h + 48u * (uint32_t)q;
 }
 // The third stage's Y_d: [rhs][path][colour][site] with 16 bytes per site
 // (spins 0..3), so a tile's 16 sites of one path and colour are one
@@ -142,13 +145,6 @@ __device__ __forceinline__ bf16x2v as_b2(uint32_t v) {
   __builtin_memcpy(&r, &v, 4);
   return r;
 }
-#ifdef II_BNET_EMU8
-__device__ __forceinline__ float emu8(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7ffffu + ((u >> 20) & 1u);
-  return __uint_as_float(u & 0xfff00000u);
-}
-#endif
 __device__ __forceinline__ uint32_t pack_b2(float re, float im) {
   bf16x2v r;
   r[0] = (__bf16)re;
@@ -156,14 +152,6 @@ __device__ __forceinline__ uint32_t pack_b2(float re, float im) {
   uint32_t w;
   __builtin_memcpy(&w, &r, 4);
   return w;
-}
-// a stored activation (records and Y)
-__device__ __forceinline__ uint32_t pack_st(float re, float im) {
-#ifdef II_BNET_EMU8
-  re = emu8(re);
-  im = emu8(im);
-#endif
-  return pack_b2(re, im);
 }
 // the 8 C values of two accumulator blocks as one B operand
 __device__ __forceinline__ bf16x8 pack8(f32x4 a, f32x4 b) {
@@ -187,6 +175,40 @@ __device__ __forceinline__ float dot2(uint32_t a, uint32_t b, float c) {
 // swapped, with the sign of the high half (-Im) flipped on the way down
 __device__ __forceinline__ uint32_t s_of(uint32_t p) {
   return __builtin_amdgcn_alignbit(p, p ^ 0x80000000u, 16);
+}
+
+// fp8 -> bf16 pair of bytes 2 HI, 2 HI + 1 of w, times sc (a power of two)
+template <bool HI>
+__device__ __forceinline__ uint32_t dq2(uint32_t w, float sc) {
+  const bf16x2v v = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(w, sc, HI);
+  uint32_t r;
+  __builtin_memcpy(&r, &v, 4);
+  return r;
+}
+// the B operands (spins 0..3 as bf16 pairs) of one colour's two k-blocks
+// from the 16 record bytes of that colour
+__device__ __forceinline__ void dq8(U32x4 w, float sc, U32x4& v0, U32x4& v1) {
+  v0 = U32x4{dq2<false>(w[0], sc), dq2<true>(w[0], sc), dq2<false>(w[1], sc), dq2<true>(w[1], sc)};
+  v1 = U32x4{dq2<false>(w[2], sc), dq2<true>(w[2], sc), dq2<false>(w[3], sc), dq2<true>(w[3], sc)};
+}
+// the e4m3 bytes of a[0..3] times m (round to nearest even)
+__device__ __forceinline__ uint32_t q8(f32x4 a, float m) {
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(a[0] * m, a[1] * m, 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(a[2] * m, a[3] * m, w, true);
+  return (uint32_t)w;
+}
+// For a largest magnitude mx = f 2^e (f in [1/2, 1)): the scale ds = 2^(e - 8)
+// and its inverse qm, so that mx qm lies in [128, 256), below e4m3's largest
+// value 448; values below mx 2^-14 fall into e4m3's subnormal range.  A zero
+// or denormal maximum gives ds = qm = 1 (its values round to zero).
+__device__ __forceinline__ void scale8(float mx, float& ds, float& qm) {
+  const uint32_t eb = (__float_as_uint(mx) >> 23) & 255u;
+  if (eb < 16u) {
+    ds = qm = 1.0f;
+    return;
+  }
+  ds = __uint_as_float((eb - 7u) << 23);
+  qm = __uint_as_float((261u - eb) << 23);
 }
 
 // B operands of M v: v[b] holds spins 0..3 of colour b as (re, im) bf16
@@ -239,8 +261,11 @@ struct BArgs {
   const uint4* frag_y;  // the Y mixes (third stage)
   const uint32_t* links;  // link records of the output parity: [8][stride][9]
   const void* in;       // hop input (the other parity)
+  const float* in_sc;   // its record scales
   const void* in_id;    // identity input (the output parity)
+  const float* id_sc;   // its record scales
   void* out;
+  float* out_sc;        // the output record's scales
   void* yout;           // the third stage's Y_d, [rhs][8 paths][3 colours][stride][4 spins]
 };
 
@@ -292,7 +317,8 @@ k_bnet_links(Geom g, const double2* __restrict__ u, uint32_t* __restrict__ out) 
 // loads issued after the stores would also wait for the stores.)
 struct PathBuf {
   uint32_t L[9];
-  U32x4 v[2][3];  // k-blocks 0 (channels q) and 1 (4 + q) of the neighbour record
+  U32x4 w[3];  // fp8 record bytes of colour c: channels q (0..7) and 4 + q (8..15)
+  float sc;    // the record half's scale
 };
 struct ItemPos {
   int rhs, s;
@@ -325,25 +351,28 @@ __device__ __forceinline__ int next_live(const BArgs& a, const NetRange& S, int 
     if (rhs_live(a, net_item(a.sw, S, it).rhs, flags)) break;
   return it;
 }
-// path dh's link record and both k-blocks (channels q, 4 + q) of the
-// neighbour record (INREC channels per site)
-template <int INREC>
-__device__ __forceinline__ void load_path(const BArgs& a, const ItemPos& P, const char* in, int dh,
-                                          int q, PathBuf& b) {
+// path dh's link record and half 0 (channels q, 4 + q of every colour) of
+// the neighbour's record (H halves per site) with its scale
+template <int H>
+__device__ __forceinline__ void load_path(const BArgs& a, const ItemPos& P, const char* in,
+                                          const float* insc, int dh, int q, PathBuf& b) {
   const Geom& g = a.g;
   load_link(a.links, link_off(g, dh), (uint32_t)P.s * (kLinkWords * 4), b.L);
-  const uint32_t vo = rec_off<INREC>((uint32_t)nb_of(g, P.c, P.s, dh), q);
+  const uint32_t nb = (uint32_t)nb_of(g, P.c, P.s, dh);
+  const uint32_t vo = r8_off<H>(nb, 0, q);
 #pragma unroll
-  for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) b.v[kb][c] = ld128(in, rec_colour<INREC>(c), vo + 64u * kb);
+  for (int c = 0; c < 3; ++c) b.w[c] = ld128(in, 16u * c, vo);
+  b.sc = __uint_as_float(ld32(insc, 0u, nb * (4u * H)));
 }
 __device__ __forceinline__ void compute_path(int dh, const PathBuf& b, const uint4* sA, int lane,
                                              f32x4 acc[3][4]) {
+  U32x4 v[2][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) dq8(b.w[c], b.sc, v[0][c], v[1][c]);
 #pragma unroll
   for (int kb = 0; kb < 2; ++kb) {
     bf16x8 B[3];
-    transport(b.L, b.v[kb], B);
+    transport(b.L, v[kb], B);
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
       const bf16x8 A =
@@ -355,26 +384,35 @@ __device__ __forceinline__ void compute_path(int dh, const PathBuf& b, const uin
   }
 }
 
-// stores 8 channels (ch0 .. ch0 + 7) of the record of site s from
-// acc[colour][mb] (C rows in store_row order: lane group q holds channels
-// ch0 + q in blocks 0, 1 and ch0 + 4 + q in blocks 2, 3), one 16-byte
-// channel per store, so that each store of a lane group writes 64
-// contiguous bytes
-template <int OUTREC>
-__device__ __forceinline__ void store_rec(char* o, int s, int ch0, int q, const f32x4* acc0,
-                                          const f32x4* acc1, const f32x4* acc2) {
+// stores half h of the record of site s (H halves per site) and its scale
+// from acc[colour][block] in store_row order (lane group q holds channel
+// 8 h + q in blocks 0, 1 and 8 h + 4 + q in blocks 2, 3): the scale comes
+// from the largest magnitude of the half over its four lane groups, then
+// each lane group writes its 48 contiguous bytes.  Every lane takes part in
+// the reduction; only valid sites store.
+template <int H>
+__device__ __forceinline__ void store_half8(char* o, float* osc, int s, int h, int q, bool valid,
+                                            const f32x4* acc0, const f32x4* acc1,
+                                            const f32x4* acc2) {
   const f32x4* acc[3] = {acc0, acc1, acc2};
+  float mx = 0.0f;
 #pragma unroll
-  for (int half = 0; half < 2; ++half) {
-    const uint32_t vo = rec_off<OUTREC>((uint32_t)s, ch0 + 4 * half + q);
+  for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const f32x4 a = acc[c][2 * half], b = acc[c][2 * half + 1];
-      const U32x4 w{pack_st(a[0], a[1]), pack_st(a[2], a[3]), pack_st(b[0], b[1]),
-                    pack_st(b[2], b[3])};
-      st128(o, rec_colour<OUTREC>(c), vo, w);
-    }
-  }
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) mx = fmaxf(mx, fabsf(acc[c][b][j]));
+  mx = fmaxf(mx, __shfl_xor(mx, 16));
+  mx = fmaxf(mx, __shfl_xor(mx, 32));
+  float ds, qm;
+  scale8(mx, ds, qm);
+  if (!valid) return;
+  const uint32_t vo = r8_off<H>((uint32_t)s, h, q);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    st128(o, 16u * c, vo,
+          U32x4{q8(acc[c][0], qm), q8(acc[c][1], qm), q8(acc[c][2], qm), q8(acc[c][3], qm)});
+  if (q == 0) st32(osc, 0u, ((uint32_t)s * H + (uint32_t)h) * 4u, __float_as_uint(ds));
 }
 
 // One item as 9 steps (paths 0..7, then the identity) over three register
@@ -386,9 +424,10 @@ __device__ __forceinline__ void store_rec(char* o, int s, int ch0, int q, const 
 // computes.  The scheduling barriers keep each request ahead of the
 // computation it overlaps (left alone, the scheduler sinks the loads to
 // their first use).
-template <int INREC, typename Ident, typename Next, typename IdStep>
-__device__ __forceinline__ void item9(const BArgs& a, const ItemPos& P, const char* in, int q,
-                                      int lane, const uint4* sA, PathBuf& X0, PathBuf& X1,
+template <int H, typename Ident, typename Next, typename IdStep>
+__device__ __forceinline__ void item9(const BArgs& a, const ItemPos& P, const char* in,
+                                      const float* insc, int q, int lane, const uint4* sA,
+                                      PathBuf& X0, PathBuf& X1,
                                       PathBuf& X2, f32x4 acc[3][4], Ident ident, Next next,
                                       IdStep idstep) {
 #define II_STEP(K, CUR, REQUEST)                  \
@@ -396,12 +435,12 @@ __device__ __forceinline__ void item9(const BArgs& a, const ItemPos& P, const ch
   __builtin_amdgcn_sched_barrier(0);              \
   compute_path(K, CUR, sA, lane, acc);            \
   __builtin_amdgcn_sched_barrier(0);
-  II_STEP(0, X0, load_path<INREC>(a, P, in, 2, q, X2))
-  II_STEP(1, X1, load_path<INREC>(a, P, in, 3, q, X0))
-  II_STEP(2, X2, load_path<INREC>(a, P, in, 4, q, X1))
-  II_STEP(3, X0, load_path<INREC>(a, P, in, 5, q, X2))
-  II_STEP(4, X1, load_path<INREC>(a, P, in, 6, q, X0))
-  II_STEP(5, X2, load_path<INREC>(a, P, in, 7, q, X1))
+  II_STEP(0, X0, load_path<H>(a, P, in, insc, 2, q, X2))
+  II_STEP(1, X1, load_path<H>(a, P, in, insc, 3, q, X0))
+  II_STEP(2, X2, load_path<H>(a, P, in, insc, 4, q, X1))
+  II_STEP(3, X0, load_path<H>(a, P, in, insc, 5, q, X2))
+  II_STEP(4, X1, load_path<H>(a, P, in, insc, 6, q, X0))
+  II_STEP(5, X2, load_path<H>(a, P, in, insc, 7, q, X1))
   II_STEP(6, X0, ident(X2))
   II_STEP(7, X1, next(X0, 0))
 #undef II_STEP
@@ -471,7 +510,8 @@ __global__ void __launch_bounds__(kThreadsFirst, 2) k_bnet_first(BArgs a) {
     }
     // the matrix products and stores after the next item's loads are issued
     __builtin_amdgcn_sched_barrier(0);
-    char* o = (char*)a.out + (size_t)P.rhs * g.stride * (kOddCh * 48);
+    char* o = (char*)a.out + (size_t)P.rhs * g.stride * (2 * kHalf8);
+    float* osc = a.out_sc + (size_t)P.rhs * g.stride * 2;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       f32x4 acc[3][MH];
@@ -489,7 +529,7 @@ __global__ void __launch_bounds__(kThreadsFirst, 2) k_bnet_first(BArgs a) {
           for (int c = 0; c < 3; ++c)
             acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[kb][c], acc[c][mb], 0, 0, 0);
         }
-      if (P.valid) store_rec<kOddCh>(o, P.s, 8 * h, q, acc[0], acc[1], acc[2]);
+      store_half8<2>(o, osc, P.s, h, q, P.valid, acc[0], acc[1], acc[2]);
     }
     if (it2 >= S.nit) break;
     it = it2;
@@ -516,11 +556,12 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
   if (it >= S.nit) return;
   ItemPos P = item_pos(g, net_item(a.sw, S, it), n, 0);
   auto inp = [&](const ItemPos& p) {
-    return (const char*)a.in + (size_t)p.rhs * g.stride * (kOddCh * 48);
+    return (const char*)a.in + (size_t)p.rhs * g.stride * (2 * kHalf8);
   };
+  auto insc = [&](const ItemPos& p) { return a.in_sc + (size_t)p.rhs * g.stride * 2; };
   PathBuf X0, X1, X2;
-  load_path<kOddCh>(a, P, inp(P), 0, q, X0);
-  load_path<kOddCh>(a, P, inp(P), 1, q, X1);
+  load_path<2>(a, P, inp(P), insc(P), 0, q, X0);
+  load_path<2>(a, P, inp(P), insc(P), 1, q, X1);
   for (;;) {
     const int it2 = next_live(a, S, it + S.step, flags);
     const ItemPos P2 = it2 < S.nit ? item_pos(g, net_item(a.sw, S, it2), n, 0) : P;
@@ -530,33 +571,27 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
     const float2* r = (const float2*)a.in_id + (size_t)P.rhs * 12 * g.stride;
-    item9<kOddCh>(
-        a, P, inp(P), q, lane, sA, X0, X1, X2, acc,
-        // the identity input r (12 complex numbers) in the 24 words of X2.v
-        [&](PathBuf& b) {
+    float2 rid[12];  // the identity input r (12 complex numbers)
+    item9<2>(
+        a, P, inp(P), insc(P), q, lane, sA, X0, X1, X2, acc,
+        [&](PathBuf&) {
 #pragma unroll
-          for (int k = 0; k < 12; ++k) {
-            const float2 x = ld_at<float2, kIdCP>(r, (uint32_t)(k * g.stride), (uint32_t)P.s);
-            b.v[(2 * k) / 12][((2 * k) % 12) / 4][(2 * k) % 4] = __float_as_uint(x.x);
-            b.v[(2 * k + 1) / 12][((2 * k + 1) % 12) / 4][(2 * k + 1) % 4] = __float_as_uint(x.y);
-          }
+          for (int k = 0; k < 12; ++k)
+            rid[k] = ld_at<float2, kIdCP>(r, (uint32_t)(k * g.stride), (uint32_t)P.s);
         },
         [&](PathBuf& b, int path) {
-          if (it2 < S.nit) load_path<kOddCh>(a, P2, inp(P2), path, q, b);
+          if (it2 < S.nit) load_path<2>(a, P2, inp(P2), insc(P2), path, q, b);
         },
         // identity: (W2_0 W1_0) r, r in channel 0 (lane group 0) of the B operand
-        [&](const PathBuf& b) {
+        [&](const PathBuf&) {
           bf16x8 Bi[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             U32x4 v;
 #pragma unroll
             for (int sp = 0; sp < 4; ++sp) {
-              const int k = 3 * sp + c;
-              const float re = __uint_as_float(b.v[(2 * k) / 12][((2 * k) % 12) / 4][(2 * k) % 4]);
-              const float im =
-                  __uint_as_float(b.v[(2 * k + 1) / 12][((2 * k + 1) % 12) / 4][(2 * k + 1) % 4]);
-              v[sp] = q == 0 ? pack_b2(re, im) : 0u;
+              const float2 x = rid[3 * sp + c];
+              v[sp] = q == 0 ? pack_b2(x.x, x.y) : 0u;
             }
             Bi[c] = as_frag(v);
           }
@@ -568,9 +603,9 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
               acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Af, Bi[c], acc[c][mb], 0, 0, 0);
           }
         });
-    if (P.valid)
-      store_rec<kEvenCh>((char*)a.out + (size_t)P.rhs * g.stride * (kEvenCh * 48), P.s, 0, q,
-                         acc[0], acc[1], acc[2]);
+    store_half8<1>((char*)a.out + (size_t)P.rhs * g.stride * kHalf8,
+                   a.out_sc + (size_t)P.rhs * g.stride, P.s, 0, q, P.valid, acc[0], acc[1],
+                   acc[2]);
     if (it2 >= S.nit) break;
     it = it2;
     P = P2;
@@ -600,38 +635,41 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
   if (it >= S.nit) return;
   ItemPos P = item_pos(g, net_item(a.sw, S, it), n, 1);
   auto inp = [&](const ItemPos& p) {
-    return (const char*)a.in + (size_t)p.rhs * g.stride * (kEvenCh * 48);
+    return (const char*)a.in + (size_t)p.rhs * g.stride * kHalf8;
   };
+  auto insc = [&](const ItemPos& p) { return a.in_sc + (size_t)p.rhs * g.stride; };
   PathBuf X0, X1, X2;
-  load_path<kEvenCh>(a, P, inp(P), 0, q, X0);
-  load_path<kEvenCh>(a, P, inp(P), 1, q, X1);
+  load_path<1>(a, P, inp(P), insc(P), 0, q, X0);
+  load_path<1>(a, P, inp(P), insc(P), 1, q, X1);
   for (;;) {
     const int it2 = next_live(a, S, it + S.step, flags);
     const ItemPos P2 = it2 < S.nit ? item_pos(g, net_item(a.sw, S, it2), n, 1) : P;
-    char* rec = (char*)a.out + (size_t)P.rhs * g.stride * (kOddCh * 48);
+    const char* rec = (const char*)a.in_id + (size_t)P.rhs * g.stride * (2 * kHalf8);
+    const float* rsc = a.id_sc + (size_t)P.rhs * g.stride * 2;
     f32x4 acc[3][MB];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
     f32x4 y[3][MB];
-    item9<kEvenCh>(
-        a, P, inp(P), q, lane, sA, X0, X1, X2, acc,
-        // the identity input: channels 8..15 (L2(odd)) of the own odd record
+    item9<1>(
+        a, P, inp(P), insc(P), q, lane, sA, X0, X1, X2, acc,
+        // the identity input: half 1 (L2(odd)) of the own odd record
         [&](PathBuf& b) {
+          const uint32_t vo = r8_off<2>((uint32_t)P.s, 1, q);
 #pragma unroll
-          for (int kb = 0; kb < KBI; ++kb) {
-            const uint32_t vo = rec_off<kOddCh>((uint32_t)P.s, 8 + 4 * kb + q);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) b.v[kb][c] = ld128<kIdCP>(rec, rec_colour<kOddCh>(c), vo);
-          }
+          for (int c = 0; c < 3; ++c) b.w[c] = ld128<kIdCP>(rec, 16u * c, vo);
+          b.sc = __uint_as_float(ld32<kIdCP>(rsc, 0u, ((uint32_t)P.s * 2u + 1u) * 4u));
         },
         [&](PathBuf& b, int path) {
-          if (it2 < S.nit) load_path<kEvenCh>(a, P2, inp(P2), path, q, b);
+          if (it2 < S.nit) load_path<1>(a, P2, inp(P2), insc(P2), path, q, b);
         },
         // identity W3_0 L2(odd), then the Y mixes: k-blocks 0, 1 from the L3
         // accumulators (blocks 2 kb, 2 kb + 1), k-blocks 2, 3 from L2(odd)
         [&](const PathBuf& b) {
+          U32x4 v[KBI][3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) dq8(b.w[c], b.sc, v[0][c], v[1][c]);
 #pragma unroll
           for (int kb = 0; kb < KBI; ++kb)
 #pragma unroll
@@ -640,7 +678,7 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
                   as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + mb * KBI + kb) * 64 + lane]));
 #pragma unroll
               for (int c = 0; c < 3; ++c)
-                acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Af, as_frag(b.v[kb][c]),
+                acc[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Af, as_frag(v[kb][c]),
                                                                      acc[c][mb], 0, 0, 0);
             }
           bf16x8 B3[2][3];
@@ -661,7 +699,7 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
 #pragma unroll
               for (int c = 0; c < 3; ++c)
                 y[c][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    Af, kb < 2 ? B3[kb][c] : as_frag(b.v[kb - 2][c]), y[c][mb], 0, 0, 0);
+                    Af, kb < 2 ? B3[kb][c] : as_frag(v[kb - 2][c]), y[c][mb], 0, 0, 0);
             }
         });
     if (P.valid) {
@@ -675,8 +713,8 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           U32x2 wv;
-          wv[0] = pack_st(y[c][mb][0], y[c][mb][1]);
-          wv[1] = pack_st(y[c][mb][2], y[c][mb][3]);
+          wv[0] = pack_b2(y[c][mb][0], y[c][mb][1]);
+          wv[1] = pack_b2(y[c][mb][2], y[c][mb][3]);
           st64(yo, y_colour((uint32_t)g.stride, c), vo, wv);
         }
       }
@@ -713,18 +751,22 @@ __global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
     const SiteCoord sc = decode_site(g, s, 0);
     float zr[3], zi[3];
     {
-      const char* ev = (const char*)a.in_id + (size_t)w.rhs * g.stride * (kEvenCh * 48);
+      const char* ev = (const char*)a.in_id + (size_t)w.rhs * g.stride * kHalf8;
+      const float esc =
+          __uint_as_float(ld32<kIdCP>(a.id_sc + (size_t)w.rhs * g.stride, 0u, (uint32_t)s * 4u));
       f32x4 C[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) C[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+      const uint32_t vo = r8_off<1>((uint32_t)s, 0, q);
 #pragma unroll
-      for (int kb = 0; kb < KBI; ++kb) {
-        const bf16x8 A = as_frag(*reinterpret_cast<const U32x4*>(&sA[kb * 64 + lane]));
-        const uint32_t vo = rec_off<kEvenCh>((uint32_t)s, 4 * kb + q);
+      for (int c = 0; c < 3; ++c) {
+        U32x4 v[KBI];
+        dq8(ld128<kIdCP>(ev, 16u * c, vo), esc, v[0], v[1]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          C[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              A, as_frag(ld128<kIdCP>(ev, rec_colour<kEvenCh>(c), vo)), C[c], 0, 0, 0);
+        for (int kb = 0; kb < KBI; ++kb) {
+          const bf16x8 A = as_frag(*reinterpret_cast<const U32x4*>(&sA[kb * 64 + lane]));
+          C[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, as_frag(v[kb]), C[c], 0, 0, 0);
+        }
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -890,8 +932,10 @@ struct BNet {
   int version = -1;            // weight version the fragments were built from
   uint4* frag = nullptr;
   size_t off[7] = {};          // first, mid, mid_id, third, third_id, third_y, last_id
-  uint32_t* odd = nullptr;     // [rhs][stride][3 colours][16 channels][4 spins] bf16 pairs
-  uint32_t* even = nullptr;    // [rhs][stride][3][8][4]
+  uint8_t* odd = nullptr;      // e4m3 records [rhs][stride][2 halves][192 bytes]
+  uint8_t* even = nullptr;     // [rhs][stride][192 bytes]
+  float* odd_sc = nullptr;     // their scales [rhs][stride][2]
+  float* even_sc = nullptr;    // [rhs][stride]
   uint32_t* ybuf = nullptr;    // [rhs][8 paths][3 colours][stride][4 spins]
   int act_rhs = 0;
   int ncu = 0;                 // compute units
@@ -1020,12 +1064,18 @@ ii_status bnet_acts(ii_context* ctx, BNet* B, int nrhs) {
   II_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (B->odd) (void)hipFree(B->odd);
   if (B->even) (void)hipFree(B->even);
+  if (B->odd_sc) (void)hipFree(B->odd_sc);
+  if (B->even_sc) (void)hipFree(B->even_sc);
   if (B->ybuf) (void)hipFree(B->ybuf);
-  B->odd = B->even = B->ybuf = nullptr;
+  B->odd = B->even = nullptr;
+  B->odd_sc = B->even_sc = nullptr;
+  B->ybuf = nullptr;
   B->act_rhs = 0;
   const size_t st = ctx->g.stride;
-  II_HIP(ctx, hipMalloc(&B->odd, (size_t)nrhs * st * kOddCh * 48));
-  II_HIP(ctx, hipMalloc(&B->even, (size_t)nrhs * st * kEvenCh * 48));
+  II_HIP(ctx, hipMalloc(&B->odd, (size_t)nrhs * st * 2 * kHalf8));
+  II_HIP(ctx, hipMalloc(&B->even, (size_t)nrhs * st * kHalf8));
+  II_HIP(ctx, hipMalloc(&B->odd_sc, (size_t)nrhs * st * 2 * sizeof(float)));
+  II_HIP(ctx, hipMalloc(&B->even_sc, (size_t)nrhs * st * sizeof(float)));
   II_HIP(ctx, hipMalloc(&B->ybuf, (size_t)nrhs * kHop * st * 48));
   B->act_rhs = nrhs;
   return II_OK;
@@ -1052,6 +1102,7 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, int by, int 
       a.links = links + lpar;
       a.in = r;
       a.out = B->odd;
+      a.out_sc = B->odd_sc;
       hipLaunchKernelGGL(k_bnet_first, dim3(nblk), dim3(kThreadsFirst), 0, ctx->stream, a);
       break;
     case 1:  // even: L2(even)
@@ -1059,8 +1110,10 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, int by, int 
       a.frag_id = B->frag + B->off[2];
       a.links = links;
       a.in = B->odd;
+      a.in_sc = B->odd_sc;
       a.in_id = r;
       a.out = B->even;
+      a.out_sc = B->even_sc;
       hipLaunchKernelGGL(k_bnet_mid, dim3(nblk), dim3(kThreadsMid), 0, ctx->stream, a);
       break;
     case 2:  // odd: L3, then Y over L1's channels of the odd records
@@ -1069,7 +1122,9 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, int by, int 
       a.frag_y = B->frag + B->off[5];
       a.links = links + lpar;
       a.in = B->even;
-      a.out = B->odd;
+      a.in_sc = B->even_sc;
+      a.in_id = B->odd;
+      a.id_sc = B->odd_sc;
       a.yout = B->ybuf;
       hipLaunchKernelGGL(k_bnet_third, dim3(nblk), dim3(kThreadsThird), 0, ctx->stream, a);
       break;
@@ -1078,6 +1133,7 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, int by, int 
       a.links = links;
       a.in = B->ybuf;
       a.in_id = B->even;
+      a.id_sc = B->even_sc;
       a.out = z;
       hipLaunchKernelGGL(k_bnet_last, dim3(nblk), dim3(kThreadsLast), 0, ctx->stream, a);
       break;
@@ -1198,6 +1254,8 @@ void bnet_release(ii_context* ctx) {
   if (B->frag) (void)hipFree(B->frag);
   if (B->odd) (void)hipFree(B->odd);
   if (B->even) (void)hipFree(B->even);
+  if (B->odd_sc) (void)hipFree(B->odd_sc);
+  if (B->even_sc) (void)hipFree(B->even_sc);
   if (B->ybuf) (void)hipFree(B->ybuf);
   delete B;
   ctx->bnet = nullptr;

@@ -258,6 +258,35 @@ def bf16_round(a):
     return b.view(np.float32).astype(np.float64)
 
 
+def e4m3_round(a):
+    """Round to the nearest OCP e4m3 (fp8) value, ties to even, for
+    magnitudes below 448: 3 mantissa bits for |a| >= 2^-6, a fixed spacing of
+    2^-9 below (the subnormal range)."""
+    a = np.asarray(a, dtype=np.float64)
+    m = np.abs(a)
+    e = np.floor(np.log2(np.where(m > 0, m, 1.0)))
+    q = np.where(m >= 2.0 ** -6, 2.0 ** (e - 3), 2.0 ** -9)
+    return np.rint(a / q) * q
+
+
+def fp8_record_round(x, nch, sites):
+    """The e4m3 storage of activation records: x holds nch channels of a full
+    field ([channel][site][24] doubles); at every site of `sites`, the 8-channel
+    groups (channels 8h .. 8h + 7) are scaled by 2^(8 - e), e the exponent of
+    their largest magnitude (frexp), rounded to e4m3 and scaled back, so the
+    largest magnitude of a group lies in [128, 256) before rounding.  A group
+    whose largest magnitude is below 2^-111 keeps scale 1 (its values round
+    to zero)."""
+    y = np.array(x, dtype=np.float64).reshape(nch, -1, 24)
+    for h in range(nch // 8):
+        blk = y[8 * h:8 * h + 8][:, sites, :]
+        mx = np.abs(blk).max(axis=(0, 2))
+        _, e = np.frexp(mx)
+        ds = np.where(mx >= 2.0 ** -111, np.ldexp(1.0, e - 8), 1.0)
+        y[8 * h:8 * h + 8][:, sites, :] = e4m3_round(blk / ds[None, :, None]) * ds[None, :, None]
+    return y.ravel()
+
+
 class Net:
     """Composition of gauge-covariant layers (linear), widths e.g. (1, 8, 8, 8, 1).
     The preconditioner acts on an even half field b: it is placed on the even
@@ -279,6 +308,28 @@ class Net:
 
     def apply(self, b):
         return self.lat.split(self.forward(b)[-1])[0]
+
+    def apply_fp8_records(self, b):
+        """The network of widths (1, 8, 8, 8, 1) as the device's batched
+        evaluation stores it: L1 and L2 on odd sites and L2 on even sites
+        rounded to e4m3 records (fp8_record_round, one scale per site and
+        8-channel group), everything else exact.  L2(odd) comes from the
+        unrounded L1 (the device forms it straight from the input), L2(even)
+        from the rounded L1(odd) and the exact L1(even) = W1_0 b."""
+        lat = self.lat
+        e, o = lat.eo_tables()
+        x0 = self.embed(b)
+        l1 = lat.net_layer(self.u, self.w[0], x0, self.rb)
+        l2 = lat.net_layer(self.u, self.w[1], l1, self.rb).reshape(8, lat.V, 24)
+        l1q = fp8_record_round(l1, 8, o)
+        l2e = lat.net_layer(self.u, self.w[1], l1q, self.rb).reshape(8, lat.V, 24)
+        l2q = np.empty_like(l2)
+        l2q[:, e, :] = l2e[:, e, :]
+        l2q[:, o, :] = l2[:, o, :]
+        l2q = fp8_record_round(fp8_record_round(l2q.ravel(), 8, o), 8, e)
+        l3 = lat.net_layer(self.u, self.w[2], l2q, self.rb)
+        z = lat.net_layer(self.u, self.w[3], l3, self.rb)
+        return lat.split(z)[0]
 
     def loss_and_grad(self, pairs):
         loss = 0.0

@@ -222,17 +222,21 @@ def test_training_reduces_loss_and_iterations(tmp_path):
 @pytest.mark.parametrize("dims,nrhs", [((4, 4, 4, 8), 5), ((6, 2, 2, 6), 3), ((8, 8, 32, 32), 3)])
 def test_batched_apply_vs_oracle(dims, nrhs):
     """ii_net_apply_batched (all rhs per launch; layers regrouped by parity
-    with composite weights, bfloat16 activations and links) against the FP64
-    oracle network on every rhs, on a lattice whose half volume is a whole
-    number of 16-site tiles and on one that is not (6x2x2x6: 72 sites), and
-    on 8x8x32x32, where on a 256-CU device the blocks own several z-planes
-    each (taken z fastest, regions grouped per XCD), with the autotuned rhs
-    grouping and three forced ones (bit-identical).
-    Bound 3e-2: bf16 rounding of the activations of four stages, of the
-    transport matrices and of the composite weights (8 significant bits
-    each).  A network with one transport path of one layer dropped is more
-    than three bounds away, so the bound separates the right network from a
-    wrong one."""
+    with composite weights, e4m3 activation records with one power-of-two
+    scale per site and 8-channel half, bfloat16 transports, links and Y)
+    against two oracles on every rhs: Net.apply_fp8_records, which rounds the
+    same three records to e4m3 and keeps the rest exact, and the exact FP64
+    network; on a lattice whose half volume is a whole number of 16-site
+    tiles and on one that is not (6x2x2x6: 72 sites), and on 8x8x32x32, where
+    on a 256-CU device the blocks own several z-planes each, with the
+    autotuned rhs grouping and three forced ones (bit-identical).
+    Bounds: 2.5e-2 against the fp8-record oracle (what is left is bf16
+    rounding of the transported vectors, the links, the composite weights and
+    Y, 8 significant bits each, and the e4m3 rounding steps this moves);
+    4e-2 against the exact network (the e4m3 records themselves, 4
+    significant bits, add about 2.6e-2).  A network with one transport path
+    of one layer dropped is more than four bounds from the fp8-record oracle,
+    so the bound separates the right network from a wrong one."""
     lat = orc.Lattice(dims)
     u = lat.apbc(lat.gauge_expH(0.4, 77))
     ctx = ii.Context(dims)
@@ -253,11 +257,14 @@ def test_batched_apply_vs_oracle(dims, nrhs):
         wrong[1][8] = 0.0
         bad = orc.Net(lat, u, wrong)
         want = [ref.apply(b[k]) for k in range(nrhs)]
+        want8 = [ref.apply_fp8_records(b[k]) for k in range(nrhs)]
         for k in range(nrhs):
-            e = rel(got[k], want[k])
-            assert e < 3e-2, (k, e)
-            assert rel(got[k], single[k]) < 3e-2
-            assert rel(bad.apply(b[k]), want[k]) > 1e-1
+            e8, e = rel(got[k], want8[k]), rel(got[k], want[k])
+            print("rhs %d: vs fp8-record oracle %.4f, vs exact %.4f" % (k, e8, e))
+            assert e8 < 2.5e-2, (k, e8)
+            assert e < 4e-2, (k, e)
+            assert rel(got[k], single[k]) < 4e-2
+            assert rel(bad.apply_fp8_records(b[k]), want8[k]) > 1e-1
         # every grouping of the rhs gives the same result (the work order
         # changes, not the arithmetic of a site)
         for force in [(1, 1, 1, 1), (2, 3, 4, 12), (12, 6, 1, 2)]:

@@ -140,3 +140,87 @@ def test_bf16_rounding_known_values_and_c_agrees(lat, gauge):
     w[0, 0, 0] = np.eye(4)
     x = np.random.default_rng(5).normal(size=lat.full_size)
     assert np.array_equal(lat.net_layer(gauge, w, x, round_bf16=True), orc.bf16_round(x))
+
+
+def _e4m3_table():
+    """The non-negative finite OCP e4m3 values below 448 from the bit layout
+    (sign, 4 exponent bits with bias 7, 3 mantissa bits; exponent 0 is
+    subnormal), in code order."""
+    out = []
+    for code in range(0x7F):
+        e, m = code >> 3, code & 7
+        out.append(m * 2.0 ** -9 if e == 0 else (1 + m / 8) * 2.0 ** (e - 7))
+    return np.array(out)
+
+
+def test_e4m3_round_grid_ties_and_known_values():
+    t = _e4m3_table()
+    assert np.all(np.diff(t) > 0) and t[-1] == 448.0
+    assert np.array_equal(orc.e4m3_round(t), t)
+    assert np.array_equal(orc.e4m3_round(-t), -t)
+    mid = (t[1:] + t[:-1]) / 2
+    codes = np.searchsorted(t, orc.e4m3_round(mid))
+    assert np.all(codes % 2 == 0)  # ties go to the even mantissa
+    nudge = 1e-9 * t[1:]
+    assert np.array_equal(orc.e4m3_round(mid + nudge), t[1:])
+    assert np.array_equal(orc.e4m3_round(mid - nudge), t[:-1])
+    known = {1.0625: 1.0, 1.1875: 1.25, 0.3: 0.3125, 2.5 * 2.0 ** -9: 2.0 ** -8,
+             3.5 * 2.0 ** -9: 2.0 ** -7, 200.0: 192.0, -13.0: -13.0}
+    for x, want in known.items():
+        assert orc.e4m3_round(x) == want, x
+
+
+def test_fp8_record_round_scale_error_and_idempotence(lat):
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=16 * lat.full_size) * np.exp(rng.normal(size=16 * lat.full_size))
+    e, o = lat.eo_tables()
+    y = orc.fp8_record_round(x, 16, o)
+    xr, yr = x.reshape(16, lat.V, 24), y.reshape(16, lat.V, 24)
+    assert np.array_equal(yr[:, e, :], xr[:, e, :])  # other sites untouched
+    # rounding again changes only groups whose largest value rounded up to
+    # 256 x scale (their scale doubles)
+    y2 = orc.fp8_record_round(y, 16, o).reshape(16, lat.V, 24)
+    for h in range(2):
+        grp, grp2 = yr[8 * h:8 * h + 8][:, o, :], y2[8 * h:8 * h + 8][:, o, :]
+        mx = np.abs(xr[8 * h:8 * h + 8][:, o, :]).max(axis=(0, 2))
+        ds = np.ldexp(1.0, np.frexp(mx)[1] - 8)
+        same_scale = np.abs(grp).max(axis=(0, 2)) < 256 * ds
+        assert same_scale.mean() > 0.5
+        assert np.array_equal(grp2[:, same_scale, :], grp[:, same_scale, :])
+    for h in range(2):
+        blk, got = xr[8 * h:8 * h + 8][:, o, :], yr[8 * h:8 * h + 8][:, o, :]
+        mx = np.abs(blk).max(axis=(0, 2))
+        ds = np.ldexp(1.0, np.frexp(mx)[1] - 8)
+        assert np.all((mx / ds >= 128) & (mx / ds < 256))
+        # normal range: half a unit in the 4th significant bit; below it the
+        # subnormal spacing 2^-9 ds
+        scaled = np.abs(blk) / ds[None, :, None]
+        err = np.abs(got - blk) / ds[None, :, None]
+        normal = scaled >= 2.0 ** -6
+        assert np.all(err[normal] <= 2.0 ** -4 * scaled[normal] + 1e-12)
+        assert np.all(err[~normal] <= 2.0 ** -10 + 1e-15)
+        # the scale is per 8-channel group: a group 2^20 times smaller keeps
+        # its relative precision
+        z = x.copy().reshape(16, lat.V, 24)
+        z[8:] *= 2.0 ** -20
+        zq = orc.fp8_record_round(z.ravel(), 16, o).reshape(16, lat.V, 24)
+        lo, loq = z[8:][:, o, :], zq[8:][:, o, :]
+        assert np.linalg.norm(loq - lo) / np.linalg.norm(lo) < 5e-2
+
+
+def test_apply_fp8_records_is_the_network_when_rounding_is_exact(lat, gauge, monkeypatch):
+    """With the record rounding replaced by the identity, the regrouped data
+    flow of apply_fp8_records (L2(odd) from the unrounded L1, L2(even) from
+    the rounded L1(odd), L3 and z from the rounded L2) is the network; with
+    the rounding on it differs from it by the e4m3 rounding, a few percent."""
+    rng = np.random.default_rng(2)
+    ws = [rand_w(rng, 8, 1), rand_w(rng, 8, 8, 0.1), rand_w(rng, 8, 8, 0.1), rand_w(rng, 1, 8, 0.1)]
+    net = orc.Net(lat, gauge, ws)
+    b = rng.normal(size=lat.half_size)
+    exact = net.apply(b)
+    rounded = net.apply_fp8_records(b)
+    d = np.linalg.norm(rounded - exact) / np.linalg.norm(exact)
+    assert 3e-3 < d < 6e-2, d
+    monkeypatch.setattr(orc, "fp8_record_round", lambda x, nch, sites: np.array(x, dtype=np.float64))
+    same = net.apply_fp8_records(b)
+    assert np.linalg.norm(same - exact) <= 1e-12 * np.linalg.norm(exact)
