@@ -99,12 +99,11 @@ template <int CP = 0>
 __device__ __forceinline__ uint32_t ld32(const void* base, uint32_t soff, uint32_t voff) {
   return __builtin_amdgcn_raw_buffer_load_b32(rsrc(base), voff, soff, CP);
 }
-// Activation records are written in 8-byte pieces (a lane group writes 32
-// contiguous bytes); they go through L2 (normal policy) so that the pieces
-// of a line merge there.  Streaming them (kNT) made the stages 1.2-3x slower.
-__device__ __forceinline__ void st64(void* base, uint32_t soff, uint32_t voff, U32x2 v) {
-  __builtin_amdgcn_raw_buffer_store_b64(v, rsrc(base), voff, soff, kStCP);
+__device__ __forceinline__ U32x2 ld64(const void* base, uint32_t soff, uint32_t voff) {
+  return __builtin_amdgcn_raw_buffer_load_b64(rsrc(base), voff, soff, 0);
 }
+// Records go through L2 with the normal policy: streaming them (kNT) made
+// the stages 1.2-3x slower.
 __device__ __forceinline__ void st128(void* base, uint32_t soff, uint32_t voff, U32x4 v) {
   __builtin_amdgcn_raw_buffer_store_b128(v, rsrc(base), voff, soff, kStCP);
 }
@@ -125,15 +124,11 @@ template <int H>
 __device__ __forceinline__ uint32_t r8_off(uint32_t s, int h, int q) {
   return s * (kHalf8 * H) + kHalf8 * (uint32_t)h + 48u * (uint32_t)q;
 }
-// The third stage's Y_d: [rhs][path][colour][site] with 16 bytes per site
-// (spins 0..3), so a tile's 16 sites of one path and colour are one
-// contiguous run.
-__device__ __forceinline__ uint32_t y_off(uint32_t stride, int path, uint32_t s) {
-  return ((uint32_t)path * 3u * stride + s) * 16u;
-}
-__device__ __forceinline__ uint32_t y_colour(uint32_t stride, int c) {
-  return (uint32_t)c * stride * 16u;
-}
+// The third stage's Y_d (one channel per path) are e4m3 too, with one scale
+// per site and rhs over its 8 paths: [rhs][path][site][spin][colour][re/im]
+// with 2 pad bytes after each spin, 32 bytes per site and path, so the last
+// stage's lane group q reads spin q of a path as one 8-byte load.
+constexpr uint32_t kY8 = 32;
 
 __device__ __forceinline__ bf16x8 as_frag(U32x4 v) {
   bf16x8 r;
@@ -266,7 +261,7 @@ struct BArgs {
   const float* id_sc;   // its record scales
   void* out;
   float* out_sc;        // the output record's scales
-  void* yout;           // the third stage's Y_d, [rhs][8 paths][3 colours][stride][4 spins]
+  void* yout;           // the third stage's Y_d (e4m3; scales in out_sc)
 };
 
 __device__ __forceinline__ uint32_t link_off(const Geom& g, int dh) {
@@ -459,19 +454,19 @@ __device__ __forceinline__ void item9(const BArgs& a, const ItemPos& P, const ch
 // products and stores.
 struct RawFirst {
   uint32_t L[2][9];
-  float2 r[2][12];
+  U32x4 r[2][3];  // r at the neighbour, bf16 input layout: colour c, spins 0..3
 };
 __device__ __forceinline__ void load_first(const BArgs& a, const ItemPos& P, int q,
                                            RawFirst& x) {
   const Geom& g = a.g;
-  const float2* r = (const float2*)a.in + (size_t)P.rhs * 12 * g.stride;
+  const uint32_t* r = (const uint32_t*)a.in + (size_t)P.rhs * 12 * g.stride;
 #pragma unroll
   for (int kb = 0; kb < 2; ++kb) {
     const int dh = 4 * kb + q;
-    const int nb = nb_of(g, P.c, P.s, dh);
+    const uint32_t nb = (uint32_t)nb_of(g, P.c, P.s, dh);
     load_link(a.links, 0u, link_off(g, dh) + (uint32_t)P.s * (kLinkWords * 4), x.L[kb]);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) x.r[kb][k] = ld_at(r, (uint32_t)(k * g.stride), (uint32_t)nb);
+    for (int c = 0; c < 3; ++c) x.r[kb][c] = ld128(r, 16u * c, nb * 48u);
   }
 }
 __global__ void __launch_bounds__(kThreadsFirst, 2) k_bnet_first(BArgs a) {
@@ -493,15 +488,7 @@ __global__ void __launch_bounds__(kThreadsFirst, 2) k_bnet_first(BArgs a) {
   for (;;) {
     bf16x8 B[KB][3];
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      U32x4 v[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int sp = 0; sp < 4; ++sp)
-          v[c][sp] = pack_b2(X.r[kb][3 * sp + c].x, X.r[kb][3 * sp + c].y);
-      transport(X.L[kb], v, B[kb]);
-    }
+    for (int kb = 0; kb < KB; ++kb) transport(X.L[kb], X.r[kb], B[kb]);
     const int it2 = next_live(a, S, it + S.step, flags);
     ItemPos P2 = P;
     if (it2 < S.nit) {
@@ -570,14 +557,13 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const float2* r = (const float2*)a.in_id + (size_t)P.rhs * 12 * g.stride;
-    float2 rid[12];  // the identity input r (12 complex numbers)
+    const uint32_t* r = (const uint32_t*)a.in_id + (size_t)P.rhs * 12 * g.stride;
+    U32x4 rid[3];  // the identity input r (bf16 input layout)
     item9<2>(
         a, P, inp(P), insc(P), q, lane, sA, X0, X1, X2, acc,
         [&](PathBuf&) {
 #pragma unroll
-          for (int k = 0; k < 12; ++k)
-            rid[k] = ld_at<float2, kIdCP>(r, (uint32_t)(k * g.stride), (uint32_t)P.s);
+          for (int c = 0; c < 3; ++c) rid[c] = ld128<kIdCP>(r, 16u * c, (uint32_t)P.s * 48u);
         },
         [&](PathBuf& b, int path) {
           if (it2 < S.nit) load_path<2>(a, P2, inp(P2), insc(P2), path, q, b);
@@ -586,15 +572,7 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
         [&](const PathBuf&) {
           bf16x8 Bi[3];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            U32x4 v;
-#pragma unroll
-            for (int sp = 0; sp < 4; ++sp) {
-              const float2 x = rid[3 * sp + c];
-              v[sp] = q == 0 ? pack_b2(x.x, x.y) : 0u;
-            }
-            Bi[c] = as_frag(v);
-          }
+          for (int c = 0; c < 3; ++c) Bi[c] = as_frag(q == 0 ? rid[c] : U32x4{0u, 0u, 0u, 0u});
 #pragma unroll
           for (int mb = 0; mb < MB; ++mb) {
             const bf16x8 Af = as_frag(*reinterpret_cast<const U32x4*>(&sA[(NH + mb) * 64 + lane]));
@@ -702,21 +680,35 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
                     Af, kb < 2 ? B3[kb][c] : as_frag(v[kb - 2][c]), y[c][mb], 0, 0, 0);
             }
         });
-    if (P.valid) {
-      // Y_d in colour planes [path][colour][site]: C row 4q + j of block mb
-      // is path 2 mb + q/2, spin 2 (q & 1) + j/2, re/im j & 1
-      char* yo = (char*)a.yout + (size_t)P.rhs * kHop * g.stride * 48;
+    {
+      // Y_d: C row 4q + j of block mb is path 2 mb + q/2, spin 2 (q & 1) + j/2,
+      // re/im j & 1; lane group q writes spins 2 (q & 1), 2 (q & 1) + 1 of
+      // that path (16 bytes).  The site's scale spans its 8 paths.
+      float mx = 0.0f;
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        const int path = 2 * mb + (q >> 1);
-        const uint32_t vo = y_off((uint32_t)g.stride, path, (uint32_t)P.s) + 8u * (q & 1);
+      for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          U32x2 wv;
-          wv[0] = pack_b2(y[c][mb][0], y[c][mb][1]);
-          wv[1] = pack_b2(y[c][mb][2], y[c][mb][3]);
-          st64(yo, y_colour((uint32_t)g.stride, c), vo, wv);
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) mx = fmaxf(mx, fabsf(y[c][mb][j]));
+      mx = fmaxf(mx, __shfl_xor(mx, 16));
+      mx = fmaxf(mx, __shfl_xor(mx, 32));
+      float ds, qm;
+      scale8(mx, ds, qm);
+      if (P.valid) {
+        char* yo = (char*)a.yout + (size_t)P.rhs * kHop * g.stride * kY8;
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+          const uint32_t path = 2u * mb + (q >> 1);
+          const uint32_t vo = (path * (uint32_t)g.stride + (uint32_t)P.s) * kY8 + 16u * (q & 1);
+          const f32x4 a0{y[0][mb][0], y[0][mb][1], y[1][mb][0], y[1][mb][1]};
+          const f32x4 a1{y[2][mb][0], y[2][mb][1], 0.0f, 0.0f};
+          const f32x4 a2{y[0][mb][2], y[0][mb][3], y[1][mb][2], y[1][mb][3]};
+          const f32x4 a3{y[2][mb][2], y[2][mb][3], 0.0f, 0.0f};
+          st128(yo, 0u, vo, U32x4{q8(a0, qm), q8(a1, qm), q8(a2, qm), q8(a3, qm)});
         }
+        if (q == 0)
+          st32(a.out_sc + (size_t)P.rhs * g.stride, 0u, (uint32_t)P.s * 4u, __float_as_uint(ds));
       }
     }
     if (it2 >= S.nit) break;
@@ -774,18 +766,19 @@ __global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
         zi[c] = C[c][1];
       }
     }
-    const char* yb = (const char*)a.in + (size_t)w.rhs * kHop * g.stride * 48;
+    const char* yb = (const char*)a.in + (size_t)w.rhs * kHop * g.stride * kY8;
+    const float* ysc = a.in_sc + (size_t)w.rhs * g.stride;
 #pragma unroll 2
     for (int dh = 0; dh < kHop; ++dh) {
       const int nb = neighbour(g, sc, s, dh >> 1, (dh & 1) == 0);
       uint32_t P[9], y[3];
       load_link(a.links, link_off(g, dh), (uint32_t)s * (kLinkWords * 4), P);
-      // Y_d at the neighbour, colour plane b: the 16 sites of a tile read
-      // one contiguous run of a path's plane
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-        y[b] = ld32(yb, y_off((uint32_t)g.stride, dh, 0u) + y_colour((uint32_t)g.stride, b),
-                    (uint32_t)nb * 16u + 4u * q);
+      // spin q of Y_d at the neighbour (colours 0..2) and the neighbour's scale
+      const U32x2 yw = ld64(yb, 0u, ((uint32_t)dh * (uint32_t)g.stride + (uint32_t)nb) * kY8 + 8u * q);
+      const float ys = __uint_as_float(ld32(ysc, 0u, (uint32_t)nb * 4u));
+      y[0] = dq2<false>(yw[0], ys);
+      y[1] = dq2<true>(yw[0], ys);
+      y[2] = dq2<false>(yw[1], ys);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -801,6 +794,36 @@ __global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
         st_at(o, (uint32_t)((3 * q + c) * g.stride), (uint32_t)s, make_float2(zr[c], zi[c]));
     }
   }
+}
+
+// Dense pseudo-random input in (-1, 1) for the stage timings (autotuner and
+// benchmark): the stages run 15-20% faster on zero fields than on the dense
+// residuals of a solve, so they are timed on dense data.
+// (n words of the bf16 input layout, each two values)
+__device__ __forceinline__ float fill_value(uint32_t i, uint32_t seed) {
+  uint32_t h = i * 0x9E3779B1u ^ seed;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return (float)(int32_t)h * 4.656612873e-10f;
+}
+__global__ void __launch_bounds__(kBlock) k_bnet_fill(uint32_t* __restrict__ p, size_t n,
+                                                      uint32_t seed) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = pack_b2(fill_value((uint32_t)(2 * i), seed), fill_value((uint32_t)(2 * i + 1), seed));
+}
+// FP32 half fields [rhs][12][stride] -> the bf16 input layout
+__global__ void __launch_bounds__(kBlock) k_bnet_pack_in(Geom g, const float2* __restrict__ r,
+                                                         uint32_t* __restrict__ out) {
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x, rhs = blockIdx.y;
+  if (hs >= g.Vh) return;
+  float2 v[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = r[((size_t)rhs * 12 + k) * g.stride + hs];
+  net_in_store(out, (size_t)rhs * g.stride + hs, v);
 }
 
 // ------------------------------------------------------------- host side
@@ -936,7 +959,8 @@ struct BNet {
   uint8_t* even = nullptr;     // [rhs][stride][192 bytes]
   float* odd_sc = nullptr;     // their scales [rhs][stride][2]
   float* even_sc = nullptr;    // [rhs][stride]
-  uint32_t* ybuf = nullptr;    // [rhs][8 paths][3 colours][stride][4 spins]
+  uint8_t* ybuf = nullptr;     // e4m3 Y_d [rhs][8 paths][stride][32 bytes]
+  float* ysc = nullptr;        // their scales [rhs][stride]
   int act_rhs = 0;
   int ncu = 0;                 // compute units
   int rg[4] = {0, 0, 0, 0};    // rhs per group of each stage (0: not chosen)
@@ -1067,23 +1091,26 @@ ii_status bnet_acts(ii_context* ctx, BNet* B, int nrhs) {
   if (B->odd_sc) (void)hipFree(B->odd_sc);
   if (B->even_sc) (void)hipFree(B->even_sc);
   if (B->ybuf) (void)hipFree(B->ybuf);
+  if (B->ysc) (void)hipFree(B->ysc);
   B->odd = B->even = nullptr;
   B->odd_sc = B->even_sc = nullptr;
   B->ybuf = nullptr;
+  B->ysc = nullptr;
   B->act_rhs = 0;
   const size_t st = ctx->g.stride;
   II_HIP(ctx, hipMalloc(&B->odd, (size_t)nrhs * st * 2 * kHalf8));
   II_HIP(ctx, hipMalloc(&B->even, (size_t)nrhs * st * kHalf8));
   II_HIP(ctx, hipMalloc(&B->odd_sc, (size_t)nrhs * st * 2 * sizeof(float)));
   II_HIP(ctx, hipMalloc(&B->even_sc, (size_t)nrhs * st * sizeof(float)));
-  II_HIP(ctx, hipMalloc(&B->ybuf, (size_t)nrhs * kHop * st * 48));
+  II_HIP(ctx, hipMalloc(&B->ybuf, (size_t)nrhs * kHop * st * kY8));
+  II_HIP(ctx, hipMalloc(&B->ysc, (size_t)nrhs * st * sizeof(float)));
   B->act_rhs = nrhs;
   return II_OK;
 }
 
 // launches stage `stage` (0..3) of z = M r with rg rhs per group
 ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, int by, int bz,
-                      const float2* r, float2* z, const int* skip = nullptr,
+                      const uint32_t* r, float2* z, const int* skip = nullptr,
                       int skip_stride = 0) {
   BNet* B = bnet_of(ctx);
   const Geom& g = ctx->g;
@@ -1126,12 +1153,14 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, int by, int 
       a.in_id = B->odd;
       a.id_sc = B->odd_sc;
       a.yout = B->ybuf;
+      a.out_sc = B->ysc;
       hipLaunchKernelGGL(k_bnet_third, dim3(nblk), dim3(kThreadsThird), 0, ctx->stream, a);
       break;
     default:  // even: z
       a.frag_id = B->frag + B->off[6];
       a.links = links;
       a.in = B->ybuf;
+      a.in_sc = B->ysc;
       a.in_id = B->even;
       a.id_sc = B->even_sc;
       a.out = z;
@@ -1144,7 +1173,7 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, int by, int 
 
 // Autotuner of the launch shapes: times every candidate (rhs per group,
 // (y, z) block of the sweep) of each stage (events around 3 launches after
-// one warm-up, on zero fields) and keeps the fastest.  Runs once per
+// one warm-up, on dense pseudo-random input) and keeps the fastest.  Runs once per
 // context and rhs count, before the first application, never inside a
 // stream capture.
 ii_status bnet_tune(ii_context* ctx, BNet* B, int nrhs) {
@@ -1156,7 +1185,8 @@ ii_status bnet_tune(ii_context* ctx, BNet* B, int nrhs) {
   void *pr = nullptr, *pz = nullptr;
   II_CHECK(workspace(ctx, 86, nrhs * he * sizeof(float2), &pr));
   II_CHECK(workspace(ctx, 87, nrhs * he * sizeof(float2), &pz));
-  II_HIP(ctx, hipMemsetAsync(pr, 0, nrhs * he * sizeof(float2), ctx->stream));
+  hipLaunchKernelGGL(k_bnet_fill, dim3(1024), dim3(kBlock), 0, ctx->stream, (uint32_t*)pr,
+                     (size_t)12 * nrhs * g.stride, 12345u);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   II_HIP(ctx, hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -1178,14 +1208,14 @@ ii_status bnet_tune(ii_context* ctx, BNet* B, int nrhs) {
         net_grid(g, B->ncu, kStageBlocksPerCU[stage], kStageThreads[stage], nrhs, rg, sh[0],
                  sh[1] > 0 ? sh[1] : g.L[2], probe, &nb);
         if (sh[1] > 0 && (probe.by != sh[0] || probe.bz != sh[1])) continue;
-        st = bnet_launch(ctx, stage, nrhs, rg, sh[0], sh[1], (const float2*)pr, (float2*)pz);
+        st = bnet_launch(ctx, stage, nrhs, rg, sh[0], sh[1], (const uint32_t*)pr, (float2*)pz);
         if (st != II_OK) break;
         if (hipEventRecord(e0, ctx->stream) != hipSuccess) {
           st = set_error(ctx, II_ERR_DEVICE, "network autotuner: timing failed");
           break;
         }
         for (int i = 0; i < 3 && st == II_OK; ++i)
-          st = bnet_launch(ctx, stage, nrhs, rg, sh[0], sh[1], (const float2*)pr, (float2*)pz);
+          st = bnet_launch(ctx, stage, nrhs, rg, sh[0], sh[1], (const uint32_t*)pr, (float2*)pz);
         if (st != II_OK) break;
         float ms = 0.f;
         if (hipEventRecord(e1, ctx->stream) != hipSuccess ||
@@ -1238,7 +1268,7 @@ ii_status bnet_prepare(ii_context* ctx, int nrhs) {
 }
 
 // z = M r for nrhs FP32 even half fields [rhs][12][stride] on the device
-ii_status bnet_apply(ii_context* ctx, int nrhs, const float2* r, float2* z, const int* skip,
+ii_status bnet_apply(ii_context* ctx, int nrhs, const uint32_t* r, float2* z, const int* skip,
                      int skip_stride) {
   II_CHECK(bnet_prepare(ctx, nrhs));
   BNet* B = bnet_of(ctx);
@@ -1257,6 +1287,7 @@ void bnet_release(ii_context* ctx) {
   if (B->odd_sc) (void)hipFree(B->odd_sc);
   if (B->even_sc) (void)hipFree(B->even_sc);
   if (B->ybuf) (void)hipFree(B->ybuf);
+  if (B->ysc) (void)hipFree(B->ysc);
   delete B;
   ctx->bnet = nullptr;
 }
@@ -1273,10 +1304,11 @@ ii_status ii_net_apply_batched(ii_context* ctx, int nrhs, const double* r_even, 
   II_CHECK(enter(ctx));
   const Geom& g = ctx->g;
   const size_t nh = (size_t)24 * g.Vh, he = half_elems(g);
-  void *p64 = nullptr, *p32 = nullptr, *pz = nullptr;
+  void *p64 = nullptr, *p32 = nullptr, *pz = nullptr, *pin = nullptr;
   II_CHECK(workspace(ctx, 80, nrhs * he * sizeof(double2), &p64));
   II_CHECK(workspace(ctx, 81, nrhs * he * sizeof(float2), &p32));
   II_CHECK(workspace(ctx, 82, nrhs * he * sizeof(float2), &pz));
+  II_CHECK(workspace(ctx, 83, (size_t)nrhs * g.stride * 12 * sizeof(uint32_t), &pin));
   double2* f64 = (double2*)p64;
   for (int k = 0; k < nrhs; ++k) {
     double* d = nullptr;
@@ -1284,7 +1316,10 @@ ii_status ii_net_apply_batched(ii_context* ctx, int nrhs, const double* r_even, 
     II_CHECK(pack_half_spinor(ctx, d, f64 + k * he));
   }
   II_CHECK(convert_64_to_32(ctx, f64, (float2*)p32, nrhs));
-  II_CHECK(bnet_apply(ctx, nrhs, (const float2*)p32, (float2*)pz));
+  hipLaunchKernelGGL(k_bnet_pack_in, dim3(blocks_for(g.Vh), nrhs), dim3(kBlock), 0, ctx->stream, g,
+                     (const float2*)p32, (uint32_t*)pin);
+  II_HIP(ctx, hipGetLastError());
+  II_CHECK(bnet_apply(ctx, nrhs, (const uint32_t*)pin, (float2*)pz));
   II_CHECK(convert_32_to_64(ctx, (const float2*)pz, f64, nrhs));
   for (int k = 0; k < nrhs; ++k) {
     double* d = nullptr;
@@ -1336,14 +1371,16 @@ ii_status ii_bench_net_batched(ii_context* ctx, int nrhs, int nrep, double* stag
   void *pr = nullptr, *pz = nullptr;
   II_CHECK(workspace(ctx, 81, nrhs * he * sizeof(float2), &pr));
   II_CHECK(workspace(ctx, 82, nrhs * he * sizeof(float2), &pz));
-  // a defined input: zeros (the stage times do not depend on the values)
-  II_HIP(ctx, hipMemsetAsync(pr, 0, nrhs * he * sizeof(float2), ctx->stream));
-  II_CHECK(bnet_apply(ctx, nrhs, (const float2*)pr, (float2*)pz));
+  // dense input (zero fields run faster than the residuals of a solve)
+  hipLaunchKernelGGL(k_bnet_fill, dim3(1024), dim3(kBlock), 0, ctx->stream, (uint32_t*)pr,
+                     (size_t)12 * nrhs * ctx->g.stride, 12345u);
+  II_HIP(ctx, hipGetLastError());
+  II_CHECK(bnet_apply(ctx, nrhs, (const uint32_t*)pr, (float2*)pz));
   BNet* B = bnet_of(ctx);
   for (int st = 0; st < 4; ++st) {
     II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     for (int i = 0; i < nrep; ++i)
-      II_CHECK(bnet_launch(ctx, st, nrhs, B->rg[st], B->by[st], B->bz[st], (const float2*)pr,
+      II_CHECK(bnet_launch(ctx, st, nrhs, B->rg[st], B->by[st], B->bz[st], (const uint32_t*)pr,
                            (float2*)pz));
     II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     II_HIP(ctx, hipEventSynchronize(ctx->ev1));

@@ -189,10 +189,11 @@ __global__ void k_g_alpha(const double* __restrict__ part, int nb, int k, GScal*
   }
 }
 
-// r -= alpha_k w_k; partials |r|^2
+// r -= alpha_k w_k; partials |r|^2; with Rb, r also in the network's bf16
+// input layout
 __global__ void __launch_bounds__(kGThreads)
 k_g_rupd(Geom g, const float2* __restrict__ W, size_t slot, int k, float2* __restrict__ R,
-         const GScal* __restrict__ gs, double* __restrict__ part) {
+         uint32_t* __restrict__ Rb, const GScal* __restrict__ gs, double* __restrict__ part) {
   const int rhs = blockIdx.y;
   const GScal& S = gs[rhs];
   if (S.idle) return;
@@ -203,14 +204,18 @@ k_g_rupd(Geom g, const float2* __restrict__ W, size_t slot, int k, float2* __res
   if (hs < g.Vh) {
     const float2* w = W + k * slot + rhs * he + hs;
     float2* r = R + rhs * he + hs;
+    float2 nv[12];
+#pragma unroll
     for (int c = 0; c < 12; ++c) {
       const float2 a = w[(size_t)c * g.stride];
       float2 v = r[(size_t)c * g.stride];
       v.x -= ar * a.x - ai * a.y;
       v.y -= ar * a.y + ai * a.x;
       r[(size_t)c * g.stride] = v;
+      nv[c] = v;
       rr += (double)v.x * v.x + (double)v.y * v.y;
     }
+    if (Rb) net_in_store(Rb, (size_t)rhs * g.stride + hs, nv);
   }
   const double s = block_sum(rr);
   if (threadIdx.x == 0) part[(size_t)rhs * gridDim.x + blockIdx.x] = s;
@@ -355,17 +360,21 @@ __global__ void k_g_init(const double* __restrict__ bb, const double* __restrict
 
 // r (FP32) = rho (FP64) for the rhs not done
 __global__ void __launch_bounds__(kGThreads)
-k_g_to32(Geom g, const double2* __restrict__ a, float2* __restrict__ b,
+k_g_to32(Geom g, const double2* __restrict__ a, float2* __restrict__ b, uint32_t* __restrict__ bb,
          const GScal* __restrict__ gs) {
   const int rhs = blockIdx.y;
   if (gs[rhs].done) return;
   const int hs = blockIdx.x * blockDim.x + threadIdx.x;
   if (hs >= g.Vh) return;
   const size_t he = (size_t)12 * g.stride;
+  float2 nv[12];
+#pragma unroll
   for (int c = 0; c < 12; ++c) {
     const double2 v = a[rhs * he + (size_t)c * g.stride + hs];
-    b[rhs * he + (size_t)c * g.stride + hs] = make_float2((float)v.x, (float)v.y);
+    nv[c] = make_float2((float)v.x, (float)v.y);
+    b[rhs * he + (size_t)c * g.stride + hs] = nv[c];
   }
+  if (bb) net_in_store(bb, (size_t)rhs * g.stride + hs, nv);
 }
 
 // z = r (no preconditioner)
@@ -386,7 +395,7 @@ struct Buf {
   size_t n = 0;
 };
 struct FGState {
-  Buf Z, W, R, T, y, be, bo, bh, rho, t64, part, gs, aux;
+  Buf Z, W, R, Rb, T, y, be, bo, bh, rho, t64, part, gs, aux;
   GScal* host = nullptr;
   int host_n = 0;
 };
@@ -407,7 +416,7 @@ ii_status grow(ii_context* ctx, Buf& b, size_t bytes) {
 void fgcr_release(void* p) {
   FGState* s = static_cast<FGState*>(p);
   if (!s) return;
-  for (Buf* b : {&s->Z, &s->W, &s->R, &s->T, &s->y, &s->be, &s->bo, &s->bh, &s->rho, &s->t64,
+  for (Buf* b : {&s->Z, &s->W, &s->R, &s->Rb, &s->T, &s->y, &s->be, &s->bo, &s->bh, &s->rho, &s->t64,
                  &s->part, &s->gs, &s->aux})
     if (b->p) (void)hipFree(b->p);
   if (s->host) (void)hipHostFree(s->host);
@@ -432,6 +441,7 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
   II_CHECK(grow(ctx, S.Z, m * slot * sizeof(float2)));
   II_CHECK(grow(ctx, S.W, m * slot * sizeof(float2)));
   II_CHECK(grow(ctx, S.R, slot * sizeof(float2)));
+  if (use_net) II_CHECK(grow(ctx, S.Rb, (size_t)nr * g.stride * 12 * sizeof(uint32_t)));
   II_CHECK(grow(ctx, S.T, slot * sizeof(float2)));
   for (Buf* b : {&S.y, &S.be, &S.bo, &S.bh, &S.rho, &S.t64})
     II_CHECK(grow(ctx, *b, nr * he * sizeof(double2)));
@@ -451,6 +461,7 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
   float2* Z = (float2*)S.Z.p;
   float2* W = (float2*)S.W.p;
   float2* R = (float2*)S.R.p;
+  uint32_t* Rb = use_net ? (uint32_t*)S.Rb.p : nullptr;  // r as the network's bf16 input
   float2* T = (float2*)S.T.p;
   double2 *y = (double2*)S.y.p, *be = (double2*)S.be.p, *bo = (double2*)S.bo.p,
           *bh = (double2*)S.bh.p, *rho = (double2*)S.rho.p, *t64 = (double2*)S.t64.p;
@@ -492,7 +503,7 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
   hipLaunchKernelGGL(k_g_init, dim3((nr + 63) / 64), dim3(64), 0, ctx->stream, aux, aux + 2 * nr,
                      tol, nr, gs);
   II_CHECK(zero64(ctx, y, nr));
-  hipLaunchKernelGGL(k_g_to32, sgrid, dim3(kGThreads), 0, ctx->stream, g, bh, R, gs);
+  hipLaunchKernelGGL(k_g_to32, sgrid, dim3(kGThreads), 0, ctx->stream, g, bh, R, Rb, gs);
   II_HIP(ctx, hipGetLastError());
 
   const int* idle = &gs[0].idle;
@@ -504,7 +515,7 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
       float2* zk = Z + k * slot;
       float2* wk = W + k * slot;
       if (use_net) {
-        II_CHECK(bnet_apply(ctx, nr, R, zk, idle, kGInts));
+        II_CHECK(bnet_apply(ctx, nr, Rb, zk, idle, kGInts));
       } else {
         hipLaunchKernelGGL(k_g_copy32, sgrid, dim3(kGThreads), 0, ctx->stream, g, R, zk, gs);
       }
@@ -527,7 +538,7 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
       hipLaunchKernelGGL(k_g_orth, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, R, gs,
                          part);
       hipLaunchKernelGGL(k_g_alpha, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, k, gs);
-      hipLaunchKernelGGL(k_g_rupd, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, R, gs,
+      hipLaunchKernelGGL(k_g_rupd, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, R, Rb, gs,
                          part);
       hipLaunchKernelGGL(k_g_rr, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, gs);
       II_HIP(ctx, hipGetLastError());
@@ -550,7 +561,7 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
     b.partials = part;
     II_CHECK(hop64(ctx, b, t64, rho, y, bh));
     hipLaunchKernelGGL(k_g_rho, dim3(nr), dim3(1024), 0, ctx->stream, part, nbh, gs);
-    hipLaunchKernelGGL(k_g_to32, sgrid, dim3(kGThreads), 0, ctx->stream, g, rho, R, gs);
+    hipLaunchKernelGGL(k_g_to32, sgrid, dim3(kGThreads), 0, ctx->stream, g, rho, R, Rb, gs);
     II_HIP(ctx, hipGetLastError());
     return II_OK;
   };

@@ -152,10 +152,29 @@ void learned_release(ii_context* ctx);
 // complex each, and their version (bumped whenever the weights change;
 // w = nullptr: the version only)
 ii_status net_host_weights(ii_context* ctx, int* version, std::vector<std::vector<float2>>* w);
-// batched network (ii_bnet.hip): z = M r for nrhs FP32 even half fields
-// [rhs][12][stride] on the device; rhs k with skip[k * skip_stride] != 0 is
-// left untouched (skip = nullptr: none)
-ii_status bnet_apply(ii_context* ctx, int nrhs, const float2* r, float2* z,
+// The batched network's input r: bf16 (re, im) pairs, [rhs][stride][colour]
+// [spin], 12 words (48 bytes) per site, one 16-byte matrix-core operand per
+// colour.  Written by the producers of r next to their FP32 copy: v[k] is
+// component k = 3 spin + colour of the site (round to nearest even).
+__device__ __forceinline__ uint32_t net_in_word(float2 v) {
+  const __bf16 re = (__bf16)v.x, im = (__bf16)v.y;
+  uint16_t a, b;
+  __builtin_memcpy(&a, &re, 2);
+  __builtin_memcpy(&b, &im, 2);
+  return (uint32_t)a | ((uint32_t)b << 16);
+}
+__device__ __forceinline__ void net_in_store(uint32_t* __restrict__ p, size_t site,
+                                             const float2 v[12]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    reinterpret_cast<uint4*>(p + site * 12)[c] =
+        make_uint4(net_in_word(v[c]), net_in_word(v[3 + c]), net_in_word(v[6 + c]),
+                   net_in_word(v[9 + c]));
+}
+// batched network (ii_bnet.hip): z = M r for nrhs even half fields, r in the
+// bf16 input layout above, z FP32 [rhs][12][stride]; rhs k with
+// skip[k * skip_stride] != 0 is left untouched (skip = nullptr: none)
+ii_status bnet_apply(ii_context* ctx, int nrhs, const uint32_t* r, float2* z,
                      const int* skip = nullptr, int skip_stride = 0);
 // builds what bnet_apply needs (links, weights, buffers) so that its
 // launches can be captured in a graph

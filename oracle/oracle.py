@@ -311,24 +311,104 @@ class Net:
 
     def apply_fp8_records(self, b):
         """The network of widths (1, 8, 8, 8, 1) as the device's batched
-        evaluation stores it: L1 and L2 on odd sites and L2 on even sites
-        rounded to e4m3 records (fp8_record_round, one scale per site and
-        8-channel group), everything else exact.  L2(odd) comes from the
-        unrounded L1 (the device forms it straight from the input), L2(even)
-        from the rounded L1(odd) and the exact L1(even) = W1_0 b."""
-        lat = self.lat
+        evaluation stores it: L1 and L2 on odd sites, L2 on even sites and the
+        one-channel mixes Y_d = W4_d L3 + (W4_0 W3_d) L2 (d = 1..8, odd sites;
+        z = (W4_0 W3_0) L2 + sum_d T_d Y_d on even sites) rounded to e4m3
+        records (fp8_record_round, one scale per site and group of 8 channels
+        or 8 paths), everything else exact.  L2(odd) comes from the unrounded
+        L1 (the device forms it straight from the input), L2(even) from the
+        rounded L1(odd) and the exact L1(even) = W1_0 b."""
+        lat, w = self.lat, self.w
         e, o = lat.eo_tables()
         x0 = self.embed(b)
-        l1 = lat.net_layer(self.u, self.w[0], x0, self.rb)
-        l2 = lat.net_layer(self.u, self.w[1], l1, self.rb).reshape(8, lat.V, 24)
+        l1 = lat.net_layer(self.u, w[0], x0, self.rb)
+        l2 = lat.net_layer(self.u, w[1], l1, self.rb).reshape(8, lat.V, 24)
         l1q = fp8_record_round(l1, 8, o)
-        l2e = lat.net_layer(self.u, self.w[1], l1q, self.rb).reshape(8, lat.V, 24)
+        l2e = lat.net_layer(self.u, w[1], l1q, self.rb).reshape(8, lat.V, 24)
         l2q = np.empty_like(l2)
         l2q[:, e, :] = l2e[:, e, :]
         l2q[:, o, :] = l2[:, o, :]
         l2q = fp8_record_round(fp8_record_round(l2q.ravel(), 8, o), 8, e)
-        l3 = lat.net_layer(self.u, self.w[2], l2q, self.rb)
-        z = lat.net_layer(self.u, self.w[3], l3, self.rb)
+        l3 = lat.net_layer(self.u, w[2], l2q, self.rb)
+        w3, w4 = np.asarray(w[2]), np.asarray(w[3])
+        zero = np.zeros((9, 8, 8, 4, 4), complex)
+        wa, wb = zero.copy(), zero.copy()
+        for d in range(1, 9):
+            wa[0, d - 1] = w4[d, 0]
+            wb[0, d - 1] = np.einsum("kst,kitu->isu", w4[0, 0], w3[d])
+        y = (lat.net_layer(self.u, wa, l3, self.rb) + lat.net_layer(self.u, wb, l2q, self.rb))
+        yq = fp8_record_round(y, 8, o)
+        wi = np.zeros((9, 1, 8, 4, 4), complex)
+        wi[0, 0] = np.einsum("kst,kitu->isu", w4[0, 0], w3[0])
+        wt = np.zeros((9, 1, 8, 4, 4), complex)
+        for d in range(1, 9):
+            wt[d, 0, d - 1] = np.eye(4)
+        z = lat.net_layer(self.u, wi, l2q, self.rb) + lat.net_layer(self.u, wt, yq, self.rb)
+        return lat.split(z)[0]
+
+    def apply_device_form(self, b, rounding=True):
+        """The batched device evaluation of the network of widths
+        (1, 8, 8, 8, 1) step by step, as ii_bnet.hip computes it, with its
+        roundings: the input r, the links and the composite weights rounded
+        to bf16 (real and imaginary parts); every transported vector of the
+        first three stages rounded to bf16 (the matrix-core operand); L3
+        rounded to bf16 before the Y mixes; the records [L1 | L2](odd),
+        L2(even) and Y rounded to e4m3 (fp8_record_round).  Sums are exact
+        here and FP32 on the device; nothing else differs.
+          [L1 | L2](odd) = sum_{d>0} C1_d T_d r,  C1_d = [W1_d ; W2_0 W1_d + W2_d W1_0]
+          L2(even) = (W2_0 W1_0) r + sum_{d>0} W2_d T_d L1(odd)
+          L3(odd)  = W3_0 L2(odd) + sum_{d>0} W3_d T_d L2(even)
+          Y_d      = W4_d L3(odd) + (W4_0 W3_d) L2(odd)
+          z(even)  = (W4_0 W3_0) L2(even) + sum_{d>0} T_d Y_d
+        With rounding=False every rounding is left out (the same regrouped
+        evaluation in exact arithmetic)."""
+        lat = self.lat
+        e, o = lat.eo_tables()
+        b16 = bf16_round if rounding else (lambda a: np.array(a, dtype=np.float64))
+        f8 = fp8_record_round if rounding else (lambda x, nch, sites: np.array(x, dtype=np.float64))
+        ub = b16(self.u)
+        W1, W2, W3, W4 = (np.asarray(w, dtype=np.complex128) for w in self.w)
+
+        def rb(w):
+            return b16(w.real) + 1j * b16(w.imag)
+
+        def comp(a, c):  # (a c)[o][i] = sum_k a[o][k] @ c[k][i]
+            return np.einsum("okst,kitu->oisu", a, c)
+
+        def layer(w, x, rnd):
+            return lat.net_layer(ub, rb(w), x, rnd and rounding)
+
+        x0 = self.embed(b16(b))
+        c1 = np.zeros((9, 16, 1, 4, 4), complex)
+        for d in range(1, 9):
+            c1[d, :8] = W1[d]
+            c1[d, 8:] = comp(W2[0], W1[d]) + comp(W2[d], W1[0])
+        s1 = f8(layer(c1, x0, True), 16, o).reshape(16, lat.V, 24)
+        c2 = np.zeros((9, 8, 8, 4, 4), complex)
+        c2[1:] = W2[1:]
+        i2 = np.zeros((9, 8, 1, 4, 4), complex)
+        i2[0] = comp(W2[0], W1[0])
+        l2e = layer(c2, s1[:8].ravel(), True) + layer(i2, x0, False)
+        l2q = l2e.reshape(8, lat.V, 24).copy()
+        l2q[:, o, :] = s1[8:, o, :]
+        l2q = f8(l2q.ravel(), 8, e)
+        c3 = np.array(W3)
+        c3[0] = 0.0
+        i3 = np.zeros_like(W3)
+        i3[0] = W3[0]
+        l3 = b16(layer(c3, l2q, True) + layer(i3, l2q, False))
+        wa = np.zeros((9, 8, 8, 4, 4), complex)
+        wb = np.zeros((9, 8, 8, 4, 4), complex)
+        for d in range(1, 9):
+            wa[0, d - 1] = W4[d, 0]
+            wb[0, d - 1] = comp(W4[0], W3[d])[0]
+        yq = f8(layer(wa, l3, False) + layer(wb, l2q, False), 8, o)
+        wi = np.zeros((9, 1, 8, 4, 4), complex)
+        wi[0] = comp(W4[0], W3[0])
+        wt = np.zeros((9, 1, 8, 4, 4), complex)
+        for d in range(1, 9):
+            wt[d, 0, d - 1] = np.eye(4)
+        z = layer(wi, l2q, False) + layer(wt, yq, False)
         return lat.split(z)[0]
 
     def loss_and_grad(self, pairs):

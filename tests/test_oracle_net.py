@@ -224,3 +224,26 @@ def test_apply_fp8_records_is_the_network_when_rounding_is_exact(lat, gauge, mon
     monkeypatch.setattr(orc, "fp8_record_round", lambda x, nch, sites: np.array(x, dtype=np.float64))
     same = net.apply_fp8_records(b)
     assert np.linalg.norm(same - exact) <= 1e-12 * np.linalg.norm(exact)
+
+
+def test_device_form_is_the_network_without_rounding(lat, gauge):
+    """Net.apply_device_form, the oracle of the batched device evaluation:
+    with its roundings left out, the regrouped evaluation (composite weights
+    of the first stage, the identity products, the Y mixes) is the network to
+    1e-12; with them (bf16 links, weights, inputs and transports, e4m3
+    records) it stays within a few percent, like apply_fp8_records."""
+    rng = np.random.default_rng(4)
+    ws = [rand_w(rng, 8, 1), rand_w(rng, 8, 8, 0.1), rand_w(rng, 8, 8, 0.1), rand_w(rng, 1, 8, 0.1)]
+    net = orc.Net(lat, gauge, ws)
+    b = rng.normal(size=lat.half_size)
+    exact = net.apply(b)
+    plain = net.apply_device_form(b, rounding=False)
+    assert np.linalg.norm(plain - exact) <= 1e-12 * np.linalg.norm(exact)
+    dev = net.apply_device_form(b)
+    d = np.linalg.norm(dev - exact) / np.linalg.norm(exact)
+    assert 3e-3 < d < 6e-2, d
+    # a dropped path of the second layer moves it far beyond that
+    wrong = [w.copy() for w in ws]
+    wrong[1][3] = 0.0
+    bad = orc.Net(lat, gauge, wrong).apply_device_form(b)
+    assert np.linalg.norm(bad - dev) / np.linalg.norm(dev) > 0.1
