@@ -47,14 +47,15 @@ MESON_BYTES_PER_SITE = 12 * 24 * 8 + 2 * 18 * 8
 # (per rhs, shared by all rhs).  Activation records are e4m3 (fp8) bytes,
 # 24 B per channel, plus one FP32 scale per site and 8-channel half (4 B);
 # Y_d is e4m3 too, 32 B per site and path (24 B and 8 pad bytes) plus one
-# scale per site; r and z are FP32 half fields (96 B per site); each launch
-# reads the 8 link records of its sites once (8 x 36 B) for every rhs.
-#   first: r at the even neighbours 96 + [L1|L2](odd) written 384 + 8
-#   mid: L1(odd) 192 + 4 + r 96 + L2(even) written 192 + 4
+# scale per site; the input r is read as its bf16 copy (48 B per site), z
+# is written as an FP32 half field (96 B per site); each launch reads the 8
+# link records of its sites once (8 x 36 B) for every rhs.
+#   first: r at the even neighbours 48 + [L1|L2](odd) written 384 + 8
+#   mid: L1(odd) 192 + 4 + r 48 + L2(even) written 192 + 4
 #   third: L2(even) 196 + L2(odd) 196 + Y(odd) written 8 x 32 + 4
 #   last: Y(odd) (one path of each of 8 neighbours, with its scale) 8 x 36 +
 #         L2(even) 196 + z written 96
-NET_STAGE_BYTES = [(96 + 392, 288), (196 + 96 + 196, 288), (196 + 196 + 260, 288),
+NET_STAGE_BYTES = [(48 + 392, 288), (196 + 48 + 196, 288), (196 + 196 + 260, 288),
                    (288 + 196 + 96, 288)]
 # MFMA flops per output site and rhs (v_mfma_f32_16x16x32_bf16, 16384 flops,
 # 16 sites per instruction, x 3 colours): first 8 x 2, mid 8 x 4 x 2 + 4,
