@@ -478,7 +478,10 @@ ii_status ii_qprop_multishift(ii_context* ctx, double kappa, const int site[4],
 ii_status ii_bench_hop(ii_context* ctx, int precision, int nrhs, int nrep, double* avg_ms);
 /* Launch-shape autotuner: times the admissible site traversals of the
  * single-rhs hop on this lattice (run automatically before the first FP64
- * single-rhs hop) and reports the timings and the choice as text. */
+ * single-rhs hop) and, for 12 rhs in each storage precision, the fused
+ * multi-rhs hop's rhs per block and tile order (run automatically before a
+ * batch's first multi-rhs hop of that precision); reports the timings and
+ * the choices as text. */
 ii_status ii_autotune(ii_context* ctx, char* report, size_t len);
 /* Times Dirac_Wilson_EvenOdd::md_force on device-resident fields (no host
  * copies): total_ms = the two hops forming Doe eta and Deo^dag zeta plus the

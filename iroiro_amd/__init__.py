@@ -148,8 +148,8 @@ class Context:
 
     def autotune(self):
         """Re-runs the launch-shape autotuner; returns its report."""
-        buf = ctypes.create_string_buffer(1024)
-        self.check(_lib.load().ii_autotune(self._h, buf, 1024), "ii_autotune")
+        buf = ctypes.create_string_buffer(8192)
+        self.check(_lib.load().ii_autotune(self._h, buf, 8192), "ii_autotune")
         return buf.value.decode()
 
     def bench_copy(self, nbytes, nrep=20):

@@ -143,9 +143,12 @@ ii_status ii_context_create(ii_context** out, const int dims[4], int device) {
   g.sZ = g.Lxh * dims[1];
   g.sT = g.Lxh * dims[1] * dims[2];
   g.stride = (g.Vh + 63) / 64 * 64;
-  // kernels address each array with 32-bit buffer offsets; the largest
-  // array indexed that way is the FP64 gauge field, 72 components
-  if ((unsigned long long)72 * g.stride * sizeof(double2) >= (1ull << 32)) return II_ERR_BAD_ARG;
+  // kernels address each array with one buffer descriptor and 32-bit
+  // offsets: an FP64 half spinor of one rhs (12 components) is the largest
+  // such span of the solvers (gauge fields take one descriptor per parity
+  // and direction, 9 components), so V/2 may reach about 22M sites (64^3 x
+  // 128 has 16.8M).  The network paths check their own, smaller limits.
+  if ((unsigned long long)12 * g.stride * sizeof(double2) >= (1ull << 32)) return II_ERR_BAD_ARG;
   // default-priority stream (no priority streams on this platform)
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
     return II_ERR_DEVICE;

@@ -1258,6 +1258,12 @@ ii_status bnet_tune(ii_context* ctx, BNet* B, int nrhs) {
 // shapes: everything bnet_apply needs to launch without allocating (so that
 // it can be captured)
 ii_status bnet_prepare(ii_context* ctx, int nrhs) {
+  // the largest array one descriptor spans: an rhs's odd records (384 B per
+  // site), then the links of one parity (8 x 36 B) and Y (8 x 32 B)
+  if ((unsigned long long)2 * kHalf8 * ctx->g.stride >= (1ull << 32))
+    return set_error(ctx, II_ERR_BAD_ARG,
+                     "lattice too large for the batched network (an rhs's activation records "
+                     "beyond 4 GiB)");
   BNet* B = nullptr;
   II_CHECK(bnet_get(ctx, &B));
   II_CHECK(bnet_links(ctx));

@@ -12,6 +12,7 @@
 #include "ii_ops.h"
 #include "ii_stencil.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <type_traits>
 #include <cstdlib>
@@ -321,6 +322,14 @@ namespace {
 // sites per tile of the multi-rhs kernel and its thread limit
 template <typename Gauge> struct MrhsShape { static constexpr int TS = 64, MAXT = 768; };
 template <> struct MrhsShape<GaugeF64> { static constexpr int TS = 64, MAXT = 768; };
+// storage precision (II_PREC_*) of a gauge accessor
+template <typename Gauge> struct GaugePrec;
+template <> struct GaugePrec<GaugeF64> { static constexpr int value = II_PREC_DOUBLE; };
+template <> struct GaugePrec<GaugeF32r12> { static constexpr int value = II_PREC_SINGLE; };
+template <> struct GaugePrec<GaugeLoR12<__half2>> { static constexpr int value = II_PREC_HALF; };
+template <> struct GaugePrec<GaugeLoR12<__hip_bfloat162>> {
+  static constexpr int value = II_PREC_BF16;
+};
 
 template <bool G5IN, bool G5OUT, bool XPAY, int EPI, typename SpIn, typename Gauge,
           typename StOut, typename SpX>
@@ -330,7 +339,8 @@ ii_status launch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, const 
   bool flip = ctx->tbc_sign < 0;
   if (h.nrhs == 1) {
     const HopOrder& o = ctx->hop_order;
-    if constexpr (EPI == kEpiStore && std::is_same<Gauge, GaugeF64>::value) {
+    if constexpr (EPI == kEpiStore && (std::is_same<Gauge, GaugeF64>::value ||
+                                       std::is_same<Gauge, GaugeF64Flat>::value)) {
       if (o.bs == 128) {
         dim3 grid(blocks_for(g.Vh, 128), 1);
         hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, EPI, SpIn, Gauge, StOut, SpX, 128>), grid,
@@ -347,14 +357,21 @@ ii_status launch_hop(ii_context* ctx, const SpIn& in, const Gauge& gauge, const 
     hipLaunchKernelGGL((k_hop<G5IN, G5OUT, XPAY, EPI, SpIn, Gauge, StOut, SpX>), grid,
                        dim3(kBlock), 0, ctx->stream, g, in, gauge, out, x, dv, h.out_par, a, flip,
                        h.partials, h.skip, h.skip_stride, ok, h.coef, h.coef_stride);
+  } else if constexpr (std::is_same<Gauge, GaugeF64Flat>::value) {
+    return set_error(ctx, II_ERR_STATE, "the flat FP64 gauge form serves single-rhs hops only");
   } else {
     constexpr int TS = MrhsShape<Gauge>::TS, MAXT = MrhsShape<Gauge>::MAXT;
-    const int rb = std::min(h.nrhs, MAXT / TS);
+    constexpr int P = GaugePrec<Gauge>::value;
+    if (h.rb == 0 && !h.no_tune && ctx->mrhs_nrhs[P] != h.nrhs) II_CHECK(tune_mrhs(ctx, P, h.nrhs));
+    const bool tuned = ctx->mrhs_nrhs[P] == h.nrhs;
+    int rb = h.rb > 0 ? h.rb : tuned ? ctx->mrhs_rb[P] : MAXT / TS;
+    rb = std::max(1, std::min(rb, std::min(h.nrhs, MAXT / TS)));
+    const HopOrder ord = h.order ? *h.order : tuned ? ctx->mrhs_ord[P] : ctx->mrhs_order;
     dim3 grid((g.Vh + TS - 1) / TS, (h.nrhs + rb - 1) / rb);
     hipLaunchKernelGGL((k_hop_mrhs<G5IN, G5OUT, XPAY, EPI, SpIn, Gauge, StOut, SpX, TS, MAXT>),
                        grid, dim3(TS, rb), 0, ctx->stream, g, in, gauge, out, x, dv, h.out_par,
-                       a, flip, h.partials, h.skip, h.skip_stride, h.nrhs, ctx->mrhs_order,
-                       h.coef, h.coef_stride);
+                       a, flip, h.partials, h.skip, h.skip_stride, h.nrhs, ord, h.coef,
+                       h.coef_stride);
   }
   II_HIP(ctx, hipGetLastError());
   return II_OK;
@@ -493,15 +510,109 @@ ii_status tune_hop(ii_context* ctx) {
   return st;
 }
 
+// The multi-rhs hop's launch shapes: rhs per block (the tile's links are
+// staged in LDS once per block, so fewer rhs per block re-read them but
+// give more, smaller blocks) and the tile order (contiguous per XCD, or
+// strips of W time slices, as for the single-rhs hop).  Timed with the
+// plain hop (no epilogue) of this precision on scratch fields of nrhs rhs,
+// one warm-up and 3 timed launches per candidate.
+ii_status tune_mrhs(ii_context* ctx, int P, int nrhs) {
+  if (P < 0 || P > 3 || nrhs < 2) return II_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  II_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) return II_OK;
+  if (P != II_PREC_DOUBLE && !ctx->gauge_lp[P]) II_CHECK(build_lowprec_gauge(ctx, P));
+  const Geom& g = ctx->g;
+  constexpr int TS = 64, RBMAX = 768 / 64;
+  std::vector<int> rbs;
+  for (int rb : {RBMAX, 6, 4, 3, 2})
+    if (rb <= std::min(nrhs, RBMAX) && std::find(rbs.begin(), rbs.end(), rb) == rbs.end())
+      rbs.push_back(rb);
+  if (rbs.empty() || rbs.front() != std::min(nrhs, RBMAX)) rbs.insert(rbs.begin(), std::min(nrhs, RBMAX));
+  std::vector<HopOrder> ords(1);
+  for (int W : {1, 2, 4}) {
+    const HopOrder o = strip_order(g, W, TS);
+    if (o.mode == 1) ords.push_back(o);
+  }
+  // scratch fields: components, then (16-bit) the per-site norms
+  const size_t per = P == II_PREC_DOUBLE ? (size_t)12 * g.stride * sizeof(double2)
+                     : P == II_PREC_SINGLE ? (size_t)12 * g.stride * sizeof(float2)
+                                           : (size_t)12 * g.stride * 4 + (size_t)g.stride * 4;
+  void *pa = nullptr, *pb = nullptr;
+  II_CHECK(workspace(ctx, 92, per * nrhs, &pa));
+  II_CHECK(workspace(ctx, 93, per * nrhs, &pb));
+  II_HIP(ctx, hipMemsetAsync(pa, 0, per * nrhs, ctx->stream));
+  LoField la, lb;
+  la.v = pa;
+  lb.v = pb;
+  la.nrm = (float*)((char*)pa + (size_t)12 * g.stride * 4 * nrhs);
+  lb.nrm = (float*)((char*)pb + (size_t)12 * g.stride * 4 * nrhs);
+  std::string rep = std::string("multi-rhs hop ") +
+                    (P == II_PREC_DOUBLE ? "fp64" : P == II_PREC_SINGLE ? "fp32"
+                                                  : P == II_PREC_HALF ? "fp16" : "bf16") +
+                    " x" + std::to_string(nrhs) + " (rhs per block / order) ms:";
+  float best = 1e30f;
+  int brb = rbs.front();
+  HopOrder bord;
+  for (int rb : rbs)
+    for (const HopOrder& o : ords) {
+      HopSpec h;
+      h.nrhs = nrhs;
+      h.out_par = 1;
+      h.a = -0.125;
+      h.rb = rb;
+      h.order = &o;
+      auto run = [&]() -> ii_status {
+        switch (P) {
+          case II_PREC_DOUBLE: return hop64(ctx, h, (const double2*)pa, (double2*)pb, nullptr, nullptr);
+          case II_PREC_SINGLE: return hop32(ctx, h, (const float2*)pa, (float2*)pb, nullptr, nullptr);
+          case II_PREC_HALF: return hop_half(ctx, h, la, lb, nullptr, nullptr);
+          default: return hop_bf16(ctx, h, la, lb, nullptr, nullptr);
+        }
+      };
+      II_CHECK(run());
+      II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+      for (int i = 0; i < 3; ++i) II_CHECK(run());
+      II_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+      II_HIP(ctx, hipEventSynchronize(ctx->ev1));
+      float ms = 0.f;
+      II_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+      ms /= 3.f;
+      char buf[48];
+      if (o.mode)
+        std::snprintf(buf, sizeof buf, " %d/strip%d=%.4f", rb, o.W, ms);
+      else
+        std::snprintf(buf, sizeof buf, " %d/contig=%.4f", rb, ms);
+      rep += buf;
+      if (ms < best) {
+        best = ms;
+        brb = rb;
+        bord = o;
+      }
+    }
+  char buf[48];
+  std::snprintf(buf, sizeof buf, " -> %d/%s", brb, bord.mode ? ("strip" + std::to_string(bord.W)).c_str() : "contig");
+  rep += buf;
+  ctx->mrhs_rb[P] = brb;
+  ctx->mrhs_ord[P] = bord;
+  ctx->mrhs_nrhs[P] = nrhs;
+  ctx->mrhs_report[P] = rep;
+  return II_OK;
+}
+
 ii_status hop64(ii_context* ctx, const HopSpec& h, const double2* in, double2* out,
                 const double2* x, const double2* dv, int rs) {
   const Geom& g = ctx->g;
   if (h.nrhs == 1 && !ctx->hop_tuned && !h.no_tune) II_CHECK(tune_hop(ctx));
   SpF64 sin{in, g.stride, rs};
-  GaugeF64 gf{ctx->gauge64, g.stride};
   StF64 so{out, g.stride};
   SpF64 sx{x ? x : in, g.stride, rs};
   SpF64 sd{dv ? dv : in, g.stride, rs};
+  if (h.nrhs == 1 && (unsigned long long)72 * g.stride * sizeof(double2) < (1ull << 32)) {
+    GaugeF64Flat gf{ctx->gauge64, g.stride};
+    return dispatch_hop(ctx, sin, gf, so, sx, sd, h);
+  }
+  GaugeF64 gf{ctx->gauge64, g.stride};
   return dispatch_hop(ctx, sin, gf, so, sx, sd, h);
 }
 

@@ -472,6 +472,9 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
   const dim3 sgrid(nbs, nr);
 
   if (use_net) II_CHECK(bnet_prepare(ctx, nr));
+  // the FP32 and FP64 multi-rhs hops' launch shapes, before the cycle graph
+  if (ctx->mrhs_nrhs[II_PREC_SINGLE] != nr) II_CHECK(tune_mrhs(ctx, II_PREC_SINGLE, nr));
+  if (ctx->mrhs_nrhs[II_PREC_DOUBLE] != nr) II_CHECK(tune_mrhs(ctx, II_PREC_DOUBLE, nr));
   II_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   // bhat = b_e + kappa H_eo b_o (eoUtils.cpp:54-57), |b|^2 of the full sources
   for (int k = 0; k < nr; ++k) {

@@ -112,11 +112,14 @@ __device__ __forceinline__ void force_dir(const Geom& g, const ForcePairs& fp,
   // both projections carry the factor 2 of the time direction; the force
   // carries it once
   const double s = MU == 3 ? 0.5 * scale : scale;
-  const uint32_t lbase = (uint32_t)((par * 4 + MU) * 9) * (uint32_t)g.stride;
+  // one descriptor per (parity, direction) block of U and F: its base in 64 bits
+  const size_t lblk = (size_t)((par * 4 + MU) * 9) * (size_t)g.stride;
+  const double2* Ub = U + lblk;
+  double2* Fb = F + lblk;
   double2 Ul[3][3];
 #pragma unroll
   for (int k = 0; k < 9; ++k)
-    Ul[k / 3][k % 3] = ld_at<double2, kNT>(U, lbase + (uint32_t)(k * g.stride), (uint32_t)hs);
+    Ul[k / 3][k % 3] = ld_at<double2, kNT>(Ub, (uint32_t)(k * g.stride), (uint32_t)hs);
   double2 Fl[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -130,7 +133,7 @@ __device__ __forceinline__ void force_dir(const Geom& g, const ForcePairs& fp,
   if (accumulate) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const double2 o = ld_at<double2>(F, lbase + (uint32_t)(k * g.stride), (uint32_t)hs);
+      const double2 o = ld_at<double2>(Fb, (uint32_t)(k * g.stride), (uint32_t)hs);
       Fl[k / 3][k % 3].x += o.x;
       Fl[k / 3][k % 3].y += o.y;
     }
@@ -138,7 +141,7 @@ __device__ __forceinline__ void force_dir(const Geom& g, const ForcePairs& fp,
   if (ta) traceless_antihermite(Fl);
 #pragma unroll
   for (int k = 0; k < 9; ++k)
-    st_at<double2, kNT>(F, lbase + (uint32_t)(k * g.stride), (uint32_t)hs, Fl[k / 3][k % 3]);
+    st_at<double2, kNT>(Fb, (uint32_t)(k * g.stride), (uint32_t)hs, Fl[k / 3][k % 3]);
 }
 
 // blockDim.x = 256: lanes 0..63 of wave w handle direction mu = w for the

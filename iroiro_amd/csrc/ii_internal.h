@@ -106,6 +106,13 @@ struct ii_context {
   // autotuned launch shapes
   ii::HopOrder hop_order;
   ii::HopOrder mrhs_order;  // tile order of the multi-rhs hop (64-site tiles)
+  // multi-rhs hop launch shape per storage precision (index II_PREC_*): rhs
+  // per block and tile order, chosen by tune_mrhs for mrhs_nrhs rhs (0: not
+  // yet; the default is 12 rhs per block in the contiguous order)
+  int mrhs_rb[4] = {0, 0, 0, 0};
+  int mrhs_nrhs[4] = {0, 0, 0, 0};
+  ii::HopOrder mrhs_ord[4];
+  std::string mrhs_report[4];
   bool hop_tuned = false;
   std::string tune_report;
   // learned preconditioner state (owned by ii_learned.hip) and the batched

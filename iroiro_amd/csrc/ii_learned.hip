@@ -663,9 +663,23 @@ ii_status launch_layer(ii_context* ctx, const float2* in, float2* out, const uin
   return II_OK;
 }
 
+// The layer kernels address a whole 8-channel activation array ([channel]
+// [parity][12][stride] FP32 complex) through one buffer descriptor with
+// 32-bit offsets, so they take lattices whose such array stays below 4 GiB
+// (V/2 up to about 2.79M sites).  The batched network (ii_bnet.hip) has no
+// such limit below the context's own.
+ii_status layer_fits(ii_context* ctx) {
+  if ((unsigned long long)8 * chan_elems(ctx->g) * sizeof(float2) >= (1ull << 32))
+    return set_error(ctx, II_ERR_BAD_ARG,
+                     "lattice too large for the single-rhs network layers (an 8-channel "
+                     "activation array beyond 4 GiB); the batched network takes it");
+  return II_OK;
+}
+
 // layer l forward (adj = false) or its adjoint, output parities [par0, par0+npar)
 ii_status run_layer(ii_context* ctx, Net* N, int l, bool adj, const float2* in, float2* out,
                     int par0 = 0, int npar = 2) {
+  II_CHECK(layer_fits(ctx));
   const LayerParams& P = N->L[l];
   const int cin = adj ? P.cout : P.cin, cout = adj ? P.cin : P.cout;
   const uint4* fr = adj ? P.frag_adj : P.frag_fw;
@@ -705,6 +719,7 @@ ii_status launch_wgrad(ii_context* ctx, Net* N, const float2* in, const float2* 
 
 ii_status run_wgrad(ii_context* ctx, Net* N, int l, const float2* in, const float2* gout,
                     float2* gacc, float scale) {
+  II_CHECK(layer_fits(ctx));
   const LayerParams& P = N->L[l];
   if (P.cin == 1 && P.cout == 8) return launch_wgrad<1, 8>(ctx, N, in, gout, gacc, scale);
   if (P.cin == 8 && P.cout == 8) return launch_wgrad<8, 8>(ctx, N, in, gout, gacc, scale);

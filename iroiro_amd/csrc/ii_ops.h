@@ -42,9 +42,13 @@ struct HopSpec {
   // optional per-rhs skip flags: rhs r is skipped when skip[r * skip_stride] != 0
   const int* skip = nullptr;
   int skip_stride = 0;
-  // FP64 single-rhs hops: do not run the traversal autotuner (which
-  // allocates) before the first launch; the current traversal is used
+  // do not run an autotuner (which allocates) before the first launch; the
+  // current launch shape is used
   bool no_tune = false;
+  // multi-rhs hops: rhs per block and tile order forced (the autotuner's
+  // candidates); rb = 0: the tuned or default shape
+  int rb = 0;
+  const HopOrder* order = nullptr;
 };
 
 // A 16-bit half field: components [rhs][12][stride] (half2 or bf16x2 as
@@ -68,6 +72,11 @@ ii_status hop_bf16(ii_context* ctx, const HopSpec& h, const LoField& in, const L
 inline int hop_blocks(const Geom& g) { return blocks_for(g.Vh); }
 // autotuner of the single-rhs hop traversal (ii_dslash.hip)
 ii_status tune_hop(ii_context* ctx);
+// Autotuner of the fused multi-rhs hop in storage precision P (II_PREC_*)
+// for nrhs rhs: times every (rhs per block, tile order) candidate and keeps
+// the fastest for that precision.  Not inside a stream capture (returns
+// without choosing).
+ii_status tune_mrhs(ii_context* ctx, int P, int nrhs);
 // number of per-rhs partial sums a DOT hop writes for nrhs right-hand sides
 int hop_dot_parts(const Geom& g, int nrhs, int precision);
 

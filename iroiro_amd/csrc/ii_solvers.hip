@@ -2644,6 +2644,14 @@ ii_status ii_propagator_solve(ii_context* ctx, double kappa, const ii_prop_param
     return set_error(ctx, II_ERR_BAD_ARG, "propagator fields must be full and of equal nrhs");
   II_CHECK(require_gauge(ctx));
   II_CHECK(enter(ctx));
+  // the multi-rhs hops' launch shapes for this batch, before any iteration
+  // is captured in a graph (a capture keeps the shape it records)
+  if (p->batched && src->nrhs > 1) {
+    const int sp = p->method == II_METHOD_FGCR ? II_PREC_SINGLE : p->precision;
+    if (ctx->mrhs_nrhs[sp] != src->nrhs) II_CHECK(tune_mrhs(ctx, sp, src->nrhs));
+    if (ctx->mrhs_nrhs[II_PREC_DOUBLE] != src->nrhs)
+      II_CHECK(tune_mrhs(ctx, II_PREC_DOUBLE, src->nrhs));
+  }
   ii_status st;
   if (p->method == II_METHOD_FGCR) {
     if (!p->batched)
@@ -2673,9 +2681,13 @@ ii_status ii_autotune(ii_context* ctx, char* report, size_t len) {
   II_CHECK(enter(ctx));
   ctx->hop_tuned = false;
   II_CHECK(tune_hop(ctx));
-  if (report && len) {
-    std::snprintf(report, len, "%s", ctx->tune_report.c_str());
+  std::string rep = ctx->tune_report;
+  for (int P = II_PREC_DOUBLE; P <= II_PREC_BF16; ++P) {
+    ctx->mrhs_nrhs[P] = 0;
+    II_CHECK(tune_mrhs(ctx, P, 12));
+    rep += "; " + ctx->mrhs_report[P];
   }
+  if (report && len) std::snprintf(report, len, "%s", rep.c_str());
   return II_OK;
 }
 
