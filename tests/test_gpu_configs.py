@@ -125,6 +125,28 @@ def test_c3_fused_12rhs_cg(c3):
     sol.close()
 
 
+@pytest.mark.parametrize("precision,batched", [("double", False), ("single", True)])
+def test_c3_solve_is_deterministic(c3, precision, batched):
+    """The same sources solved twice give bitwise identical solutions and iteration
+    counts: every reduction in the solvers sums in a fixed order (no atomics)."""
+    ctx, kappa = c3["ctx"], c3["kappa"]
+    src = ii.DeviceField(ctx, "full", 12)
+    sol = ii.DeviceField(ctx, "full", 12)
+    src.point_sources((0, 0, 0, 0))
+    solver = ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=precision,
+                                 batched=batched)
+    runs = []
+    for _ in range(2):
+        outs = solver.solve(src, sol)
+        runs.append(([o.Iterations for o in outs],
+                     np.stack([sol.download(k) for k in range(12)])))
+    print("C3 %s batched=%s: iterations %s" % (precision, batched, runs[0][0]))
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1])
+    src.close()
+    sol.close()
+
+
 def test_c3_multishift_12_poles(c3):
     ctx, lat, u, kappa = c3["ctx"], c3["lat"], c3["u"], c3["kappa"]
     shifts = [10.0 ** (-4 + 4 * k / 11.0) for k in range(12)]
