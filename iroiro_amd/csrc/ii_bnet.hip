@@ -262,6 +262,8 @@ struct BArgs {
   void* out;
   float* out_sc;        // the output record's scales
   void* yout;           // the third stage's Y_d (e4m3; scales in out_sc)
+  int* ctr;             // work-queue counters: 8 tickets, then 8 leave counts, 4 KiB apart
+                        // (nullptr: static order)
 };
 
 __device__ __forceinline__ uint32_t link_off(const Geom& g, int dh) {
@@ -346,6 +348,81 @@ __device__ __forceinline__ int next_live(const BArgs& a, const NetRange& S, int 
     if (rhs_live(a, net_item(a.sw, S, it).rhs, flags)) break;
   return it;
 }
+// Work order of a stage launch.  Dynamic (up to 64 rhs, with counters): the
+// waves of XCD group x take the items of its list over the live rhs
+// (net_item_live) from ctr[x], kQChunk consecutive items per ticket (one
+// atomic), each ticket requested one item ahead and read after the item
+// before it has been stored, so that the atomic's latency hides behind
+// that item.  The items in flight then stay a
+// window of consecutive items of the list (neighbour rows shared in L2) even
+// when waves run at different speeds; with a fixed share per wave they
+// drift apart over the hundreds of items each wave takes.  The last wave of
+// a group to leave resets ctr[x] and its leave count ctr[8 + x] to zero for
+// the next launch, which stream order puts after this one.  Static
+// otherwise: wave j takes items j, j + step, ... of the list of all rhs.
+// Items per ticket: 2 (measured at 32^3x64, 12 rhs, stages in ms: one item
+// per ticket 2.59 / 5.16 / 5.58 / 2.11, two 2.58 / 5.18 / 5.57 / 1.90, four
+// the same as two; the static order 2.73 / 5.61 / 6.09 / 2.08).  The
+// counters sit 4 KiB apart: in one cache line the atomics of the eight XCDs
+// were serialised, every stage took 9 ms.
+constexpr int kQChunk = 2;
+constexpr int kCtrStride = 1024;
+struct ItemQ {
+  NetRange S;
+  NetLive L;
+  int* ctr;  // nullptr: static order
+  int x;     // XCD group (block mod nxcd)
+  int nwx;   // waves of the group
+  int flags;
+};
+__device__ __forceinline__ ItemQ item_q(const BArgs& a, int nw, int wave, int lane) {
+  ItemQ Q;
+  Q.S = net_range(a.sw, blockIdx.x, gridDim.x, nw, wave);
+  Q.flags = skip_lanes(a, lane);
+  Q.ctr = (a.ctr && a.sw.nrhs <= 64) ? a.ctr : nullptr;
+  Q.x = blockIdx.x % a.sw.nxcd;
+  Q.nwx = (int)(gridDim.x / a.sw.nxcd) * nw;
+  Q.L = net_live(Q.S, __ballot(lane < a.sw.nrhs && Q.flags == 0));
+  return Q;
+}
+// requests the ticket that the item after it will need (lane 0's atomic;
+// only at the end of a chunk)
+__device__ __forceinline__ int q_request(const ItemQ& Q, int lane, int it) {
+  int t = 0;
+  if (Q.ctr && lane == 0 && (it + 1) % kQChunk == 0)
+    t = atomicAdd(Q.ctr + Q.x * kCtrStride, 1);
+  return t;
+}
+// the item after it (-1: none): the next of the chunk, else the first of
+// the requested ticket's chunk; the next live item of the static order
+__device__ __forceinline__ int q_take(const BArgs& a, const ItemQ& Q, int ticket, int it) {
+  if (Q.ctr) {
+    int t = it + 1;
+    if (t % kQChunk == 0) t = __builtin_amdgcn_readfirstlane(ticket) * kQChunk;
+    return t < Q.L.nit ? t : -1;
+  }
+  const int n = next_live(a, Q.S, it + Q.S.step, Q.flags);
+  return n < Q.S.nit ? n : -1;
+}
+__device__ __forceinline__ int q_first(const BArgs& a, const ItemQ& Q, int lane) {
+  if (Q.ctr) return q_take(a, Q, q_request(Q, lane, -1), -1);
+  const int n = next_live(a, Q.S, Q.S.first, Q.flags);
+  return n < Q.S.nit ? n : -1;
+}
+__device__ __forceinline__ NetItem q_item(const BArgs& a, const ItemQ& Q, int it) {
+  return Q.ctr ? net_item_live(a.sw, Q.S, Q.L, it) : net_item(a.sw, Q.S, it);
+}
+// every wave calls this once, after its last item
+__device__ __forceinline__ void q_leave(const ItemQ& Q, int lane) {
+  if (Q.ctr && lane == 0) {
+    int* c = Q.ctr + Q.x * kCtrStride;
+    if (atomicAdd(c + 8 * kCtrStride, 1) == Q.nwx - 1) {
+      atomicExch(c, 0);
+      atomicExch(c + 8 * kCtrStride, 0);
+    }
+  }
+}
+
 // path dh's link record and half 0 (channels q, 4 + q of every colour) of
 // the neighbour's record (H halves per site) with its scale
 template <int H>
@@ -478,21 +555,21 @@ __global__ void __launch_bounds__(kThreadsFirst, 2) k_bnet_first(BArgs a) {
   const Geom& g = a.g;
   const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
-  const int flags = skip_lanes(a, lane);
-  int it = next_live(a, S, S.first, flags);
-  if (it >= S.nit) return;
-  ItemPos P = item_pos(g, net_item(a.sw, S, it), n, 1);
+  const ItemQ Q = item_q(a, NW, wave, lane);
+  int it = q_first(a, Q, lane);
+  int itn = it >= 0 ? q_take(a, Q, q_request(Q, lane, it), it) : -1;
+  ItemPos P{};
+  if (it >= 0) P = item_pos(g, q_item(a, Q, it), n, 1);
   RawFirst X;
-  load_first(a, P, q, X);
-  for (;;) {
+  if (it >= 0) load_first(a, P, q, X);
+  while (it >= 0) {
+    const int tk = itn >= 0 ? q_request(Q, lane, itn) : 0;
     bf16x8 B[KB][3];
 #pragma unroll
     for (int kb = 0; kb < KB; ++kb) transport(X.L[kb], X.r[kb], B[kb]);
-    const int it2 = next_live(a, S, it + S.step, flags);
     ItemPos P2 = P;
-    if (it2 < S.nit) {
-      P2 = item_pos(g, net_item(a.sw, S, it2), n, 1);
+    if (itn >= 0) {
+      P2 = item_pos(g, q_item(a, Q, itn), n, 1);
       load_first(a, P2, q, X);
     }
     // the matrix products and stores after the next item's loads are issued
@@ -518,10 +595,12 @@ __global__ void __launch_bounds__(kThreadsFirst, 2) k_bnet_first(BArgs a) {
         }
       store_half8<2>(o, osc, P.s, h, q, P.valid, acc[0], acc[1], acc[2]);
     }
-    if (it2 >= S.nit) break;
-    it = it2;
+    const int itnn = itn >= 0 ? q_take(a, Q, tk, itn) : -1;
+    it = itn;
+    itn = itnn;
     P = P2;
   }
+  q_leave(Q, lane);
 }
 
 // stage 2, even sites: L2(even) = (W2_0 W1_0) r + sum_{d>0} W2_d T_d L1(odd)
@@ -537,21 +616,23 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
   const Geom& g = a.g;
   const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
-  const int flags = skip_lanes(a, lane);
-  int it = next_live(a, S, S.first, flags);
-  if (it >= S.nit) return;
-  ItemPos P = item_pos(g, net_item(a.sw, S, it), n, 0);
+  const ItemQ Q = item_q(a, NW, wave, lane);
+  int it = q_first(a, Q, lane);
+  int itn = it >= 0 ? q_take(a, Q, q_request(Q, lane, it), it) : -1;
+  ItemPos P{};
+  if (it >= 0) P = item_pos(g, q_item(a, Q, it), n, 0);
   auto inp = [&](const ItemPos& p) {
     return (const char*)a.in + (size_t)p.rhs * g.stride * (2 * kHalf8);
   };
   auto insc = [&](const ItemPos& p) { return a.in_sc + (size_t)p.rhs * g.stride * 2; };
   PathBuf X0, X1, X2;
-  load_path<2>(a, P, inp(P), insc(P), 0, q, X0);
-  load_path<2>(a, P, inp(P), insc(P), 1, q, X1);
-  for (;;) {
-    const int it2 = next_live(a, S, it + S.step, flags);
-    const ItemPos P2 = it2 < S.nit ? item_pos(g, net_item(a.sw, S, it2), n, 0) : P;
+  if (it >= 0) {
+    load_path<2>(a, P, inp(P), insc(P), 0, q, X0);
+    load_path<2>(a, P, inp(P), insc(P), 1, q, X1);
+  }
+  while (it >= 0) {
+    const int tk = itn >= 0 ? q_request(Q, lane, itn) : 0;
+    const ItemPos P2 = itn >= 0 ? item_pos(g, q_item(a, Q, itn), n, 0) : P;
     f32x4 acc[3][MB];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -566,7 +647,7 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
           for (int c = 0; c < 3; ++c) rid[c] = ld128<kIdCP>(r, 16u * c, (uint32_t)P.s * 48u);
         },
         [&](PathBuf& b, int path) {
-          if (it2 < S.nit) load_path<2>(a, P2, inp(P2), insc(P2), path, q, b);
+          if (itn >= 0) load_path<2>(a, P2, inp(P2), insc(P2), path, q, b);
         },
         // identity: (W2_0 W1_0) r, r in channel 0 (lane group 0) of the B operand
         [&](const PathBuf&) {
@@ -584,10 +665,12 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
     store_half8<1>((char*)a.out + (size_t)P.rhs * g.stride * kHalf8,
                    a.out_sc + (size_t)P.rhs * g.stride, P.s, 0, q, P.valid, acc[0], acc[1],
                    acc[2]);
-    if (it2 >= S.nit) break;
-    it = it2;
+    const int itnn = itn >= 0 ? q_take(a, Q, tk, itn) : -1;
+    it = itn;
+    itn = itnn;
     P = P2;
   }
+  q_leave(Q, lane);
 }
 
 // stage 3, odd sites: L3(odd) = W3_0 L2(odd) + sum_{d>0} W3_d T_d L2(even)
@@ -607,21 +690,23 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
   const Geom& g = a.g;
   const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
-  const int flags = skip_lanes(a, lane);
-  int it = next_live(a, S, S.first, flags);
-  if (it >= S.nit) return;
-  ItemPos P = item_pos(g, net_item(a.sw, S, it), n, 1);
+  const ItemQ Q = item_q(a, NW, wave, lane);
+  int it = q_first(a, Q, lane);
+  int itn = it >= 0 ? q_take(a, Q, q_request(Q, lane, it), it) : -1;
+  ItemPos P{};
+  if (it >= 0) P = item_pos(g, q_item(a, Q, it), n, 1);
   auto inp = [&](const ItemPos& p) {
     return (const char*)a.in + (size_t)p.rhs * g.stride * kHalf8;
   };
   auto insc = [&](const ItemPos& p) { return a.in_sc + (size_t)p.rhs * g.stride; };
   PathBuf X0, X1, X2;
-  load_path<1>(a, P, inp(P), insc(P), 0, q, X0);
-  load_path<1>(a, P, inp(P), insc(P), 1, q, X1);
-  for (;;) {
-    const int it2 = next_live(a, S, it + S.step, flags);
-    const ItemPos P2 = it2 < S.nit ? item_pos(g, net_item(a.sw, S, it2), n, 1) : P;
+  if (it >= 0) {
+    load_path<1>(a, P, inp(P), insc(P), 0, q, X0);
+    load_path<1>(a, P, inp(P), insc(P), 1, q, X1);
+  }
+  while (it >= 0) {
+    const int tk = itn >= 0 ? q_request(Q, lane, itn) : 0;
+    const ItemPos P2 = itn >= 0 ? item_pos(g, q_item(a, Q, itn), n, 1) : P;
     const char* rec = (const char*)a.in_id + (size_t)P.rhs * g.stride * (2 * kHalf8);
     const float* rsc = a.id_sc + (size_t)P.rhs * g.stride * 2;
     f32x4 acc[3][MB];
@@ -640,7 +725,7 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
           b.sc = __uint_as_float(ld32<kIdCP>(rsc, 0u, ((uint32_t)P.s * 2u + 1u) * 4u));
         },
         [&](PathBuf& b, int path) {
-          if (it2 < S.nit) load_path<1>(a, P2, inp(P2), insc(P2), path, q, b);
+          if (itn >= 0) load_path<1>(a, P2, inp(P2), insc(P2), path, q, b);
         },
         // identity W3_0 L2(odd), then the Y mixes: k-blocks 0, 1 from the L3
         // accumulators (blocks 2 kb, 2 kb + 1), k-blocks 2, 3 from L2(odd)
@@ -711,10 +796,12 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
           st32(a.out_sc + (size_t)P.rhs * g.stride, 0u, (uint32_t)P.s * 4u, __float_as_uint(ds));
       }
     }
-    if (it2 >= S.nit) break;
-    it = it2;
+    const int itnn = itn >= 0 ? q_take(a, Q, tk, itn) : -1;
+    it = itn;
+    itn = itnn;
     P = P2;
   }
+  q_leave(Q, lane);
 }
 
 // --------------------------------------------------------------- stage 4
@@ -732,11 +819,12 @@ __global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
   const Geom& g = a.g;
   const int lane = threadIdx.x & 63, q = lane >> 4, n = lane & 15;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const NetRange S = net_range(a.sw, blockIdx.x, gridDim.x, NW, wave);
-  const int flags = skip_lanes(a, lane);
-  for (int it = S.first; it < S.nit; it += S.step) {
-    const NetItem w = net_item(a.sw, S, it);
-    if (!rhs_live(a, w.rhs, flags)) continue;
+  const ItemQ Q = item_q(a, NW, wave, lane);
+  int it = q_first(a, Q, lane);
+  int itn = it >= 0 ? q_take(a, Q, q_request(Q, lane, it), it) : -1;
+  while (it >= 0) {
+    const int tk = itn >= 0 ? q_request(Q, lane, itn) : 0;
+    const NetItem w = q_item(a, Q, it);
     const int site = w.tile * 16 + n;
     const bool valid = site < g.Vh;
     const int s = valid ? site : g.Vh - 1;
@@ -793,7 +881,11 @@ __global__ void __launch_bounds__(kThreadsLast, 4) k_bnet_last(BArgs a) {
       for (int c = 0; c < 3; ++c)
         st_at(o, (uint32_t)((3 * q + c) * g.stride), (uint32_t)s, make_float2(zr[c], zi[c]));
     }
+    const int itnn = itn >= 0 ? q_take(a, Q, tk, itn) : -1;
+    it = itn;
+    itn = itnn;
   }
+  q_leave(Q, lane);
 }
 
 // Dense pseudo-random input in (-1, 1) for the stage timings (autotuner and
@@ -967,6 +1059,7 @@ struct BNet {
   int by[4] = {1, 1, 1, 1};    // (y, z) block of each stage's sweep (bz 0: L_z)
   int bz[4] = {0, 0, 0, 0};
   int tuned_nrhs = 0;          // the rhs count rg was chosen for
+  int* ctr = nullptr;          // the stages' work-queue counters (zero between launches)
   std::string report;
 };
 
@@ -979,6 +1072,9 @@ ii_status bnet_get(ii_context* ctx, BNet** out) {
     int n = 0;
     II_HIP(ctx, hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, ctx->device));
     B->ncu = std::max(1, n);
+    II_HIP(ctx, hipMalloc(&B->ctr, 16 * kCtrStride * sizeof(int)));
+    II_HIP(ctx, hipMemsetAsync(B->ctr, 0, 16 * kCtrStride * sizeof(int), ctx->stream));
+    II_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   *out = bnet_of(ctx);
   return II_OK;
@@ -1120,6 +1216,7 @@ ii_status bnet_launch(ii_context* ctx, int stage, int nrhs, int rg, int by, int 
   a.g = g;
   a.skip = skip;
   a.skip_stride = skip_stride;
+  a.ctr = B->ctr;
   int nblk = 0;
   net_grid(g, B->ncu, kStageBlocksPerCU[stage], kStageThreads[stage], nrhs, rg, by,
            bz > 0 ? bz : g.L[2], a.sw, &nblk);
@@ -1294,6 +1391,7 @@ void bnet_release(ii_context* ctx) {
   if (B->even_sc) (void)hipFree(B->even_sc);
   if (B->ybuf) (void)hipFree(B->ybuf);
   if (B->ysc) (void)hipFree(B->ysc);
+  if (B->ctr) (void)hipFree(B->ctr);
   delete B;
   ctx->bnet = nullptr;
 }

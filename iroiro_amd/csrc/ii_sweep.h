@@ -83,12 +83,8 @@ __host__ __device__ __forceinline__ NetRange net_range(const NetSweep& s, int bl
   return NetRange{t0, ntl, ngrp * ntl * s.rg, j * nw + wave, (nblk / s.nxcd) * nw};
 }
 
-__host__ __device__ __forceinline__ NetItem net_item(const NetSweep& s, const NetRange& R, int it) {
-  const int per = R.ntl * s.rg;
-  const int grp = it / per;
-  const int rem = it - grp * per;
-  const int o = rem / s.rg;
-  const int rhs = grp * s.rg + (rem - o * s.rg);
+// tile of position o (0 .. ntl - 1) of the XCD's tile order
+__host__ __device__ __forceinline__ int net_tile(const NetSweep& s, const NetRange& R, int o) {
   if (s.nt > 0) {
     const int tt = o % s.nt;
     int rest = o / s.nt;
@@ -98,9 +94,47 @@ __host__ __device__ __forceinline__ NetItem net_item(const NetSweep& s, const Ne
     rest /= s.bz;
     const int nby = s.ytiles / s.by;
     const int yi = (rest % nby) * s.by + yb, zz = (rest / nby) * s.bz + zb;
-    return NetItem{R.t0 + tt * s.ztiles + zz * s.ytiles + yi, rhs};
+    return R.t0 + tt * s.ztiles + zz * s.ytiles + yi;
   }
-  return NetItem{R.t0 + o, rhs};
+  return R.t0 + o;
+}
+__host__ __device__ __forceinline__ NetItem net_item(const NetSweep& s, const NetRange& R, int it) {
+  const int per = R.ntl * s.rg;
+  const int grp = it / per;
+  const int rem = it - grp * per;
+  const int o = rem / s.rg;
+  const int rhs = grp * s.rg + (rem - o * s.rg);
+  return NetItem{net_tile(s, R, o), rhs};
+}
+
+// The list of an XCD over the live rhs only (the dynamic order of the stage
+// kernels, which hand the list out from a counter): live is a bit mask of
+// rhs 0..63; the k-th live rhs takes the place of rhs k, in groups of rg
+// (the last group holds what is left), so the list has ntl x nlive items
+// and none of them is skipped.
+struct NetLive {
+  unsigned long long mask;
+  int nlive, nit;
+};
+__host__ __device__ __forceinline__ NetLive net_live(const NetRange& R, unsigned long long mask) {
+  const int n = __builtin_popcountll(mask);
+  return NetLive{mask, n, R.ntl * n};
+}
+// the k-th set bit of m (k < popcount(m))
+__host__ __device__ __forceinline__ int kth_bit(unsigned long long m, int k) {
+  for (int i = 0; i < k; ++i) m &= m - 1;
+  return __builtin_ctzll(m);
+}
+__host__ __device__ __forceinline__ NetItem net_item_live(const NetSweep& s, const NetRange& R,
+                                                          const NetLive& L, int it) {
+  const int per = R.ntl * s.rg;
+  const int grp = it / per;
+  const int rem = it - grp * per;
+  const int left = L.nlive - grp * s.rg;
+  const int rgi = left < s.rg ? left : s.rg;
+  const int o = rem / rgi;
+  const int k = grp * s.rg + (rem - o * rgi);
+  return NetItem{net_tile(s, R, o), kth_bit(L.mask, k)};
 }
 
 // Half index of the neighbour of site s (coordinates c, either parity) along

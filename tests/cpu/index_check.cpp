@@ -10,7 +10,8 @@
 //     or 128 threads per block, the XCD time-strip orders) and the 64-site
 //     tiles of the multi-rhs hop cover [0, V/2) exactly once;
 //   * the batched network's stage kernels take every (16-site tile, rhs)
-//     item once (ii_sweep.h: per-XCD slabs, t-fastest fronts, rhs groups).
+//     item once (ii_sweep.h: per-XCD slabs, t-fastest fronts, rhs groups),
+//     in the static order and in the dynamic order over the live rhs.
 // Built with the host sanitizers (address, undefined behaviour) by
 // tests/test_sanitizers.py; exits non-zero on the first violation.
 #include "../../iroiro_amd/csrc/ii_stencil.h"
@@ -136,6 +137,30 @@ int main() {
                     "(ncu %d, %d blocks per CU, rg %d, nt %d, by %d, bz %d)", L[0], L[1], L[2],
                     L[3], (int)(i / nrhs), (int)(i % nrhs), th[i], ncu, per_cu, rg, s.nt, s.by,
                     s.bz);
+            // the dynamic order: each XCD's list over the live rhs, handed out
+            // from one counter per XCD, takes every (tile, live rhs) once
+            const unsigned long long all = nrhs == 64 ? ~0ull : (1ull << nrhs) - 1;
+            for (unsigned long long mask : {all, all & 0x5ull, all & ~1ull, 0ull,
+                                            all & (1ull << (nrhs - 1))}) {
+              std::vector<int> tl((size_t)s.tiles * nrhs, 0);
+              for (int x = 0; x < s.nxcd && x < nblk; ++x) {
+                const NetRange R = net_range(s, x, nblk, nw, 0);
+                const NetLive Lv = net_live(R, mask);
+                for (int it = 0; it < Lv.nit; ++it) {
+                  const NetItem m = net_item_live(s, R, Lv, it);
+                  const bool ok = m.tile >= R.t0 && m.tile < R.t0 + R.ntl && m.rhs >= 0 &&
+                                  m.rhs < nrhs && ((mask >> m.rhs) & 1);
+                  CHECK(ok, "dynamic network item %d -> (tile %d, rhs %d)", it, m.tile, m.rhs);
+                  if (ok) tl[(size_t)m.tile * nrhs + m.rhs]++;
+                }
+              }
+              for (size_t i = 0; i < tl.size(); ++i) {
+                const int want = (mask >> (i % nrhs)) & 1 ? 1 : 0;
+                CHECK(tl[i] == want, "L=%dx%dx%dx%d dynamic network item (tile %d, rhs %d) "
+                      "covered %d times, want %d (mask %llx, rg %d, nblk %d)", L[0], L[1], L[2],
+                      L[3], (int)(i / nrhs), (int)(i % nrhs), tl[i], want, mask, rg, nblk);
+              }
+            }
           }
   }
   if (fails) return 1;
