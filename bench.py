@@ -222,6 +222,9 @@ def main():
                          "the same without it, BiCGStab on Dhat or CG on Dhat^dag Dhat")
     ap.add_argument("--nkrylov", type=int, default=8,
                     help="flexible GCR directions per restart cycle")
+    ap.add_argument("--northo", type=int, default=0,
+                    help="flexible GCR: directions each new one is orthogonalised against "
+                         "(0: the library default, 1; --nkrylov or more: the full GCR)")
     ap.add_argument("--no-others", action="store_true",
                     help="skip timing the other methods")
     ap.add_argument("--no-cpu", action="store_true")
@@ -394,7 +397,8 @@ def main():
         if method in ("fgcr-learned", "fgcr"):
             return ii.PropagatorSolver(ctx, kappa, tol=1e-10, batched=True, method="fgcr",
                                        use_learned=(method == "fgcr-learned"),
-                                       use_graph=use_graph, nkrylov=args.nkrylov)
+                                       use_graph=use_graph, nkrylov=args.nkrylov,
+                                       northo=args.northo)
         return ii.PropagatorSolver(ctx, kappa, tol=1e-10, precision=args.precision, batched=True,
                                    method=method)
 
@@ -507,9 +511,10 @@ def main():
     method_desc = {
         "fgcr-learned": "batched 12-rhs flexible GCR on Dhat with the trained gauge-covariant "
                         "network as preconditioner (bf16 activations and links, MFMA channel "
-                        "mixing), FP32 Krylov vectors and FP32 Dhat, restart cycles of %d "
+                        "mixing), FP32 Krylov vectors and FP32 Dhat, each direction "
+                        "orthogonalised against the previous %d, restart cycles of %d "
                         "directions captured as HIP graphs, FP64 true-residual restarts"
-                        % args.nkrylov,
+                        % (args.northo if args.northo > 0 else 1, args.nkrylov),
         "fgcr": "batched 12-rhs flexible GCR on Dhat without preconditioner, FP32 Krylov "
                 "vectors, FP64 true-residual restarts",
         "bicgstab": "batched 12-rhs BiCGStab on Dhat, %s storage, fp64 reliable updates"

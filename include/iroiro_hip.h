@@ -341,6 +341,9 @@ typedef struct {
                             direction (0 = default 16) */
   int nkrylov;           /* flexible GCR (use_learned): directions per restart cycle
                             (0 = default) */
+  int northo;            /* flexible GCR: each new direction is orthogonalised against
+                            the last northo directions of its cycle (0 = default 1;
+                            nkrylov or more: all of them, the full GCR) */
 } ii_prop_params;
 enum { II_METHOD_CG = 0, II_METHOD_BICGSTAB = 1, II_METHOD_FGCR = 2 };
 

@@ -919,13 +919,13 @@ class PropagatorSolver:
 
     def __init__(self, ctx, kappa, tol=1e-10, precision="double", batched=False, maxiter=100000,
                  delta=0.0, use_graph=False, use_learned=False, method="cg",
-                 restart_factor=0.0, nkrylov=0):
+                 restart_factor=0.0, nkrylov=0, northo=0):
         self.ctx = ctx
         self.kappa = float(kappa)
         self.params = _lib.PropParamsC(tol, maxiter, PRECISIONS[precision], int(batched),
                                        float(delta), int(use_graph), int(use_learned),
                                        _lib.II_METHOD[method], float(restart_factor),
-                                       int(nkrylov))
+                                       int(nkrylov), int(northo))
 
     def solve(self, src, sol):
         outs = (_lib.SolverOutputC * src.nrhs)()

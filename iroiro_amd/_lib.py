@@ -77,7 +77,7 @@ class PropParamsC(ctypes.Structure):
                 ("batched", ctypes.c_int), ("delta", ctypes.c_double),
                 ("use_graph", ctypes.c_int), ("use_learned", ctypes.c_int),
                 ("method", ctypes.c_int), ("restart_factor", ctypes.c_double),
-                ("nkrylov", ctypes.c_int)]
+                ("nkrylov", ctypes.c_int), ("northo", ctypes.c_int)]
 
 
 II_METHOD = {"cg": 0, "bicgstab": 1, "fgcr": 2}

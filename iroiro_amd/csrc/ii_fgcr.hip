@@ -10,10 +10,13 @@
 // per rhs, <a, b> = sum conj(a) b):
 //   z_k = M r                        (the batched network, ii_bnet.hip)
 //   w_k = Dhat z_k                   (two FP32 multi-rhs hops)
-//   beta_j = <w_j, w_k> / |w_j|^2,   w_k -= sum_j beta_j w_j   (j < k)
+//   beta_j = <w_j, w_k> / |w_j|^2,   w_k -= sum_j beta_j w_j   (k - n <= j < k)
 //   alpha_k = <w_k, r> / |w_k|^2,    r -= alpha_k w_k
-// (classical Gram-Schmidt: one pass for all the dot products, one for the
-// update).  The solution is never formed inside a cycle: with
+// (classical Gram-Schmidt against the last n = northo directions of the
+// cycle: one pass for the dot products, one for the update; n >= the cycle
+// length is the full GCR, smaller n the truncated one, which keeps the
+// residual minimal along each new direction but not over the whole cycle,
+// and reads fewer vectors per direction).  The solution is never formed inside a cycle: with
 // z'_k = z_k - sum_j beta_kj z'_j = sum_i T_ki z_i, the cycle's update is
 // sum_i gamma_i z_i, gamma_i = sum_k alpha_k T_ki, added to the FP64
 // solution in one pass at the end of the cycle.  Then the FP64 true
@@ -80,10 +83,10 @@ __device__ __forceinline__ double gsum(const double* part, int nplanes, int j, i
 
 // ------------------------------------------------------------ site kernels
 // FP32 half fields [rhs][12][stride]; slot fields [slot][rhs][12][stride].
-// <w_j, w_k> for j < k into planes 2j, 2j+1
+// <w_j, w_k> for j0 <= j < k into planes 2j, 2j+1
 __global__ void __launch_bounds__(kGThreads)
-k_g_dots(Geom g, const float2* __restrict__ W, size_t slot, int k, const GScal* __restrict__ gs,
-         double* __restrict__ part) {
+k_g_dots(Geom g, const float2* __restrict__ W, size_t slot, int j0, int k,
+         const GScal* __restrict__ gs, double* __restrict__ part) {
   const int rhs = blockIdx.y;
   if (gs[rhs].idle) return;
   const int hs = blockIdx.x * blockDim.x + threadIdx.x;
@@ -97,7 +100,7 @@ k_g_dots(Geom g, const float2* __restrict__ W, size_t slot, int k, const GScal* 
       const float2 b = wk[(size_t)c * g.stride];
 #pragma unroll
       for (int j = 0; j < kMaxK; ++j)
-        if (j < k) {
+        if (j >= j0 && j < k) {
           const float2 a = W[j * slot + rhs * he + (size_t)c * g.stride + hs];
           acc[2 * j] += (double)a.x * b.x + (double)a.y * b.y;
           acc[2 * j + 1] += (double)a.x * b.y - (double)a.y * b.x;
@@ -107,16 +110,17 @@ k_g_dots(Geom g, const float2* __restrict__ W, size_t slot, int k, const GScal* 
   const int nb = gridDim.x;
 #pragma unroll
   for (int j = 0; j < 2 * kMaxK; ++j)
-    if (j < 2 * k) {
+    if (j >= 2 * j0 && j < 2 * k) {
       const double s = block_sum(acc[j]);
       if (threadIdx.x == 0) part[((size_t)rhs * 2 * kMaxK + j) * nb + blockIdx.x] = s;
     }
 }
 
-__global__ void k_g_beta(const double* __restrict__ part, int nb, int k, GScal* __restrict__ gs) {
+__global__ void k_g_beta(const double* __restrict__ part, int nb, int j0, int k,
+                         GScal* __restrict__ gs) {
   GScal& S = gs[blockIdx.x];
   if (S.idle) return;
-  for (int j = 0; j < k; ++j) {
+  for (int j = j0; j < k; ++j) {
     const double re = gsum(part, 2 * kMaxK, 2 * j, nb);
     const double im = gsum(part, 2 * kMaxK, 2 * j + 1, nb);
     if (threadIdx.x == 0) {
@@ -126,9 +130,9 @@ __global__ void k_g_beta(const double* __restrict__ part, int nb, int k, GScal* 
   }
 }
 
-// w_k -= sum_j beta_kj w_j; partials |w_k|^2 (plane 0), <w_k, r> (1, 2)
+// w_k -= sum_{j0 <= j < k} beta_kj w_j; partials |w_k|^2 (plane 0), <w_k, r> (1, 2)
 __global__ void __launch_bounds__(kGThreads)
-k_g_orth(Geom g, float2* __restrict__ W, size_t slot, int k, const float2* __restrict__ R,
+k_g_orth(Geom g, float2* __restrict__ W, size_t slot, int j0, int k, const float2* __restrict__ R,
          const GScal* __restrict__ gs, double* __restrict__ part) {
   const int rhs = blockIdx.y;
   const GScal& S = gs[rhs];
@@ -138,8 +142,8 @@ k_g_orth(Geom g, float2* __restrict__ W, size_t slot, int k, const float2* __res
   float br[kMaxK], bi[kMaxK];
 #pragma unroll
   for (int j = 0; j < kMaxK; ++j) {
-    br[j] = j < k ? (float)S.beta[k][j][0] : 0.f;
-    bi[j] = j < k ? (float)S.beta[k][j][1] : 0.f;
+    br[j] = (j >= j0 && j < k) ? (float)S.beta[k][j][0] : 0.f;
+    bi[j] = (j >= j0 && j < k) ? (float)S.beta[k][j][1] : 0.f;
   }
   double ww = 0.0, wr = 0.0, wi = 0.0;
   if (hs < g.Vh) {
@@ -149,7 +153,7 @@ k_g_orth(Geom g, float2* __restrict__ W, size_t slot, int k, const float2* __res
       float2 v = wk[(size_t)c * g.stride];
 #pragma unroll
       for (int j = 0; j < kMaxK; ++j)
-        if (j < k) {
+        if (j >= j0 && j < k) {
           const float2 a = W[j * slot + rhs * he + (size_t)c * g.stride + hs];
           v.x -= br[j] * a.x - bi[j] * a.y;
           v.y -= br[j] * a.y + bi[j] * a.x;
@@ -431,6 +435,8 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
   const int nr = src->nrhs;
   const bool use_net = prm->use_learned != 0;
   const int m = std::max(1, std::min(prm->nkrylov > 0 ? prm->nkrylov : 8, kMaxK));
+  // orthogonalisation window: the last nor directions of the cycle
+  const int nor = prm->northo > 0 ? prm->northo : 1;
   const double tol = prm->tol > 0 ? prm->tol : 1e-10;
   const int maxiter = prm->maxiter > 0 ? prm->maxiter : 100000;
   if (!ctx->gauge_lp[II_PREC_SINGLE]) II_CHECK(build_lowprec_gauge(ctx, II_PREC_SINGLE));
@@ -533,13 +539,14 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
       a.a = -kappa * kappa;
       a.xpay = true;
       II_CHECK(hop32(ctx, a, T, wk, zk, nullptr));
+      const int j0 = std::max(0, k - nor);
       if (k > 0) {
-        hipLaunchKernelGGL(k_g_dots, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, gs,
+        hipLaunchKernelGGL(k_g_dots, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, j0, k, gs,
                            part);
-        hipLaunchKernelGGL(k_g_beta, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, k, gs);
+        hipLaunchKernelGGL(k_g_beta, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, j0, k, gs);
       }
-      hipLaunchKernelGGL(k_g_orth, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, R, gs,
-                         part);
+      hipLaunchKernelGGL(k_g_orth, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, j0, k, R,
+                         gs, part);
       hipLaunchKernelGGL(k_g_alpha, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, k, gs);
       hipLaunchKernelGGL(k_g_rupd, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, R, Rb, gs,
                          part);

@@ -285,13 +285,17 @@ def _fgcr_setup(dims, kappa, seed=5):
     return lat, u, ctx, net
 
 
-@pytest.mark.parametrize("use_net,use_graph", [(True, False), (False, False), (True, True)])
-def test_fgcr_batched_point_sources(use_net, use_graph):
+@pytest.mark.parametrize("use_net,use_graph,northo", [(True, False, 0), (False, False, 0),
+                                                     (True, True, 0), (True, False, 3),
+                                                     (True, True, 8)])
+def test_fgcr_batched_point_sources(use_net, use_graph, northo):
     """The batched mixed-precision flexible GCR (every rhs per launch, FP32
     Krylov vectors, the network on bfloat16 activations, FP64 restarts): the
     12 point-source columns reach the full-operator true residual 1e-10
     (oracle) and the oracle's FP64 solution to 1e-8, with and without the
-    network and with the restart cycle captured as a HIP graph."""
+    network, with the restart cycle captured as a HIP graph, and with each
+    direction orthogonalised against the previous one (default), three, or
+    all of the cycle (the full GCR)."""
     dims, kappa = (8, 8, 8, 8), 0.1355
     lat, u, ctx, net = _fgcr_setup(dims, kappa)
     try:
@@ -300,9 +304,10 @@ def test_fgcr_batched_point_sources(use_net, use_graph):
         src.point_sources((0, 0, 0, 0))
         outs = ii.PropagatorSolver(ctx, kappa, tol=1e-10, batched=True, method="fgcr",
                                    use_learned=use_net, use_graph=use_graph,
-                                   nkrylov=8).solve(src, sol)
-        print("fgcr net=%s graph=%s: directions %s, cycles %s, fallback %s"
-              % (use_net, use_graph, [o.Iterations for o in outs], [o.restarts for o in outs],
+                                   nkrylov=8, northo=northo).solve(src, sol)
+        print("fgcr net=%s graph=%s northo=%d: directions %s, cycles %s, fallback %s"
+              % (use_net, use_graph, northo, [o.Iterations for o in outs],
+                 [o.restarts for o in outs],
                  [o.inner_iterations for o in outs]))
         assert max(o.true_residual for o in outs) <= 1e-10
         for rhs in (0, 4, 11):
