@@ -34,22 +34,28 @@
 // Y mixes of k_bnet_third take L3 straight from the accumulators of the L3
 // products: the C rows a lane holds become its B operand, and the A columns
 // of the L3 part are ordered to match.
-// Transports run on v_dot2_f32_bf16: the link matrices are stored per site
-// and path as 9 words P_ab = (Re M_ab, -Im M_ab) of bf16, so
-// Re(M v) = sum_b dot2(P_ab, v_b) and, with S_ab = (Im M_ab, Re M_ab) formed
-// from P_ab in two instructions, Im(M v) = sum_b dot2(S_ab, v_b), for
-// v_b = (Re, Im) bf16.
+// The link matrices are stored per site and path as 9 words
+// P_ab = (Re M_ab, -Im M_ab) of bf16; S_ab = (Im M_ab, Re M_ab) is formed from
+// P_ab in two instructions.  The 8-channel transports of the middle stages
+// run on v_mfma_f32_4x4x4_16b_bf16, one 4x4 block per site: rows
+// (Re, Im) of the output colours with P_ab, S_ab as the A rows, columns the
+// lane's (channel, spin) vectors (compute_path); the one-channel transports of
+// the first and last stages run on v_dot2_f32_bf16 (Re(M v) = sum_b
+// dot2(P_ab, v_b), Im(M v) = sum_b dot2(S_ab, v_b), v_b = (Re, Im) bf16).
 //
-// Activations are bfloat16, one dword (re, im) per complex number, in
-// records [rhs][site][colour][channel][spin]: the 16 bytes of one channel
-// and colour are exactly one B operand (spins 0..3, re/im), and the four
-// channels one load instruction of a lane group reads are contiguous.  The input r and
-// the output z are FP32 half fields [rhs][12 = 3 spin + colour][stride].
+// Activations are e4m3 records with one power-of-two scale per site and
+// group of 8 channels, [rhs][site][group of 8][lane group t][colour]
+// [channel t, 4 + t][spin][re, im]: a lane group's 16 bytes of one colour
+// hold its two channels, dequantized
+// by v_cvt_scalef32_pk_bf16_fp8 straight into bf16 B operands.  The input r
+// is a bf16 copy [rhs][site][colour][spin] written by the solver's residual
+// update; the output z is an FP32 half field [rhs][12 = 3 spin + colour][stride].
 //
-// Order of work (ii_sweep.h): the waves of the blocks on one XCD share
-// that XCD's slab of tiles, taken as one front of consecutive (tile, rhs)
-// items; the rhs come in groups of rg (the autotuner's choice per stage)
-// whose waves share a tile's link records through L1.
+// Order of work (ii_sweep.h): the waves of the blocks on one XCD take that
+// XCD's slab of (tile, rhs) items in list order from a work-queue counter,
+// so the items in flight form one front of consecutive items; the rhs come
+// in groups of rg (the autotuner's choice per stage) whose items share a
+// tile's link records through L2.
 #include "ii_ops.h"
 #include "ii_stencil.h"
 #include "ii_sweep.h"
@@ -436,15 +442,62 @@ __device__ __forceinline__ void load_path(const BArgs& a, const ItemPos& P, cons
   for (int c = 0; c < 3; ++c) b.w[c] = ld128(in, 16u * c, vo);
   b.sc = __uint_as_float(ld32(insc, 0u, nb * (4u * H)));
 }
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ s16x4 as_s4(uint32_t lo, uint32_t hi) {
+  const U32x2 t{lo, hi};
+  return __builtin_bit_cast(s16x4, t);
+}
+// One path of a middle stage.  The path's record and link were loaded in the
+// transport layout: lane 4 n + t holds site n's lane group t (channels t and
+// 4 + t).  The colour transports run on v_mfma_f32_4x4x4_16b_bf16, one 4x4
+// block per site (block = lane / 4): the real 8x8 form of M (rows
+// (Re, Im) of out colours 0..2 and two zero rows, columns (Re, Im) of in
+// colours 0..2 and two zero columns) in 2 x 2 blocks, A row t of each block
+// held by lane t of the site (P_ab for even rows, S_ab for odd ones), each
+// lane's 8 vectors (channel, spin) as the B columns.  The bf16 results are
+// then moved to the channel-mixing layout (lane 16 t + n) by ds_bpermute
+// (src4: 4 x the source lane of this lane).
 __device__ __forceinline__ void compute_path(int dh, const PathBuf& b, const uint4* sA, int lane,
-                                             f32x4 acc[3][4]) {
+                                             int src4, f32x4 acc[3][4]) {
   U32x4 v[2][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) dq8(b.w[c], b.sc, v[0][c], v[1][c]);
+  const int t = lane & 3;
+  const bool odd = (t & 1) != 0, low = t < 2;
+  uint32_t r0[3], r2[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const uint32_t p0 = (t & 2) ? b.L[3 + i] : b.L[i];
+    r0[i] = odd ? s_of(p0) : p0;
+    const uint32_t p2 = b.L[6 + i];
+    r2[i] = low ? (odd ? s_of(p2) : p2) : 0u;
+  }
+  const s16x4 A00 = as_s4(r0[0], r0[1]), A01 = as_s4(r0[2], 0u);
+  const s16x4 A10 = as_s4(r2[0], r2[1]), A11 = as_s4(r2[2], 0u);
 #pragma unroll
   for (int kb = 0; kb < 2; ++kb) {
+    U32x4 wv[3];
+#pragma unroll
+    for (int sp = 0; sp < 4; ++sp) {
+      const s16x4 Blo = as_s4(v[kb][0][sp], v[kb][1][sp]), Bhi = as_s4(v[kb][2][sp], 0u);
+      f32x4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
+      d0 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(A00, Blo, d0, 0, 0, 0);
+      d0 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(A01, Bhi, d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(A10, Blo, d1, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(A11, Bhi, d1, 0, 0, 0);
+      wv[0][sp] = pack_b2(d0[0], d0[1]);
+      wv[1][sp] = pack_b2(d0[2], d0[3]);
+      wv[2][sp] = pack_b2(d1[0], d1[1]);
+    }
     bf16x8 B[3];
-    transport(b.L, v[kb], B);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      U32x4 m;
+#pragma unroll
+      for (int sp = 0; sp < 4; ++sp)
+        m[sp] = (uint32_t)__builtin_amdgcn_ds_bpermute(src4, (int)wv[c][sp]);
+      B[c] = as_frag(m);
+    }
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
       const bf16x8 A =
@@ -498,14 +551,14 @@ __device__ __forceinline__ void store_half8(char* o, float* osc, int s, int h, i
 // their first use).
 template <int H, typename Ident, typename Next, typename IdStep>
 __device__ __forceinline__ void item9(const BArgs& a, const ItemPos& P, const char* in,
-                                      const float* insc, int q, int lane, const uint4* sA,
-                                      PathBuf& X0, PathBuf& X1,
+                                      const float* insc, int q, int lane, int src4,
+                                      const uint4* sA, PathBuf& X0, PathBuf& X1,
                                       PathBuf& X2, f32x4 acc[3][4], Ident ident, Next next,
                                       IdStep idstep) {
 #define II_STEP(K, CUR, REQUEST)                  \
   REQUEST;                                        \
   __builtin_amdgcn_sched_barrier(0);              \
-  compute_path(K, CUR, sA, lane, acc);            \
+  compute_path(K, CUR, sA, lane, src4, acc);      \
   __builtin_amdgcn_sched_barrier(0);
   II_STEP(0, X0, load_path<H>(a, P, in, insc, 2, q, X2))
   II_STEP(1, X1, load_path<H>(a, P, in, insc, 3, q, X0))
@@ -619,20 +672,31 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
   const ItemQ Q = item_q(a, NW, wave, lane);
   int it = q_first(a, Q, lane);
   int itn = it >= 0 ? q_take(a, Q, q_request(Q, lane, it), it) : -1;
-  ItemPos P{};
-  if (it >= 0) P = item_pos(g, q_item(a, Q, it), n, 0);
+  // the paths' loads and transports run in the transport layout (lane 4 n' + qt)
+  const int qt = lane & 3, src4 = 4 * (4 * (lane & 15) + (lane >> 4));
+  ItemPos P{}, Pt{};
+  if (it >= 0) {
+    const NetItem w0 = q_item(a, Q, it);
+    P = item_pos(g, w0, n, 0);
+    Pt = item_pos(g, w0, lane >> 2, 0);
+  }
   auto inp = [&](const ItemPos& p) {
     return (const char*)a.in + (size_t)p.rhs * g.stride * (2 * kHalf8);
   };
   auto insc = [&](const ItemPos& p) { return a.in_sc + (size_t)p.rhs * g.stride * 2; };
   PathBuf X0, X1, X2;
   if (it >= 0) {
-    load_path<2>(a, P, inp(P), insc(P), 0, q, X0);
-    load_path<2>(a, P, inp(P), insc(P), 1, q, X1);
+    load_path<2>(a, Pt, inp(Pt), insc(Pt), 0, qt, X0);
+    load_path<2>(a, Pt, inp(Pt), insc(Pt), 1, qt, X1);
   }
   while (it >= 0) {
     const int tk = itn >= 0 ? q_request(Q, lane, itn) : 0;
-    const ItemPos P2 = itn >= 0 ? item_pos(g, q_item(a, Q, itn), n, 0) : P;
+    ItemPos P2 = P, Pt2 = Pt;
+    if (itn >= 0) {
+      const NetItem w2 = q_item(a, Q, itn);
+      P2 = item_pos(g, w2, n, 0);
+      Pt2 = item_pos(g, w2, lane >> 2, 0);
+    }
     f32x4 acc[3][MB];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -641,13 +705,13 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
     const uint32_t* r = (const uint32_t*)a.in_id + (size_t)P.rhs * 12 * g.stride;
     U32x4 rid[3];  // the identity input r (bf16 input layout)
     item9<2>(
-        a, P, inp(P), insc(P), q, lane, sA, X0, X1, X2, acc,
+        a, Pt, inp(Pt), insc(Pt), qt, lane, src4, sA, X0, X1, X2, acc,
         [&](PathBuf&) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) rid[c] = ld128<kIdCP>(r, 16u * c, (uint32_t)P.s * 48u);
         },
         [&](PathBuf& b, int path) {
-          if (itn >= 0) load_path<2>(a, P2, inp(P2), insc(P2), path, q, b);
+          if (itn >= 0) load_path<2>(a, Pt2, inp(Pt2), insc(Pt2), path, qt, b);
         },
         // identity: (W2_0 W1_0) r, r in channel 0 (lane group 0) of the B operand
         [&](const PathBuf&) {
@@ -669,6 +733,7 @@ __global__ void __launch_bounds__(kThreadsMid, 2) k_bnet_mid(BArgs a) {
     it = itn;
     itn = itnn;
     P = P2;
+    Pt = Pt2;
   }
   q_leave(Q, lane);
 }
@@ -693,20 +758,31 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
   const ItemQ Q = item_q(a, NW, wave, lane);
   int it = q_first(a, Q, lane);
   int itn = it >= 0 ? q_take(a, Q, q_request(Q, lane, it), it) : -1;
-  ItemPos P{};
-  if (it >= 0) P = item_pos(g, q_item(a, Q, it), n, 1);
+  // the paths' loads and transports run in the transport layout (lane 4 n' + qt)
+  const int qt = lane & 3, src4 = 4 * (4 * (lane & 15) + (lane >> 4));
+  ItemPos P{}, Pt{};
+  if (it >= 0) {
+    const NetItem w0 = q_item(a, Q, it);
+    P = item_pos(g, w0, n, 1);
+    Pt = item_pos(g, w0, lane >> 2, 1);
+  }
   auto inp = [&](const ItemPos& p) {
     return (const char*)a.in + (size_t)p.rhs * g.stride * kHalf8;
   };
   auto insc = [&](const ItemPos& p) { return a.in_sc + (size_t)p.rhs * g.stride; };
   PathBuf X0, X1, X2;
   if (it >= 0) {
-    load_path<1>(a, P, inp(P), insc(P), 0, q, X0);
-    load_path<1>(a, P, inp(P), insc(P), 1, q, X1);
+    load_path<1>(a, Pt, inp(Pt), insc(Pt), 0, qt, X0);
+    load_path<1>(a, Pt, inp(Pt), insc(Pt), 1, qt, X1);
   }
   while (it >= 0) {
     const int tk = itn >= 0 ? q_request(Q, lane, itn) : 0;
-    const ItemPos P2 = itn >= 0 ? item_pos(g, q_item(a, Q, itn), n, 1) : P;
+    ItemPos P2 = P, Pt2 = Pt;
+    if (itn >= 0) {
+      const NetItem w2 = q_item(a, Q, itn);
+      P2 = item_pos(g, w2, n, 1);
+      Pt2 = item_pos(g, w2, lane >> 2, 1);
+    }
     const char* rec = (const char*)a.in_id + (size_t)P.rhs * g.stride * (2 * kHalf8);
     const float* rsc = a.id_sc + (size_t)P.rhs * g.stride * 2;
     f32x4 acc[3][MB];
@@ -716,7 +792,7 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
       for (int mb = 0; mb < MB; ++mb) acc[c][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
     f32x4 y[3][MB];
     item9<1>(
-        a, P, inp(P), insc(P), q, lane, sA, X0, X1, X2, acc,
+        a, Pt, inp(Pt), insc(Pt), qt, lane, src4, sA, X0, X1, X2, acc,
         // the identity input: half 1 (L2(odd)) of the own odd record
         [&](PathBuf& b) {
           const uint32_t vo = r8_off<2>((uint32_t)P.s, 1, q);
@@ -725,7 +801,7 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
           b.sc = __uint_as_float(ld32<kIdCP>(rsc, 0u, ((uint32_t)P.s * 2u + 1u) * 4u));
         },
         [&](PathBuf& b, int path) {
-          if (itn >= 0) load_path<1>(a, P2, inp(P2), insc(P2), path, q, b);
+          if (itn >= 0) load_path<1>(a, Pt2, inp(Pt2), insc(Pt2), path, qt, b);
         },
         // identity W3_0 L2(odd), then the Y mixes: k-blocks 0, 1 from the L3
         // accumulators (blocks 2 kb, 2 kb + 1), k-blocks 2, 3 from L2(odd)
@@ -800,6 +876,7 @@ __global__ void __launch_bounds__(kThreadsThird, 2) k_bnet_third(BArgs a) {
     it = itn;
     itn = itnn;
     P = P2;
+    Pt = Pt2;
   }
   q_leave(Q, lane);
 }
