@@ -57,11 +57,15 @@ MESON_BYTES_PER_SITE = 12 * 24 * 8 + 2 * 18 * 8
 #         L2(even) 196 + z written 96
 NET_STAGE_BYTES = [(48 + 392, 288), (196 + 48 + 196, 288), (196 + 196 + 260, 288),
                    (288 + 196 + 96, 288)]
-# MFMA flops per output site and rhs (v_mfma_f32_16x16x32_bf16, 16384 flops,
-# 16 sites per instruction, x 3 colours): first 8 x 2, mid 8 x 4 x 2 + 4,
-# third 8 x 4 x 2 + 4 x 2 + 4 x 4 (the Y mixes), last 2 (identity; the
-# transports run on v_dot2_f32_bf16)
-NET_STAGE_MFMA_FLOPS = [48 * 1024, 204 * 1024, 264 * 1024, 6 * 1024]
+# MFMA flops per output site and rhs as issued (v_mfma_f32_16x16x32_bf16,
+# 16384 flops, 16 sites per instruction, x 3 colours): first 8 x 2, mid
+# 8 x 4 x 2 + 4, third 8 x 4 x 2 + 4 x 2 + 4 x 4 (the Y mixes), last 2
+# (identity; the first and last stages transport on v_dot2_f32_bf16); plus,
+# in the middle stages, the colour transports on v_mfma_f32_4x4x4_16b_bf16
+# (2048 flops, one site per 4x4 block, 16 per instruction): 8 paths x 32
+# instructions per 16 sites = 32 Ki per site (the 8 x 8 real form of a 3 x 3
+# complex matrix, 56% of it nonzero)
+NET_STAGE_MFMA_FLOPS = [48 * 1024, (204 + 32) * 1024, (264 + 32) * 1024, 6 * 1024]
 
 CONFIGS = {
     "c5": dict(dims=(32, 32, 32, 64), beta=6.0, kappa=0.1355, sweeps=200,
@@ -220,7 +224,7 @@ def main():
                     choices=["fgcr-learned", "fgcr", "bicgstab", "cg"],
                     help="timed solver: flexible GCR with the learned preconditioner (default), "
                          "the same without it, BiCGStab on Dhat or CG on Dhat^dag Dhat")
-    ap.add_argument("--nkrylov", type=int, default=8,
+    ap.add_argument("--nkrylov", type=int, default=10,
                     help="flexible GCR directions per restart cycle")
     ap.add_argument("--northo", type=int, default=0,
                     help="flexible GCR: directions each new one is orthogonalised against "

@@ -340,7 +340,7 @@ typedef struct {
                             restart_factor times the recursive one restarts the search
                             direction (0 = default 16) */
   int nkrylov;           /* flexible GCR (use_learned): directions per restart cycle
-                            (0 = default) */
+                            (0 = default 10) */
   int northo;            /* flexible GCR: each new direction is orthogonalised against
                             the last northo directions of its cycle (0 = default 1;
                             nkrylov or more: all of them, the full GCR) */

@@ -434,7 +434,7 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
   const Geom& g = ctx->g;
   const int nr = src->nrhs;
   const bool use_net = prm->use_learned != 0;
-  const int m = std::max(1, std::min(prm->nkrylov > 0 ? prm->nkrylov : 8, kMaxK));
+  const int m = std::max(1, std::min(prm->nkrylov > 0 ? prm->nkrylov : 10, kMaxK));
   // orthogonalisation window: the last nor directions of the cycle
   const int nor = prm->northo > 0 ? prm->northo : 1;
   const double tol = prm->tol > 0 ? prm->tol : 1e-10;
