@@ -275,6 +275,34 @@ def test_batched_apply_vs_oracle(dims, nrhs):
         ctx.close()
 
 
+def test_batched_apply_beyond_64_rhs():
+    """Batches of more than 64 rhs take the stages' static work order (the
+    work queues hold up to 64 live rhs): 66 rhs on 4^3x8 give, rhs by rhs,
+    bitwise the result of a batch of 3 (dynamic order) for the rhs they
+    share, and every rhs matches the device-form oracle."""
+    dims = (4, 4, 4, 8)
+    lat = orc.Lattice(dims)
+    u = lat.apbc(lat.gauge_expH(0.4, 78))
+    ctx = ii.Context(dims)
+    try:
+        ctx.upload_gauge(u)
+        net = ii.LearnedPreconditioner(ctx)
+        net.init_neumann(0.13, noise=0.3, seed=6)
+        ws = [net.get_weights(layer).astype(np.complex128) for layer in range(4)]
+        rng = np.random.default_rng(9)
+        b = rng.normal(size=(66, lat.half_size))
+        big = net.apply_batched(b)
+        small = net.apply_batched(b[[0, 40, 65]])
+        for k, j in enumerate((0, 40, 65)):
+            assert np.array_equal(big[j], small[k]), j
+        ref = orc.Net(lat, u, ws)
+        worst = max(rel(big[j], ref.apply_device_form(b[j])) for j in (0, 13, 64, 65))
+        print("66 rhs: max error against the device-form oracle %.2e" % worst)
+        assert worst < 1e-2
+    finally:
+        ctx.close()
+
+
 def _fgcr_setup(dims, kappa, seed=5):
     lat = orc.Lattice(dims)
     u = lat.apbc(lat.gauge_expH(0.6, seed))
