@@ -16,7 +16,10 @@
 // cycle: one pass for the dot products, one for the update; n >= the cycle
 // length is the full GCR, smaller n the truncated one, which keeps the
 // residual minimal along each new direction but not over the whole cycle,
-// and reads fewer vectors per direction).  The solution is never formed inside a cycle: with
+// and reads fewer vectors per direction; with n = 1 the dot pass also gives
+// |w'_k|^2 and <w'_k, r> from <w_{k-1}, w_k>, |w_k|^2, <w_k, r> and
+// <w_{k-1}, r>, so the orthogonalisation and the residual update share one
+// pass).  The solution is never formed inside a cycle: with
 // z'_k = z_k - sum_j beta_kj z'_j = sum_i T_ki z_i, the cycle's update is
 // sum_i gamma_i z_i, gamma_i = sum_k alpha_k T_ki, added to the FP64
 // solution in one pass at the end of the cycle.  Then the FP64 true
@@ -215,6 +218,108 @@ k_g_rupd(Geom g, const float2* __restrict__ W, size_t slot, int k, float2* __res
       float2 v = r[(size_t)c * g.stride];
       v.x -= ar * a.x - ai * a.y;
       v.y -= ar * a.y + ai * a.x;
+      r[(size_t)c * g.stride] = v;
+      nv[c] = v;
+      rr += (double)v.x * v.x + (double)v.y * v.y;
+    }
+    if (Rb) net_in_store(Rb, (size_t)rhs * g.stride + hs, nv);
+  }
+  const double s = block_sum(rr);
+  if (threadIdx.x == 0) part[(size_t)rhs * gridDim.x + blockIdx.x] = s;
+}
+
+// Window 1 (each direction orthogonalised against the previous one only),
+// k >= 1: one pass for every scalar the direction needs, one for the
+// update.  Partials of <w_{k-1}, w_k> (planes 0, 1), |w_k|^2 (2),
+// <w_k, r> (3, 4), <w_{k-1}, r> (5, 6).
+__global__ void __launch_bounds__(kGThreads)
+k_g_dots1(Geom g, const float2* __restrict__ W, size_t slot, int k, const float2* __restrict__ R,
+          const GScal* __restrict__ gs, double* __restrict__ part) {
+  const int rhs = blockIdx.y;
+  if (gs[rhs].idle) return;
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t he = (size_t)12 * g.stride;
+  double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (hs < g.Vh) {
+    const float2* wp = W + (k - 1) * slot + rhs * he + hs;
+    const float2* wk = W + k * slot + rhs * he + hs;
+    const float2* r = R + rhs * he + hs;
+    for (int c = 0; c < 12; ++c) {
+      const float2 a = wp[(size_t)c * g.stride], b = wk[(size_t)c * g.stride],
+                   v = r[(size_t)c * g.stride];
+      acc[0] += (double)a.x * b.x + (double)a.y * b.y;
+      acc[1] += (double)a.x * b.y - (double)a.y * b.x;
+      acc[2] += (double)b.x * b.x + (double)b.y * b.y;
+      acc[3] += (double)b.x * v.x + (double)b.y * v.y;
+      acc[4] += (double)b.x * v.y - (double)b.y * v.x;
+      acc[5] += (double)a.x * v.x + (double)a.y * v.y;
+      acc[6] += (double)a.x * v.y - (double)a.y * v.x;
+    }
+  }
+  const int nb = gridDim.x;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const double sj = block_sum(acc[j]);
+    if (threadIdx.x == 0) part[((size_t)rhs * 7 + j) * nb + blockIdx.x] = sj;
+  }
+}
+
+// beta = <w_{k-1}, w_k> / |w_{k-1}|^2; for w' = w_k - beta w_{k-1}:
+// |w'|^2 = |w_k|^2 - |<w_{k-1}, w_k>|^2 / |w_{k-1}|^2 and
+// <w', r> = <w_k, r> - conj(beta) <w_{k-1}, r> (the sums are FP64 over exact
+// products of the stored FP32 values); alpha = <w', r> / |w'|^2
+__global__ void k_g_ab1(const double* __restrict__ part, int nb, int k, GScal* __restrict__ gs) {
+  GScal& S = gs[blockIdx.x];
+  if (S.idle) return;
+  double v[7];
+  for (int j = 0; j < 7; ++j) v[j] = gsum(part, 7, j, nb);
+  if (threadIdx.x == 0) {
+    const double wp = S.w2[k - 1];
+    const double br = v[0] / wp, bi = v[1] / wp;
+    const double ww = v[2] - (v[0] * v[0] + v[1] * v[1]) / wp;
+    const double wr = v[3] - (br * v[5] + bi * v[6]);
+    const double wi = v[4] - (br * v[6] - bi * v[5]);
+    S.beta[k][k - 1][0] = br;
+    S.beta[k][k - 1][1] = bi;
+    S.w2[k] = ww;
+    S.alpha[k][0] = wr / ww;
+    S.alpha[k][1] = wi / ww;
+    if (!(ww > 0.0) || !gfin(br) || !gfin(bi) || !gfin(S.alpha[k][0]) || !gfin(S.alpha[k][1])) {
+      S.nan = 1;
+      S.idle = 1;
+      S.done = 1;
+    }
+  }
+}
+
+// w_k -= beta w_{k-1}, r -= alpha w_k (the new w_k); partials |r|^2; with Rb,
+// r also in the network's bf16 input layout
+__global__ void __launch_bounds__(kGThreads)
+k_g_upd1(Geom g, float2* __restrict__ W, size_t slot, int k, float2* __restrict__ R,
+         uint32_t* __restrict__ Rb, const GScal* __restrict__ gs, double* __restrict__ part) {
+  const int rhs = blockIdx.y;
+  const GScal& S = gs[rhs];
+  if (S.idle) return;
+  const float br = (float)S.beta[k][k - 1][0], bi = (float)S.beta[k][k - 1][1];
+  const float ar = (float)S.alpha[k][0], ai = (float)S.alpha[k][1];
+  const int hs = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t he = (size_t)12 * g.stride;
+  double rr = 0.0;
+  if (hs < g.Vh) {
+    const float2* wp = W + (k - 1) * slot + rhs * he + hs;
+    float2* wk = W + k * slot + rhs * he + hs;
+    float2* r = R + rhs * he + hs;
+    float2 nv[12];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      const float2 a = wp[(size_t)c * g.stride];
+      float2 w = wk[(size_t)c * g.stride];
+      w.x -= br * a.x - bi * a.y;
+      w.y -= br * a.y + bi * a.x;
+      wk[(size_t)c * g.stride] = w;
+      float2 v = r[(size_t)c * g.stride];
+      v.x -= ar * w.x - ai * w.y;
+      v.y -= ar * w.y + ai * w.x;
       r[(size_t)c * g.stride] = v;
       nv[c] = v;
       rr += (double)v.x * v.x + (double)v.y * v.y;
@@ -540,16 +645,25 @@ ii_status fgcr_batched(ii_context* ctx, void** state, double kappa, const ii_pro
       a.xpay = true;
       II_CHECK(hop32(ctx, a, T, wk, zk, nullptr));
       const int j0 = std::max(0, k - nor);
-      if (k > 0) {
-        hipLaunchKernelGGL(k_g_dots, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, j0, k, gs,
-                           part);
-        hipLaunchKernelGGL(k_g_beta, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, j0, k, gs);
+      if (nor == 1 && k > 0) {
+        // window 1: all scalars from one pass, then one update pass
+        hipLaunchKernelGGL(k_g_dots1, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, R,
+                           gs, part);
+        hipLaunchKernelGGL(k_g_ab1, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, k, gs);
+        hipLaunchKernelGGL(k_g_upd1, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, R, Rb,
+                           gs, part);
+      } else {
+        if (k > 0) {
+          hipLaunchKernelGGL(k_g_dots, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, j0, k,
+                             gs, part);
+          hipLaunchKernelGGL(k_g_beta, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, j0, k, gs);
+        }
+        hipLaunchKernelGGL(k_g_orth, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, j0, k, R,
+                           gs, part);
+        hipLaunchKernelGGL(k_g_alpha, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, k, gs);
+        hipLaunchKernelGGL(k_g_rupd, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, R, Rb,
+                           gs, part);
       }
-      hipLaunchKernelGGL(k_g_orth, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, j0, k, R,
-                         gs, part);
-      hipLaunchKernelGGL(k_g_alpha, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, k, gs);
-      hipLaunchKernelGGL(k_g_rupd, sgrid, dim3(kGThreads), 0, ctx->stream, g, W, slot, k, R, Rb, gs,
-                         part);
       hipLaunchKernelGGL(k_g_rr, dim3(nr), dim3(256), 0, ctx->stream, part, nbs, gs);
       II_HIP(ctx, hipGetLastError());
     }
