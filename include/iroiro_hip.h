@@ -331,16 +331,16 @@ typedef struct {
                             counterpart; sloppy precisions, batched);
                             II_METHOD_FGCR: flexible GCR on Dhat (no reference counterpart;
                             batched only): every rhs in each launch, FP32 Krylov vectors
-                            and FP32 Dhat, the network (use_learned) on bfloat16
-                            activations, restarted every nkrylov directions from the
+                            and FP32 Dhat, the network (use_learned) on e4m3
+                            activations with bfloat16 matrix-core arithmetic,
+                            restarted every nkrylov directions from the
                             FP64 true residual; use_graph captures a restart cycle as
                             one HIP graph; batched mixed CG on the remaining defect if
                             the flexible iteration stalls */
   double restart_factor; /* a reliable update whose recomputed residual |r|^2 exceeds
                             restart_factor times the recursive one restarts the search
                             direction (0 = default 16) */
-  int nkrylov;           /* flexible GCR (use_learned): directions per restart cycle
-                            (0 = default 10) */
+  int nkrylov;           /* flexible GCR: directions per restart cycle (0 = default 10) */
   int northo;            /* flexible GCR: each new direction is orthogonalised against
                             the last northo directions of its cycle (0 = default 1;
                             nkrylov or more: all of them, the full GCR) */
