@@ -7,7 +7,8 @@ propagator (delta at the origin, one per spin x colour).  One step = one
 12-rhs propagator solve to the FP64 true residual |b - D x|/|b| <= 1e-10 per
 rhs with mixed precision + batching + the learned preconditioner: the batched
 flexible GCR on the even/odd Schur operator with the gauge-covariant network
-(trained on this gauge field, bf16 activations and links, MFMA channel mixing)
+(trained on this gauge field; e4m3 activations, bf16 links and matrix-core
+arithmetic for the channel mixing and the 8-channel colour transports)
 as preconditioner, FP32 Krylov vectors, FP64 true-residual restarts, each
 restart cycle replayed as a captured HIP graph; sources already resident in
 HBM.  The network's training (64 plain-CG pairs) is timed apart.
@@ -514,8 +515,9 @@ def main():
 
     method_desc = {
         "fgcr-learned": "batched 12-rhs flexible GCR on Dhat with the trained gauge-covariant "
-                        "network as preconditioner (bf16 activations and links, MFMA channel "
-                        "mixing), FP32 Krylov vectors and FP32 Dhat, each direction "
+                        "network as preconditioner (e4m3 activations, bf16 links, MFMA channel "
+                        "mixing and colour transports), FP32 Krylov vectors and FP32 Dhat, "
+                        "each direction "
                         "orthogonalised against the previous %d, restart cycles of %d "
                         "directions captured as HIP graphs, FP64 true-residual restarts"
                         % (args.northo if args.northo > 0 else 1, args.nkrylov),
