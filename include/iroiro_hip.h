@@ -486,6 +486,14 @@ ii_status ii_bench_hop(ii_context* ctx, int precision, int nrhs, int nrep, doubl
  * batch's first multi-rhs hop of that precision); reports the timings and
  * the choices as text. */
 ii_status ii_autotune(ii_context* ctx, char* report, size_t len);
+
+/* The autotuner's choices of this context (the single-rhs hop order, the multi-rhs hop shape
+ * of each tuned precision and batch size, the network stages' shapes) to a text file, and
+ * back into a context on the same lattice, so that a later run skips the timing.  No
+ * counterpart in the reference (it has no autotuner).  Loading a file written for another
+ * lattice returns II_ERR_BAD_ARG. */
+ii_status ii_tune_save(ii_context* ctx, const char* path);
+ii_status ii_tune_load(ii_context* ctx, const char* path);
 /* Times Dirac_Wilson_EvenOdd::md_force on device-resident fields (no host
  * copies): total_ms = the two hops forming Doe eta and Deo^dag zeta plus the
  * force kernel; force_ms = the force kernel alone (both parities).  HIP

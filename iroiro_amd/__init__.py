@@ -152,6 +152,14 @@ class Context:
         self.check(_lib.load().ii_autotune(self._h, buf, 8192), "ii_autotune")
         return buf.value.decode()
 
+    def tune_save(self, path):
+        """Writes the autotuner's choices of this context to a text file."""
+        self.check(_lib.load().ii_tune_save(self._h, os.fsencode(path)), "ii_tune_save")
+
+    def tune_load(self, path):
+        """Loads autotuner choices saved on the same lattice (no timing runs)."""
+        self.check(_lib.load().ii_tune_load(self._h, os.fsencode(path)), "ii_tune_load")
+
     def bench_copy(self, nbytes, nrep=20):
         """Average ms of a plain device copy of nbytes (read + write)."""
         ms = ctypes.c_double()

@@ -36,7 +36,7 @@ EXPORTS = [
     "ii_net_layer_wgrad", "ii_net_loss_grad", "ii_net_apply", "ii_net_apply_batched",
     "ii_net_train", "ii_net_save",
     "ii_net_load", "ii_net_bench", "ii_bench_net_batched", "ii_net_tune", "ii_fgcr_solve",
-    "ii_autotune", "ii_bench_hop", "ii_bench_copy",
+    "ii_autotune", "ii_tune_save", "ii_tune_load", "ii_bench_hop", "ii_bench_copy",
     "ii_dirac_md_force", "ii_dirac_full_md_force", "ii_dirac_full_md_force_pm",
     "ii_traceless_antihermite", "ii_action_nf2_md_force", "ii_bench_md_force",
     "ii_rational_approx", "ii_rational_solve", "ii_action_nf_md_force",
@@ -184,6 +184,8 @@ def load():
         "ii_net_tune": [vp, ci, ip, ip, ctypes.c_char_p, ctypes.c_size_t],
         "ii_fgcr_solve": [vp, cd, dp, dp, cd, ci, ci, ci, so],
         "ii_autotune": [vp, ctypes.c_char_p, ctypes.c_size_t],
+        "ii_tune_save": [vp, ctypes.c_char_p],
+        "ii_tune_load": [vp, ctypes.c_char_p],
         "ii_bench_hop": [vp, ci, ci, ci, dp],
         "ii_bench_copy": [vp, ctypes.c_size_t, ci, dp],
     }

@@ -1458,6 +1458,30 @@ ii_status bnet_apply(ii_context* ctx, int nrhs, const uint32_t* r, float2* z, co
   return II_OK;
 }
 
+void bnet_shapes_get(ii_context* ctx, int* tuned_nrhs, int rg[4], int by[4], int bz[4]) {
+  const BNet* B = bnet_of(ctx);
+  *tuned_nrhs = B ? B->tuned_nrhs : 0;
+  for (int st = 0; st < 4; ++st) {
+    rg[st] = B ? B->rg[st] : 0;
+    by[st] = B ? B->by[st] : 1;
+    bz[st] = B ? B->bz[st] : 0;
+  }
+}
+
+ii_status bnet_shapes_set(ii_context* ctx, int tuned_nrhs, const int rg[4], const int by[4],
+                          const int bz[4]) {
+  BNet* B = nullptr;
+  II_CHECK(bnet_get(ctx, &B));
+  for (int st = 0; st < 4; ++st) {
+    B->rg[st] = rg[st];
+    B->by[st] = by[st];
+    B->bz[st] = bz[st];
+  }
+  B->tuned_nrhs = tuned_nrhs;
+  B->report = "network shapes loaded from a tuning file";
+  return II_OK;
+}
+
 void bnet_release(ii_context* ctx) {
   BNet* B = bnet_of(ctx);
   if (!B) return;

@@ -188,6 +188,10 @@ ii_status bnet_apply(ii_context* ctx, int nrhs, const uint32_t* r, float2* z,
 // builds what bnet_apply needs (links, weights, buffers) so that its
 // launches can be captured in a graph
 ii_status bnet_prepare(ii_context* ctx, int nrhs);
+// the network stages' tuned launch shapes for tuned_nrhs rhs (0: not tuned)
+void bnet_shapes_get(ii_context* ctx, int* tuned_nrhs, int rg[4], int by[4], int bz[4]);
+ii_status bnet_shapes_set(ii_context* ctx, int tuned_nrhs, const int rg[4], const int by[4],
+                          const int bz[4]);
 void bnet_release(ii_context* ctx);
 // batched flexible GCR with (use_learned) the network as preconditioner
 // (ii_fgcr.hip); *state holds its workspaces, released by fgcr_release

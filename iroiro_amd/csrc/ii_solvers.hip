@@ -18,6 +18,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <string>
 #include <vector>
 
 namespace ii {
@@ -2689,6 +2691,88 @@ ii_status ii_autotune(ii_context* ctx, char* report, size_t len) {
   }
   if (report && len) std::snprintf(report, len, "%s", rep.c_str());
   return II_OK;
+}
+
+// Tuning file: one line per choice, for one lattice.
+//   iroiro-hip tuning 1
+//   lattice Lx Ly Lz Lt
+//   hop mode W bpp bs                      (the single-rhs FP64 hop's order)
+//   mrhs P nrhs rb mode W bpp bs           (each tuned multi-rhs precision P)
+//   net nrhs rg0..rg3 by0..by3 bz0..bz3    (the network stages)
+ii_status ii_tune_save(ii_context* ctx, const char* path) {
+  if (!ctx || !path) return II_ERR_BAD_ARG;
+  const Geom& g = ctx->g;
+  std::FILE* f = std::fopen(path, "w");
+  if (!f) return set_error(ctx, II_ERR_BAD_ARG, std::string("cannot create tuning file ") + path);
+  std::fprintf(f, "iroiro-hip tuning 1\nlattice %d %d %d %d\n", g.L[0], g.L[1], g.L[2], g.L[3]);
+  if (ctx->hop_tuned) {
+    const HopOrder& o = ctx->hop_order;
+    std::fprintf(f, "hop %d %d %d %d\n", o.mode, o.W, o.bpp, o.bs);
+  }
+  for (int P = 0; P < 4; ++P)
+    if (ctx->mrhs_nrhs[P] > 0) {
+      const HopOrder& o = ctx->mrhs_ord[P];
+      std::fprintf(f, "mrhs %d %d %d %d %d %d %d\n", P, ctx->mrhs_nrhs[P], ctx->mrhs_rb[P], o.mode,
+                   o.W, o.bpp, o.bs);
+    }
+  int nn = 0, rg[4], by[4], bz[4];
+  bnet_shapes_get(ctx, &nn, rg, by, bz);
+  if (nn > 0)
+    std::fprintf(f, "net %d %d %d %d %d %d %d %d %d %d %d %d %d\n", nn, rg[0], rg[1], rg[2],
+                 rg[3], by[0], by[1], by[2], by[3], bz[0], bz[1], bz[2], bz[3]);
+  const bool bad = std::ferror(f) != 0;
+  if (std::fclose(f) != 0 || bad)
+    return set_error(ctx, II_ERR_BAD_ARG, std::string("cannot write tuning file ") + path);
+  return II_OK;
+}
+
+ii_status ii_tune_load(ii_context* ctx, const char* path) {
+  if (!ctx || !path) return II_ERR_BAD_ARG;
+  std::FILE* f = std::fopen(path, "r");
+  if (!f) return set_error(ctx, II_ERR_BAD_ARG, std::string("cannot open tuning file ") + path);
+  const Geom& g = ctx->g;
+  char line[256];
+  int ver = 0, L[4] = {0, 0, 0, 0};
+  bool ok = std::fgets(line, sizeof line, f) && std::sscanf(line, "iroiro-hip tuning %d", &ver) == 1 &&
+            ver == 1 && std::fgets(line, sizeof line, f) &&
+            std::sscanf(line, "lattice %d %d %d %d", &L[0], &L[1], &L[2], &L[3]) == 4;
+  if (!ok) {
+    std::fclose(f);
+    return set_error(ctx, II_ERR_BAD_ARG, std::string("not a tuning file: ") + path);
+  }
+  if (L[0] != g.L[0] || L[1] != g.L[1] || L[2] != g.L[2] || L[3] != g.L[3]) {
+    std::fclose(f);
+    return set_error(ctx, II_ERR_BAD_ARG, "tuning file is for another lattice");
+  }
+  ii_status st = II_OK;
+  while (st == II_OK && std::fgets(line, sizeof line, f)) {
+    HopOrder o;
+    int P = 0, n = 0, rb = 0;
+    int v[13];
+    if (std::sscanf(line, "hop %d %d %d %d", &o.mode, &o.W, &o.bpp, &o.bs) == 4) {
+      ctx->hop_order = o;
+      ctx->hop_tuned = true;
+      ctx->tune_report = "hop order loaded from a tuning file";
+    } else if (std::sscanf(line, "mrhs %d %d %d %d %d %d %d", &P, &n, &rb, &o.mode, &o.W, &o.bpp,
+                           &o.bs) == 7) {
+      if (P < 0 || P > 3 || n < 1 || rb < 1) {
+        st = set_error(ctx, II_ERR_BAD_ARG, "bad mrhs line in tuning file");
+        break;
+      }
+      ctx->mrhs_nrhs[P] = n;
+      ctx->mrhs_rb[P] = rb;
+      ctx->mrhs_ord[P] = o;
+      ctx->mrhs_report[P] = "multi-rhs hop shape loaded from a tuning file";
+    } else if (std::sscanf(line, "net %d %d %d %d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2],
+                           &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11],
+                           &v[12]) == 13) {
+      st = bnet_shapes_set(ctx, v[0], v + 1, v + 5, v + 9);
+    } else {
+      st = set_error(ctx, II_ERR_BAD_ARG, std::string("unknown line in tuning file: ") + line);
+    }
+  }
+  std::fclose(f);
+  return st;
 }
 
 ii_status ii_bench_copy(ii_context* ctx, size_t bytes, int nrep, double* avg_ms) {
